@@ -1,0 +1,161 @@
+#!/usr/bin/env python3
+"""Benchmark of the Smith-Waterman hot path on one MI355X.
+
+Headline workload: 1M long-tail extension jobs (reads of 100/150/250 bp, query lengths 70% < 30,
+25% 30-150, 5% 150-250, target = query + 100, band 100, h0 U[19,250]), run through the batched
+form with the batch resident in HBM.  A step is one pass of the extension kernels over the whole
+batch.  Alongside it the same batch is timed in the plain form, the 200k-job mate-rescue workload
+is timed in both forms, and the CPU oracle is timed on the host cores.
+
+Prints one JSON line.
+"""
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "bwa-mem4_amd"))
+
+import numpy as np  # noqa: E402
+
+import bwg  # noqa: E402
+
+METRIC = "extension GCUPS + jobs/s, batched vs plain GPU (bit-exact); p50/p99 chunk latency"
+K_EXT = 16   # integer ops per extension cell (crates/bwa-extend/src/sw.rs:377-412)
+K_RESC = 13  # integer ops per rescue cell (crates/bwa-extend/src/sw.rs:993-1029)
+VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s on 256 CUs at 2.4 GHz
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def time_steps(ctx, staged, steps, warmup):
+    for _ in range(warmup):
+        staged.run()
+    kern_ns, main_ns, launches = [], [], []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        staged.run()
+        kern_ns.append(ctx.stat(bwg.STAT_LAST_KERNEL_NS))
+        main_ns.append(ctx.stat(bwg.STAT_LAST_MAIN_NS))
+        launches.append(ctx.stat(bwg.STAT_LAST_MAIN_LAUNCHES))
+    wall = time.perf_counter() - t0
+    return wall / steps, float(np.mean(kern_ns)) * 1e-9, float(np.mean(main_ns)) * 1e-9, float(np.mean(launches))
+
+
+def cpu_baseline(ext_batch, mat, n_threads):
+    """The oracle on the host cores over the whole extension batch (bounded: a few seconds on 16
+    threads).  Also returns the workload's evaluated-cell count, the GCUPS denominator."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as ob
+    t0 = time.perf_counter()
+    out, cells = ob.extend_batch(ext_batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=n_threads, want_cells=True)
+    dt = time.perf_counter() - t0
+    return out, cells, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--jobs", type=int, default=1_000_000)
+    ap.add_argument("--rescue-jobs", type=int, default=200_000)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-check", action="store_true")
+    ap.add_argument("--cpu-threads", type=int, default=16)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this path has no multi-GPU step (jobs shard across GPUs "
+              "with no exchange); run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    mat = bwg.default_mat()
+    ext_p = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+    ksw_p = bwg.ksw_params(5, mat, 6, 1, 6, 1, 20, 1)
+    log("generating %d long-tail extension jobs and %d rescue jobs" % (args.jobs, args.rescue_jobs))
+    ext = bwg.generate(bwg.WORKLOAD_EXT_TAIL, args.jobs, seed=42)
+    resc = bwg.generate(bwg.WORKLOAD_RESCUE, args.rescue_jobs, seed=42)
+
+    ctx = bwg.Context(0)
+    res = {}
+    outs = {}
+    for form, fname in ((bwg.FORM_BATCHED, "batched"), (bwg.FORM_PLAIN, "plain")):
+        try:
+            ctx.set_form(form)
+            log("extension %s: staging" % fname)
+            st = ctx.stage(ext, ext_p)
+            st.run()  # first run builds the form's plan and buffers
+            per, kern, main, nl = time_steps(ctx, st, args.steps if form == bwg.FORM_BATCHED else max(1, args.steps // 3), args.warmup)
+            outs["ext_" + fname] = st.results()
+            st.close()
+            res["ext_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
+            log("extension %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
+            log("rescue %s: staging" % fname)
+            st = ctx.stage(resc, ksw_p)
+            st.run()
+            per, kern, main, nl = time_steps(ctx, st, max(1, args.steps // 2), 1)
+            outs["resc_" + fname] = st.results()
+            st.close()
+            res["resc_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
+            log("rescue %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
+        except bwg.BwgError as e:
+            log("form %s unavailable: %s" % (fname, e))
+    head = "ext_batched" if "ext_batched" in res else "ext_plain"
+    if "ext_batched" in outs and "ext_plain" in outs:
+        assert np.array_equal(outs["ext_batched"], outs["ext_plain"]), "forms disagree"
+
+    cpu = None
+    cells = None
+    if not args.no_cpu:
+        log("cpu oracle over the extension batch on %d threads" % args.cpu_threads)
+        cpu_out, cells, cpu_dt = cpu_baseline(ext, mat, args.cpu_threads)
+        if not args.no_check:
+            ok = np.array_equal(cpu_out, outs[head])
+            log("bit-exact vs oracle: %s" % ok)
+            if not ok:
+                bad = np.nonzero(cpu_out != outs[head])[0]
+                log("mismatches: %d, first %d" % (bad.size, bad[0]))
+                sys.exit(1)
+        cpu = {"value": args.jobs / cpu_dt, "unit": "jobs/s", "cores": args.cpu_threads,
+               "kind": "port", "gcups": cells / cpu_dt / 1e9,
+               "sample": "all %d long-tail extension jobs, scalar C oracle, static partition" % args.jobs}
+
+    h = res[head]
+    jobs_s = args.jobs / h["step_s"]
+    line = {
+        "metric": METRIC, "value": jobs_s, "unit": "jobs/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": h["step_s"] * 1e3, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "int32", "data": "synthetic",
+        "config": {"workload": "batched uneven extension: %d ksw_extend2 jobs, reads 100/150/250bp, "
+                               "qlen 70%%<30/25%% 30-150/5%% 150-250, tlen=qlen+100, w=100, zdrop=100, "
+                               "end_bonus=5, h0 U[19,250], seed 42" % args.jobs,
+                   "form": head.split("_")[1]},
+    }
+    if cells:
+        line["gcups"] = cells / h["step_s"] / 1e9
+        line["gcups_kernel"] = cells / h["kernel_s"] / 1e9
+        line["cells"] = cells
+        ops = cells * K_EXT / h["main_s"] / 1e12
+        line["roofline"] = {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS,
+                            "unit": "Tops/s", "frac": ops / VALU_PEAK_TOPS, "traffic": None}
+    if "ext_plain" in res:
+        line["plain_jobs_per_s"] = args.jobs / res["ext_plain"]["step_s"]
+        line["speedup_vs_plain"] = jobs_s / line["plain_jobs_per_s"]
+    for f in ("batched", "plain"):
+        if "resc_" + f in res:
+            r = res["resc_" + f]
+            line["rescue_%s_jobs_per_s" % f] = args.rescue_jobs / r["step_s"]
+            line["rescue_%s_gcell_s" % f] = float(np.sum(resc.q_len.astype(np.int64) * resc.t_len)) / r["step_s"] / 1e9
+    if cpu:
+        line["cpu_baseline"] = cpu
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

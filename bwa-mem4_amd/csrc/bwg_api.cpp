@@ -1,0 +1,244 @@
+// Host runtime of libbwg.so: the C ABI declared in include/bwg.h.
+//
+// A context owns one HIP stream on one device.  A batch is staged into device memory (arena,
+// job descriptors, result array, scratch), run, and its results copied back; the synchronous
+// entry points bwg_extend_batch / bwg_ksw_align2_batch are exactly that sequence.  Every
+// precondition the reference kernels rely on is checked on the host before anything reaches the
+// device, and any HIP failure is returned as BWG_EIO: there is no CPU fallback anywhere.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <atomic>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "bwg_internal.h"
+#include "bwg_runtime.h"
+
+using namespace bwg;
+
+namespace {
+
+const char *kName = "hip-gfx950";
+
+int check_mat(int32_t m, const int8_t *mat) {
+    (void)mat;
+    return m == 5 ? BWG_OK : BWG_EINVAL;
+}
+
+int32_t mat_max(const int8_t *mat) {
+    int32_t mx = mat[0];
+    for (int k = 1; k < 25; ++k) mx = std::max<int32_t>(mx, mat[k]);
+    return mx;
+}
+
+bool codes_ok(const uint8_t *p, size_t len) {
+    uint8_t acc = 0;
+    for (size_t k = 0; k < len; ++k) acc |= (uint8_t)(p[k] > 4);
+    return acc == 0;
+}
+
+// Shared validation of the arena description.
+int check_jobs(size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
+               const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len) {
+    if (n == 0) return BWG_OK;
+    if (!arena || !q_off || !q_len || !t_off || !t_len) return BWG_EINVAL;
+    for (size_t k = 0; k < n; ++k) {
+        if (q_off[k] > arena_len || q_len[k] > arena_len - q_off[k]) return BWG_EINVAL;
+        if (t_off[k] > arena_len || t_len[k] > arena_len - t_off[k]) return BWG_EINVAL;
+        if (q_len[k] > kMaxQlen || t_len[k] > kMaxTlen) return BWG_E2BIG;
+        if (!codes_ok(arena + q_off[k], q_len[k]) || !codes_ok(arena + t_off[k], t_len[k]))
+            return BWG_EINVAL;
+    }
+    return BWG_OK;
+}
+
+}  // namespace
+
+int bwg_map_hip(hipError_t e) {
+    if (e == hipSuccess) return BWG_OK;
+    if (e == hipErrorOutOfMemory) return BWG_ENOMEM;
+    return BWG_EIO;
+}
+
+extern "C" {
+
+const char *bwg_strerror(int code) {
+    switch (code) {
+    case BWG_OK: return "ok";
+    case BWG_EIO: return "device error";
+    case BWG_E2BIG: return "job exceeds a supported size";
+    case BWG_EAGAIN: return "HBM budget cannot admit the chunk now";
+    case BWG_ENOMEM: return "out of memory";
+    case BWG_ENODEV: return "no usable gfx950 device";
+    case BWG_EINVAL: return "invalid argument (a kernel precondition is violated)";
+    default: return "unknown error";
+    }
+}
+
+int bwg_create(int device, bwg_ctx **out) {
+    if (!out) return BWG_EINVAL;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || device < 0 || device >= ndev) return BWG_ENODEV;
+    hipDeviceProp_t prop;
+    if (hipGetDeviceProperties(&prop, device) != hipSuccess) return BWG_ENODEV;
+    if (std::strncmp(prop.gcnArchName, "gfx950", 6) != 0) return BWG_ENODEV;
+    bwg_ctx *c = new (std::nothrow) bwg_ctx();
+    if (!c) return BWG_ENOMEM;
+    c->device = device;
+    c->num_cus = prop.multiProcessorCount;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        return BWG_EIO;
+    }
+    for (auto &e : c->ev)
+        if (hipEventCreate(&e) != hipSuccess) {
+            bwg_destroy(c);
+            return BWG_EIO;
+        }
+    *out = c;
+    return BWG_OK;
+}
+
+void bwg_destroy(bwg_ctx *c) {
+    if (!c) return;
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->pool.release_all();
+    for (auto &e : c->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (c->stream) (void)hipStreamDestroy(c->stream);
+    delete c;
+}
+
+const char *bwg_name(const bwg_ctx *) { return kName; }
+
+int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
+    if (!c) return BWG_EINVAL;
+    switch (key) {
+    case BWG_OPT_FORM:
+        if (v != BWG_FORM_PLAIN && v != BWG_FORM_BATCHED) return BWG_EINVAL;
+        c->form = (int)v;
+        return BWG_OK;
+    default: return BWG_EINVAL;
+    }
+}
+
+int64_t bwg_get_stat(const bwg_ctx *c, int key) {
+    if (!c) return -1;
+    switch (key) {
+    case BWG_OPT_FORM: return c->form;
+    case BWG_STAT_LAUNCHES: return c->launches.load();
+    case BWG_STAT_JOBS: return c->jobs.load();
+    case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;
+    case BWG_STAT_LAST_MAIN_NS: return c->last_main_ns;
+    case BWG_STAT_LAST_MAIN_LAUNCHES: return c->last_main_launches;
+    default: return -1;
+    }
+}
+
+int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                        bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_mat(p->m, p->mat);
+    if (rc) return rc;
+    if (p->e_del < 1 || p->e_ins < 1) return BWG_EINVAL;
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    if (rc) return rc;
+    for (size_t k = 0; k < n; ++k)
+        if (h0[k] <= 0 || q_len[k] < 1) return BWG_EINVAL;
+    ExtScoring sc{};
+    std::memcpy(sc.mat, p->mat, 25);
+    sc.o_del = p->o_del;
+    sc.e_del = p->e_del;
+    sc.o_ins = p->o_ins;
+    sc.e_ins = p->e_ins;
+    sc.w = p->w;
+    sc.end_bonus = p->end_bonus;
+    sc.zdrop = p->zdrop;
+    sc.max_sc = mat_max(p->mat);
+    return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, out);
+}
+
+int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
+                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_mat(p->m, p->mat);
+    if (rc) return rc;
+    if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    if (rc) return rc;
+    KswScoring sc{};
+    std::memcpy(sc.mat, p->mat, 25);
+    sc.o_del = p->o_del;
+    sc.e_del = p->e_del;
+    sc.o_ins = p->o_ins;
+    sc.e_ins = p->e_ins;
+    sc.minsc = p->minsc;
+    sc.max_sc = p->max_sc;
+    return stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, out);
+}
+
+int bwg_batch_run(bwg_ctx *c, bwg_batch *b) {
+    if (!c || !b) return BWG_EINVAL;
+    return run_batch(c, b);
+}
+
+int bwg_batch_results(bwg_ctx *c, bwg_batch *b, void *out) {
+    if (!c || !b || (!out && b->n)) return BWG_EINVAL;
+    return fetch_results(c, b, out);
+}
+
+void bwg_batch_destroy(bwg_ctx *c, bwg_batch *b) {
+    if (!c || !b) return;
+    destroy_batch(c, b);
+}
+
+int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                     size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                     const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                     bwg_ext_result *out) {
+    if (n == 0) return (c && p) ? BWG_OK : BWG_EINVAL;
+    bwg_batch *b = nullptr;
+    int rc = bwg_batch_stage_ext(c, p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, &b);
+    if (rc) return rc;
+    rc = bwg_batch_run(c, b);
+    if (!rc) rc = bwg_batch_results(c, b, out);
+    bwg_batch_destroy(c, b);
+    return rc;
+}
+
+int bwg_extend(bwg_ctx *c, const bwg_ext_params *p, const uint8_t *query, uint32_t qlen,
+               const uint8_t *target, uint32_t tlen, int32_t h0, bwg_ext_result *out) {
+    if (!c || !p || !out || (!query && qlen) || (!target && tlen)) return BWG_EINVAL;
+    std::vector<uint8_t> arena((size_t)qlen + tlen);
+    if (qlen) std::memcpy(arena.data(), query, qlen);
+    if (tlen) std::memcpy(arena.data() + qlen, target, tlen);
+    uint64_t qo = 0, to = qlen;
+    return bwg_extend_batch(c, p, 1, arena.data(), arena.size(), &qo, &qlen, &to, &tlen, &h0, out);
+}
+
+int bwg_ksw_align2_batch(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
+                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                         const uint64_t *t_off, const uint32_t *t_len, bwg_ksw_result *out) {
+    if (n == 0) return (c && p) ? BWG_OK : BWG_EINVAL;
+    bwg_batch *b = nullptr;
+    int rc = bwg_batch_stage_ksw(c, p, n, arena, arena_len, q_off, q_len, t_off, t_len, &b);
+    if (rc) return rc;
+    rc = bwg_batch_run(c, b);
+    if (!rc) rc = bwg_batch_results(c, b, out);
+    bwg_batch_destroy(c, b);
+    return rc;
+}
+
+}  // extern "C"

@@ -1,0 +1,65 @@
+// Internal declarations shared by the host runtime (bwg_api.cpp) and the kernel files.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+#include "../../include/bwg.h"
+
+namespace bwg {
+
+// Job descriptor as the kernels read it: 32 bytes, one load per job.
+struct ExtJob {
+    uint64_t q_off, t_off;  // into the device arena
+    uint32_t qlen, tlen;
+    int32_t h0;
+    uint32_t aux;  // plain form: scratch slot offset in int2 units / 8; batched: original index
+};
+struct KswJob {
+    uint64_t q_off, t_off;
+    uint32_t qlen, tlen;
+    uint32_t aux0, aux1;
+};
+
+// Scoring shared by all jobs of an extension batch, plus the values derived from it once.
+struct ExtScoring {
+    int8_t mat[28];
+    int32_t o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop;
+    int32_t max_sc;
+};
+struct KswScoring {
+    int8_t mat[28];
+    int32_t o_del, e_del, o_ins, e_ins, minsc, max_sc;
+};
+
+// Integer closed form of ksw_extend2's f64 band clamp (crates/bwa-extend/src/sw.rs:288-308):
+// trunc((double)n/e + 1.0) floored at 1 equals (n <= 0 ? 1 : n/e + 1) for n = qlen*max_sc +
+// end_bonus - o and e >= 1.
+__host__ __device__ inline int32_t clamp_band(int32_t w, int32_t qlen, const ExtScoring &s) {
+    int64_t base = (int64_t)qlen * s.max_sc + s.end_bonus;
+    int64_t ni = base - s.o_ins, nd = base - s.o_del;
+    int32_t a = ni <= 0 ? 1 : (int32_t)(ni / s.e_ins + 1);
+    int32_t b = nd <= 0 ? 1 : (int32_t)(nd / s.e_del + 1);
+    if (w > a) w = a;
+    if (w > b) w = b;
+    return w;
+}
+
+// bwa's rescue lane rule (crates/bwa-mem/src/pe.rs:420-424, matesw.rs:676-686).
+__host__ __device__ inline int32_t ksw_lanes(int32_t qlen, int32_t max_sc) {
+    return qlen * max_sc < 250 ? 16 : 8;
+}
+__host__ __device__ inline int32_t ksw_padded(int32_t qlen, int32_t max_sc) {
+    int32_t l = ksw_lanes(qlen, max_sc);
+    return (qlen + l - 1) / l * l;
+}
+
+// ---- launchers (defined in the .hip files) ----
+// Plain form: one thread per job, state in global memory.
+hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, uint32_t n,
+                            const ExtScoring &sc, int2 *scratch, bwg_ext_result *out);
+hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const KswJob *jobs, uint32_t n,
+                            const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out);
+
+}  // namespace bwg

@@ -1,0 +1,252 @@
+// Plain form of both kernels: one thread per job, jobs in arrival order, per-job DP state in
+// global memory, one launch per batch.  This is the baseline the batched form is measured against,
+// and it is written to follow the reference's scalar control flow one-for-one:
+//   extension  crates/bwa-extend/src/sw.rs:203-493   (ksw_extend2)
+//   rescue     crates/bwa-extend/src/sw.rs:921-1075  (ksw_local_fwd), :789-852 (tracker),
+//              crates/bwa-neon/src/matesw.rs:299-497 (batched_ksw_align2: forward pass, then the
+//              reverse start-recovery pass on the reversed prefixes)
+#include "bwg_internal.h"
+
+namespace bwg {
+namespace {
+
+__global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restrict__ arena,
+                                                        const ExtJob *__restrict__ jobs,
+                                                        uint32_t n, ExtScoring sc,
+                                                        int2 *__restrict__ scratch,
+                                                        bwg_ext_result *__restrict__ out) {
+    __shared__ int8_t smat[32];
+    if (threadIdx.x < 25) smat[threadIdx.x] = sc.mat[threadIdx.x];
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const ExtJob J = jobs[k];
+    const uint8_t *q = arena + J.q_off;
+    const uint8_t *t = arena + J.t_off;
+    const int32_t qlen = (int32_t)J.qlen, tlen = (int32_t)J.tlen, h0 = J.h0;
+    int2 *eh = scratch + J.aux;  // eh[j] = {H, E} over absolute columns 0..qlen
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
+
+    // row -1 (sw.rs:263-279)
+    for (int32_t j = 0; j <= qlen; ++j) eh[j] = make_int2(0, 0);
+    eh[0].x = h0;
+    {
+        int32_t v = h0 > oe_ins ? h0 - oe_ins : 0;
+        eh[1].x = v;
+        for (int32_t j = 2; j <= qlen && v > e_ins; ++j) {
+            v -= e_ins;
+            eh[j].x = v;
+        }
+    }
+    const int32_t w = clamp_band(sc.w, qlen, sc);
+
+    int32_t best = h0, best_i = -1, best_j = -1, gbest_i = -1, gscore = -1, max_off = 0;
+    int32_t beg = 0, end = qlen;
+    for (int32_t i = 0; i < tlen; ++i) {
+        const int8_t *srow = smat + 5 * t[i];
+        int32_t f = 0, rmax = 0, rj = -1;
+        if (beg < i - w) beg = i - w;
+        if (end > i + w + 1) end = i + w + 1;
+        if (end > qlen) end = qlen;
+        int32_t h1 = beg == 0 ? max(h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
+        int32_t j = beg;
+        for (; j < end; ++j) {
+            int2 v = eh[j];
+            int32_t mm = v.x != 0 ? v.x + srow[q[j]] : 0;
+            int32_t h = max(max(mm, v.y), f);
+            int32_t e = max(v.y - e_del, max(mm - oe_del, 0));
+            eh[j] = make_int2(h1, e);
+            h1 = h;
+            if (!(rmax > h)) {
+                rj = j;
+                rmax = h;
+            }
+            f = max(f - e_ins, max(mm - oe_ins, 0));
+        }
+        eh[end] = make_int2(h1, 0);
+        if (j == qlen && gscore <= h1) {
+            gbest_i = i;
+            gscore = h1;
+        }
+        if (rmax == 0) break;
+        if (rmax > best) {
+            best = rmax;
+            best_i = i;
+            best_j = rj;
+            max_off = max(max_off, abs(rj - i));
+        } else if (sc.zdrop > 0) {
+            int32_t di = i - best_i, dj = rj - best_j;
+            if (di > dj) {
+                if (best - rmax - (di - dj) * e_del > sc.zdrop) break;
+            } else if (best - rmax - (dj - di) * e_ins > sc.zdrop) {
+                break;
+            }
+        }
+        int32_t a = beg;
+        while (a < end) {
+            int2 v = eh[a];
+            if (v.x != 0 || v.y != 0) break;
+            ++a;
+        }
+        beg = a;
+        int32_t z = end;
+        while (z >= beg) {
+            int2 v = eh[z];
+            if (v.x != 0 || v.y != 0) break;
+            --z;
+        }
+        end = z + 2 < qlen ? z + 2 : qlen;
+    }
+    bwg_ext_result r;
+    r.score = best;
+    r.qle = best_j + 1;
+    r.tle = best_i + 1;
+    r.gtle = gbest_i + 1;
+    r.gscore = gscore;
+    r.max_off = max_off;
+    out[k] = r;
+}
+
+// One local-SW pass (ksw_local_fwd) for one thread.  REV reads the query prefix [0, qlen_real)
+// and the target prefix [0, tlen) back to front, which is the reverse start-recovery pass of
+// batched_ksw_align2 without materialising the reversed copies.  `he` holds {H(i-1,j), E(i,j)}
+// for the padded columns; `rowmax` (optional) receives each row's maximum for the tracker.
+template <bool REV>
+__device__ void local_pass(const uint8_t *q, int32_t qlen_real, int32_t qpad, const uint8_t *t,
+                           int32_t tlen, const int8_t *smat, const KswScoring &sc, int32_t endsc,
+                           int2 *he, int32_t *rowmax, int32_t &gmax, int32_t &te, int32_t &qe,
+                           int32_t &last_row) {
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    for (int32_t j = 0; j < qpad; ++j) he[j] = make_int2(0, 0);
+    gmax = 0;
+    te = -1;
+    qe = -1;
+    last_row = tlen - 1;
+    for (int32_t i = 0; i < tlen; ++i) {
+        const int8_t *srow = smat + 5 * (REV ? t[tlen - 1 - i] : t[i]);
+        int32_t f = 0, hd = 0, imax = 0, icol = 0;
+        for (int32_t j = 0; j < qpad; ++j) {
+            int32_t s = j < qlen_real ? srow[REV ? q[qlen_real - 1 - j] : q[j]] : 0;
+            int2 v = he[j];
+            int32_t h = max(max(hd + s, 0), max(v.y, f));
+            if (h > imax) {
+                imax = h;
+                icol = j;
+            }
+            int32_t e = max(max(v.y - sc.e_del, h - oe_del), 0);
+            f = max(max(f - sc.e_ins, h - oe_ins), 0);
+            he[j] = make_int2(h, e);
+            hd = v.x;
+        }
+        if (rowmax) rowmax[i] = imax;
+        if (imax > gmax) {
+            gmax = imax;
+            te = i;
+            qe = icol;
+            if (gmax >= endsc) {
+                last_row = i;
+                break;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void ksw_plain_kernel(const uint8_t *__restrict__ arena,
+                                                        const KswJob *__restrict__ jobs,
+                                                        uint32_t n, KswScoring sc,
+                                                        uint8_t *__restrict__ scratch,
+                                                        bwg_ksw_result *__restrict__ out) {
+    __shared__ int8_t smat[32];
+    if (threadIdx.x < 25) smat[threadIdx.x] = sc.mat[threadIdx.x];
+    __syncthreads();
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= n) return;
+    const KswJob J = jobs[k];
+    const int32_t qlen = (int32_t)J.qlen, tlen = (int32_t)J.tlen;
+    bwg_ksw_result r;
+    r.score = 0;
+    r.qb = r.qe = r.tb = r.te = r.score2 = r.te2 = -1;
+    if (qlen == 0 || tlen == 0) {
+        out[k] = r;
+        return;
+    }
+    const uint8_t *q = arena + J.q_off;
+    const uint8_t *t = arena + J.t_off;
+    const int32_t qpad = ksw_padded(qlen, sc.max_sc);
+    uint8_t *base = scratch + (size_t)J.aux0 * 16;
+    int2 *he = (int2 *)base;
+    int32_t *rowmax = (int32_t *)(base + (size_t)(qpad + 16) * sizeof(int2));
+
+    int32_t gmax, te, qe, last;
+    local_pass<false>(q, qlen, qpad, t, tlen, smat, sc, INT32_MAX, he, rowmax, gmax, te, qe, last);
+    r.score = gmax;
+    r.te = te;
+    r.qe = te >= 0 ? qe : -1;
+    // score2/te2: replay the row maxima through the suboptimal tracker (sw.rs:806-851) keeping
+    // only what finish() needs: the best entry outside the exclusion window around te.
+    {
+        int32_t s2 = -1, t2 = -1;
+        if (gmax > 0) {
+            const int32_t half = (gmax + sc.max_sc - 1) / sc.max_sc;
+            const int32_t lo = te - half, hi = te + half;
+            int32_t cs = 0, cr = -2;  // the open (last) tracker entry; cr = -2 means none
+            for (int32_t i = 0; i <= last; ++i) {
+                int32_t v = rowmax[i];
+                if (v < sc.minsc) continue;
+                if (cr >= -1 && cr + 1 == i) {
+                    if (cs < v) {
+                        cs = v;
+                        cr = i;
+                    }
+                    continue;
+                }
+                if (cr >= -1 && (cr < lo || cr > hi) && cs > s2) {
+                    s2 = cs;
+                    t2 = cr;
+                }
+                cs = v;
+                cr = i;
+            }
+            if (cr >= -1 && (cr < lo || cr > hi) && cs > s2) {
+                s2 = cs;
+                t2 = cr;
+            }
+        }
+        r.score2 = s2;
+        r.te2 = t2;
+    }
+    if (r.score >= sc.minsc && r.qe >= 0) {
+        const int32_t rq = min(r.qe + 1, qlen), rt = r.te + 1;
+        int32_t rs, rte, rqe, rlast;
+        local_pass<true>(q, rq, ksw_padded(rq, sc.max_sc), t, rt, smat, sc, r.score, he, nullptr,
+                         rs, rte, rqe, rlast);
+        if (rs == r.score) {
+            r.tb = r.te - rte;
+            r.qb = r.qe - rqe;
+        }
+    }
+    out[k] = r;
+}
+
+}  // namespace
+
+hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, uint32_t n,
+                            const ExtScoring &sc, int2 *scratch, bwg_ext_result *out) {
+    if (n == 0) return hipSuccess;
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+                       sc, scratch, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const KswJob *jobs, uint32_t n,
+                            const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out) {
+    if (n == 0) return hipSuccess;
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+                       sc, scratch, out);
+    return hipGetLastError();
+}
+
+}  // namespace bwg

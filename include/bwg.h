@@ -1,0 +1,129 @@
+/*
+ * bwg: MI355X (gfx950) Smith-Waterman backend for bwa-mem4's extension and mate-rescue hot path.
+ *
+ * C ABI of libbwg.so.  Plain pointers and sizes only.  Every entry point returns 0 (BWG_OK) on
+ * success and a negative errno-style code otherwise; nothing falls back to a CPU path.
+ *
+ * The two compute entry points replace, one for one, the reference's plug points for this path:
+ *
+ *   bwg_extend_batch      SwBackend::extend_batch   crates/bwa-extend/src/lib.rs:154-175
+ *                         (called at crates/bwa-mem/src/across.rs:1402-1413; the production
+ *                         implementor is chosen at crates/bwa-cli/src/cmd_mem.rs:2214)
+ *   bwg_extend            SwBackend::extend         crates/bwa-extend/src/lib.rs:115-129
+ *   bwg_ksw_align2_batch  batched_ksw_align2        crates/bwa-neon/src/matesw.rs:299-309
+ *                         (called at crates/bwa-mem/src/pe.rs:1090-1100)
+ *   bwg_name              SwBackend::name           crates/bwa-extend/src/lib.rs:85
+ *
+ * Jobs are described as offsets/lengths into one caller-owned byte arena holding 2-bit base codes
+ * (0=A 1=C 2=G 3=T 4=N), which is how a Rust caller can pass its borrowed slices without copying.
+ * Results are written into caller-allocated arrays, one per job, in job order.
+ */
+#ifndef BWG_H
+#define BWG_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum {
+    BWG_OK = 0,
+    BWG_EIO = -5,      /* a HIP runtime call or kernel failed */
+    BWG_E2BIG = -7,    /* a job exceeds a supported size (see bwg_limits) */
+    BWG_EAGAIN = -11,  /* serving: the HBM budget cannot admit the chunk right now */
+    BWG_ENOMEM = -12,  /* device or host allocation failed */
+    BWG_ENODEV = -19,  /* no usable gfx950 device */
+    BWG_EINVAL = -22   /* a precondition of the reference kernel is violated */
+};
+
+/* ExtendResult, crates/bwa-extend/src/sw.rs:138-162 (field order kept). */
+typedef struct { int32_t score, qle, tle, gtle, gscore, max_off; } bwg_ext_result;
+/* KswAlignResult, crates/bwa-extend/src/sw.rs:749-774 (field order kept). */
+typedef struct { int32_t score, qb, qe, tb, te, score2, te2; } bwg_ksw_result;
+
+/* Shared scoring of one extend_batch call: the arguments of SwBackend::extend_batch other than
+ * the jobs.  m must be 5; mat is row-major mat[t*5+q]. */
+typedef struct {
+    int32_t m;
+    int8_t mat[25];
+    int32_t o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop;
+} bwg_ext_params;
+
+/* Shared scoring of one batched_ksw_align2 call. */
+typedef struct {
+    int32_t m;
+    int8_t mat[25];
+    int32_t o_del, e_del, o_ins, e_ins, minsc, max_sc;
+} bwg_ksw_params;
+
+typedef struct bwg_ctx bwg_ctx;
+
+/* Create a context on HIP device `device` (its own stream and device buffers).  Returns
+ * BWG_ENODEV when the device is absent or is not gfx950. */
+int bwg_create(int device, bwg_ctx **out);
+void bwg_destroy(bwg_ctx *ctx);
+/* "hip-gfx950" (SwBackend::name). */
+const char *bwg_name(const bwg_ctx *ctx);
+const char *bwg_strerror(int code);
+
+/* Validation applied to every job (BWG_EINVAL on failure), the preconditions of ksw_extend2 and
+ * ksw_align2: m == 5; e_del, e_ins >= 1; base codes <= 4; extension h0 > 0 and qlen >= 1;
+ * rescue max_sc >= 1.  Size limits (BWG_E2BIG): extension qlen <= 4096 and tlen <= 65535; rescue
+ * qlen <= 4096 and tlen <= 65535. */
+
+/* Banded seed extension (ksw_extend2 semantics) for n jobs sharing `p`.  Thread-safe; may be
+ * called concurrently from many host threads. */
+int bwg_extend_batch(bwg_ctx *ctx, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                     size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                     const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                     bwg_ext_result *out);
+
+/* One extension (SwBackend::extend). */
+int bwg_extend(bwg_ctx *ctx, const bwg_ext_params *p, const uint8_t *query, uint32_t qlen,
+               const uint8_t *target, uint32_t tlen, int32_t h0, bwg_ext_result *out);
+
+/* Mate-rescue local alignment (batched_ksw_align2 semantics) for n jobs sharing `p`. */
+int bwg_ksw_align2_batch(bwg_ctx *ctx, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
+                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                         const uint64_t *t_off, const uint32_t *t_len, bwg_ksw_result *out);
+
+/* ---- execution form and statistics ---------------------------------------------------------
+ * BWG_OPT_FORM selects how the kernels are mapped: BWG_FORM_PLAIN is one thread per job in
+ * arrival order with per-job state in global memory and one launch per batch; BWG_FORM_BATCHED
+ * (default) bins jobs by length and score range and runs each bin with the mapping the autotuner
+ * picked.  Both give identical results. */
+enum { BWG_FORM_PLAIN = 0, BWG_FORM_BATCHED = 1 };
+enum {
+    BWG_OPT_FORM = 1,
+    BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
+    BWG_STAT_JOBS = 101,        /* jobs completed on the device */
+    BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
+    BWG_STAT_LAST_MAIN_NS = 103,   /* device time of the last batch's dominant DP kernel */
+    BWG_STAT_LAST_MAIN_LAUNCHES = 104 /* launches of that dominant kernel in the last batch */
+};
+int bwg_set_option(bwg_ctx *ctx, int key, int64_t value);
+int64_t bwg_get_stat(const bwg_ctx *ctx, int key);
+
+/* ---- device-resident batches ----------------------------------------------------------------
+ * Stage a batch into device memory once (host-to-device copy), then run it any number of times
+ * with its inputs resident in HBM, and copy results back.  bwg_extend_batch and
+ * bwg_ksw_align2_batch are exactly stage + run + results + destroy. */
+typedef struct bwg_batch bwg_batch;
+int bwg_batch_stage_ext(bwg_ctx *ctx, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                        bwg_batch **out);
+int bwg_batch_stage_ksw(bwg_ctx *ctx, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
+                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, bwg_batch **out);
+/* Enqueue the batch's kernels on the context stream and wait for them. */
+int bwg_batch_run(bwg_ctx *ctx, bwg_batch *b);
+/* Copy results (bwg_ext_result or bwg_ksw_result, n of them) to host memory. */
+int bwg_batch_results(bwg_ctx *ctx, bwg_batch *b, void *out);
+void bwg_batch_destroy(bwg_ctx *ctx, bwg_batch *b);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BWG_H */

@@ -1,0 +1,123 @@
+/*
+ * CPU oracle for the Smith-Waterman hot path (test infrastructure only).
+ *
+ * This is a plain-C restatement of the reference's scalar kernels, used by the test suite and by
+ * the CPU-baseline leg of bench.py as the checker. Nothing in the product library links or calls
+ * it.  Every function cites the reference file:line it follows (paths relative to the reference
+ * checkout root).
+ *
+ *   orc_ksw_extend2        crates/bwa-extend/src/sw.rs:203-493   (ksw_extend2)
+ *   orc_clamp_band         crates/bwa-extend/src/sw.rs:288-308,
+ *                          crates/bwa-neon/src/batched.rs:1305-1329
+ *   orc_tracker_*          crates/bwa-extend/src/sw.rs:789-852   (SuboptimalTracker)
+ *   orc_ksw_local_fwd      crates/bwa-extend/src/sw.rs:921-1075  (ksw_local_fwd)
+ *   orc_ksw_align2         crates/bwa-extend/src/sw.rs:1108-1185 (ksw_align2)
+ *   orc_padded_qlen        crates/bwa-neon/src/matesw.rs:676-686 (ksw_padded_qlen),
+ *                          lane rule crates/bwa-mem/src/pe.rs:420-424
+ *   orc_batched_align2_job crates/bwa-neon/src/matesw.rs:299-497 (batched_ksw_align2, per job)
+ *
+ * A second, independent restatement in the lockstep (SoA, many jobs per pass) form lives in
+ * ksw_lockstep.c, and brute-force checkers in ksw_brute.c.
+ */
+#ifndef KSW_ORACLE_H
+#define KSW_ORACLE_H
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct { int32_t score, qle, tle, gtle, gscore, max_off; } orc_ext_result;
+typedef struct { int32_t score, qb, qe, tb, te, score2, te2; } orc_ksw_result;
+typedef struct { int32_t score, te, qe, score2, te2; } orc_fwd_result;
+
+/* Band clamp of ksw_extend2: f64 formula exactly as the reference evaluates it. */
+int32_t orc_clamp_band(int32_t w, int32_t qlen, int32_t max_sc, int32_t end_bonus, int32_t o_ins,
+                       int32_t e_ins, int32_t o_del, int32_t e_del);
+/* Integer closed form of the same clamp (what the product library uses); compared exhaustively. */
+int32_t orc_clamp_band_int(int32_t w, int32_t qlen, int32_t max_sc, int32_t end_bonus,
+                           int32_t o_ins, int32_t e_ins, int32_t o_del, int32_t e_del);
+
+/* ksw_extend2. `cells` (optional) accumulates the number of inner-loop cells evaluated. */
+orc_ext_result orc_ksw_extend2(int32_t qlen, const uint8_t *query, int32_t tlen,
+                               const uint8_t *target, int32_t m, const int8_t *mat, int32_t o_del,
+                               int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t w,
+                               int32_t end_bonus, int32_t zdrop, int32_t h0, int64_t *cells);
+
+/* Suboptimal tracker (score2/te2).  `buf` must hold at least (number of pushed rows) pairs. */
+typedef struct { int32_t *score; int32_t *row; int32_t n; } orc_tracker;
+void orc_tracker_push_row(orc_tracker *b, int32_t row, int32_t row_max, int32_t minsc);
+void orc_tracker_finish(const orc_tracker *b, int32_t gmax, int32_t te, int32_t max_sc,
+                        int32_t *score2, int32_t *te2);
+
+/* ksw_local_fwd: query padded with score-0 columns to a multiple of `lanes`. */
+orc_fwd_result orc_ksw_local_fwd(int32_t qlen, const uint8_t *query, int32_t tlen,
+                                 const uint8_t *target, int32_t m, const int8_t *mat,
+                                 int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                                 int32_t minsc, int32_t endsc, int32_t max_sc, int32_t lanes,
+                                 int64_t *cells);
+
+int32_t orc_lanes_for(int32_t qlen, int32_t max_sc);
+int32_t orc_padded_qlen(int32_t qlen, int32_t max_sc);
+
+/* ksw_align2 (scalar, sw.rs semantics: both passes use the forward `lanes`, full target kept). */
+orc_ksw_result orc_ksw_align2(int32_t qlen, const uint8_t *query, int32_t tlen,
+                              const uint8_t *target, int32_t m, const int8_t *mat, int32_t o_del,
+                              int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t minsc,
+                              int32_t max_sc, int32_t lanes, int64_t *cells);
+
+/* batched_ksw_align2 for one job (matesw.rs semantics: each pass padded by its own length, the
+ * reverse pass runs on the reversed prefix target[..=te] only). */
+orc_ksw_result orc_batched_align2_job(int32_t qlen, const uint8_t *query, int32_t tlen,
+                                      const uint8_t *target, int32_t m, const int8_t *mat,
+                                      int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                                      int32_t minsc, int32_t max_sc, int64_t *cells);
+
+/* Batch drivers over a byte arena (the same job description the product C ABI takes), run on
+ * `nthreads` host threads with a static partition.  Used by tests and by the CPU baseline. */
+void orc_extend_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                      const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, int32_t m,
+                      const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins,
+                      int32_t e_ins, int32_t w, int32_t end_bonus, int32_t zdrop,
+                      orc_ext_result *out, int64_t *cells_out, int nthreads);
+void orc_align2_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                      const uint64_t *t_off, const uint32_t *t_len, int32_t m, const int8_t *mat,
+                      int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t minsc,
+                      int32_t max_sc, int batched_semantics, orc_ksw_result *out,
+                      int64_t *cells_out, int nthreads);
+
+/* Lockstep twins (ksw_lockstep.c): crates/bwa-neon/src/batched.rs:947-1251 and
+ * crates/bwa-neon/src/matesw.rs:933-1232, `lanes` jobs per pass in SoA layout. */
+void orc_lockstep_extend(size_t n, const uint8_t *const *queries, const int32_t *qlens,
+                         const uint8_t *const *targets, const int32_t *tlens, const int32_t *h0,
+                         int32_t m, const int8_t *mat, int32_t o_del, int32_t e_del,
+                         int32_t o_ins, int32_t e_ins, int32_t w, int32_t end_bonus,
+                         int32_t zdrop, int32_t lanes, orc_ext_result *out);
+void orc_lockstep_local_fwd(size_t n, const uint8_t *const *queries, const int32_t *qlens,
+                            const uint8_t *const *targets, const int32_t *tlens,
+                            const int32_t *minsc, const int32_t *endsc, int32_t m,
+                            const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins,
+                            int32_t e_ins, int32_t max_sc, int32_t lanes, orc_fwd_result *out);
+
+/* Brute-force checkers (ksw_brute.c). */
+/* Unbanded extension recurrence without band, z-drop or zero-row break: best score only
+ * (crates/bwa-extend/src/sw.rs:1255-1305). */
+int32_t orc_brute_extend_unbanded(int32_t qlen, const uint8_t *query, int32_t tlen,
+                                  const uint8_t *target, int32_t m, const int8_t *mat,
+                                  int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                                  int32_t h0);
+/* Textbook Gotoh local alignment (three matrices, gaps opened from H), best score only. */
+int32_t orc_brute_gotoh_local(int32_t qlen, const uint8_t *query, int32_t tlen,
+                              const uint8_t *target, int32_t m, const int8_t *mat, int32_t o_del,
+                              int32_t e_del, int32_t o_ins, int32_t e_ins);
+/* Exhaustive enumeration of local alignment paths on tiny inputs (qlen,tlen <= 7): best score,
+ * the first target row reaching it, and the smallest query column reaching it in that row. */
+void orc_brute_enum_local(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target,
+                          int32_t m, const int8_t *mat, int32_t o_del, int32_t e_del,
+                          int32_t o_ins, int32_t e_ins, int32_t *score, int32_t *te, int32_t *qe);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,260 @@
+"""ctypes binding to the CPU oracle (oracle/liboracle_ksw.so).
+
+Test infrastructure: only the tests, ``__graft_entry__.smoke()`` and bench.py's CPU-baseline leg
+load this, as the checker.  The product library never does.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_ksw.so")
+
+
+class ExtResult(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("score", "qle", "tle", "gtle", "gscore", "max_off")]
+
+    def tup(self):
+        return (self.score, self.qle, self.tle, self.gtle, self.gscore, self.max_off)
+
+
+class KswResult(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("score", "qb", "qe", "tb", "te", "score2", "te2")]
+
+    def tup(self):
+        return (self.score, self.qb, self.qe, self.tb, self.te, self.score2, self.te2)
+
+
+class FwdResult(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("score", "te", "qe", "score2", "te2")]
+
+    def tup(self):
+        return (self.score, self.te, self.qe, self.score2, self.te2)
+
+
+EXT_DTYPE = np.dtype([(n, "<i4") for n in ("score", "qle", "tle", "gtle", "gscore", "max_off")])
+KSW_DTYPE = np.dtype([(n, "<i4") for n in ("score", "qb", "qe", "tb", "te", "score2", "te2")])
+
+_P = C.c_void_p
+_I = C.c_int32
+_lib = None
+
+
+def build_oracle():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-C", os.path.join(ROOT, "oracle")])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            build_oracle()
+        L = C.CDLL(ORACLE_SO)
+        L.orc_ksw_extend2.restype = ExtResult
+        L.orc_ksw_extend2.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_ksw_local_fwd.restype = FwdResult
+        L.orc_ksw_local_fwd.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_ksw_align2.restype = KswResult
+        L.orc_ksw_align2.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_batched_align2_job.restype = KswResult
+        L.orc_batched_align2_job.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _P]
+        for f in ("orc_clamp_band", "orc_clamp_band_int"):
+            getattr(L, f).restype = _I
+            getattr(L, f).argtypes = [_I] * 8
+        L.orc_lanes_for.restype = _I
+        L.orc_lanes_for.argtypes = [_I, _I]
+        L.orc_padded_qlen.restype = _I
+        L.orc_padded_qlen.argtypes = [_I, _I]
+        L.orc_tracker_push_row.restype = None
+        L.orc_tracker_push_row.argtypes = [_P, _I, _I, _I]
+        L.orc_tracker_finish.restype = None
+        L.orc_tracker_finish.argtypes = [_P, _I, _I, _I, _P, _P]
+        L.orc_extend_batch.restype = None
+        L.orc_extend_batch.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _I, _P,
+                                       _I, _I, _I, _I, _I, _I, _I, _P, _P, C.c_int]
+        L.orc_align2_batch.restype = None
+        L.orc_align2_batch.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _I, _P,
+                                       _I, _I, _I, _I, _I, _I, C.c_int, _P, _P, C.c_int]
+        L.orc_lockstep_extend.restype = None
+        L.orc_lockstep_extend.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _I, _P,
+                                          _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_lockstep_local_fwd.restype = None
+        L.orc_lockstep_local_fwd.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _I, _P,
+                                             _I, _I, _I, _I, _I, _I, _P]
+        L.orc_brute_extend_unbanded.restype = _I
+        L.orc_brute_extend_unbanded.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I]
+        L.orc_brute_gotoh_local.restype = _I
+        L.orc_brute_gotoh_local.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I]
+        L.orc_brute_enum_local.restype = None
+        L.orc_brute_enum_local.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P, _P, _P]
+        _lib = L
+    return _lib
+
+
+def _buf(seq):
+    a = np.ascontiguousarray(np.asarray(seq, dtype=np.uint8))
+    return a, a.ctypes.data
+
+
+def _mat(mat):
+    a = np.ascontiguousarray(np.asarray(mat, dtype=np.int8))
+    assert a.size == 25
+    return a, a.ctypes.data
+
+
+def scmat(a=1, b=4):
+    """bwa's 5x5 matrix: match a, mismatch -b, N row/column -1 (crates/bwa-core/src/opt.rs:445)."""
+    m = np.full((5, 5), -1, dtype=np.int8)
+    for i in range(4):
+        for j in range(4):
+            m[i, j] = a if i == j else -b
+    return m.reshape(-1)
+
+
+def extend2(q, t, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0, want_cells=False):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    cells = C.c_int64(0)
+    r = lib().orc_ksw_extend2(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins, w,
+                              end_bonus, zdrop, h0, C.addressof(cells))
+    return (r.tup(), cells.value) if want_cells else r.tup()
+
+
+def local_fwd(q, t, mat, o_del, e_del, o_ins, e_ins, minsc, endsc, max_sc, lanes):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    return lib().orc_ksw_local_fwd(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins,
+                                   minsc, endsc, max_sc, lanes, None).tup()
+
+
+def align2(q, t, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, lanes):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    return lib().orc_ksw_align2(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins,
+                                minsc, max_sc, lanes, None).tup()
+
+
+def batched_align2_job(q, t, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    return lib().orc_batched_align2_job(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins,
+                                        e_ins, minsc, max_sc, None).tup()
+
+
+class Tracker:
+    """The suboptimal tracker, driven row by row (crates/bwa-extend/src/sw.rs:789-852)."""
+
+    def __init__(self, cap=4096):
+        self.score = (C.c_int32 * cap)()
+        self.row = (C.c_int32 * cap)()
+
+        class T(C.Structure):
+            _fields_ = [("score", _P), ("row", _P), ("n", C.c_int32)]
+
+        self.t = T(C.addressof(self.score), C.addressof(self.row), 0)
+
+    def push_row(self, row, row_max, minsc):
+        lib().orc_tracker_push_row(C.addressof(self.t), row, row_max, minsc)
+
+    def entries(self):
+        return [(self.score[k], self.row[k]) for k in range(self.t.n)]
+
+    def finish(self, gmax, te, max_sc):
+        s2, t2 = C.c_int32(0), C.c_int32(0)
+        lib().orc_tracker_finish(C.addressof(self.t), gmax, te, max_sc, C.addressof(s2), C.addressof(t2))
+        return s2.value, t2.value
+
+
+def extend_batch(batch, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, nthreads=8,
+                 want_cells=False):
+    """batch: a JobBatch-like object with arena/q_off/q_len/t_off/t_len/h0 numpy arrays."""
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=EXT_DTYPE)
+    ma, mp = _mat(mat)
+    cells = C.c_int64(0)
+    lib().orc_extend_batch(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                           batch.t_off.ctypes.data, batch.t_len.ctypes.data, batch.h0.ctypes.data,
+                           5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop,
+                           out.ctypes.data, C.addressof(cells), nthreads)
+    return (out, cells.value) if want_cells else out
+
+
+def align2_batch(batch, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, batched=True, nthreads=8,
+                 want_cells=False):
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=KSW_DTYPE)
+    ma, mp = _mat(mat)
+    cells = C.c_int64(0)
+    lib().orc_align2_batch(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                           batch.t_off.ctypes.data, batch.t_len.ctypes.data, 5, mp,
+                           o_del, e_del, o_ins, e_ins, minsc, max_sc, 1 if batched else 0,
+                           out.ctypes.data, C.addressof(cells), nthreads)
+    return (out, cells.value) if want_cells else out
+
+
+def _ptr_arrays(seqs):
+    bufs = [np.ascontiguousarray(np.asarray(s, dtype=np.uint8)) for s in seqs]
+    ptrs = (C.c_void_p * max(1, len(bufs)))(*[b.ctypes.data for b in bufs])
+    lens = np.array([len(b) for b in bufs], dtype=np.int32)
+    return bufs, ptrs, lens
+
+
+def lockstep_extend(queries, targets, h0s, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop,
+                    lanes=16):
+    qb, qp, ql = _ptr_arrays(queries)
+    tb, tp, tl = _ptr_arrays(targets)
+    h0 = np.asarray(h0s, dtype=np.int32)
+    ma, mp = _mat(mat)
+    out = (ExtResult * max(1, len(queries)))()
+    lib().orc_lockstep_extend(len(queries), C.addressof(qp), ql.ctypes.data, C.addressof(tp),
+                              tl.ctypes.data, h0.ctypes.data, 5, mp, o_del, e_del, o_ins, e_ins,
+                              w, end_bonus, zdrop, lanes, C.addressof(out))
+    return [out[k].tup() for k in range(len(queries))]
+
+
+def lockstep_local_fwd(queries, targets, minsc, endsc, mat, o_del, e_del, o_ins, e_ins, max_sc,
+                       lanes=16):
+    qb, qp, ql = _ptr_arrays(queries)
+    tb, tp, tl = _ptr_arrays(targets)
+    mi = np.asarray(minsc, dtype=np.int32)
+    en = np.asarray(endsc, dtype=np.int32)
+    ma, mp = _mat(mat)
+    out = (FwdResult * max(1, len(queries)))()
+    lib().orc_lockstep_local_fwd(len(queries), C.addressof(qp), ql.ctypes.data, C.addressof(tp),
+                                 tl.ctypes.data, mi.ctypes.data, en.ctypes.data, 5, mp,
+                                 o_del, e_del, o_ins, e_ins, max_sc, lanes, C.addressof(out))
+    return [out[k].tup() for k in range(len(queries))]
+
+
+def brute_extend_unbanded(q, t, mat, o_del, e_del, o_ins, e_ins, h0):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    return lib().orc_brute_extend_unbanded(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins,
+                                           e_ins, h0)
+
+
+def brute_gotoh_local(q, t, mat, o_del, e_del, o_ins, e_ins, pad_to=None):
+    q = list(q)
+    if pad_to is not None:
+        q = q + [5] * (pad_to - len(q))
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    return lib().orc_brute_gotoh_local(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins)
+
+
+def brute_enum_local(q, t, mat, o_del, e_del, o_ins, e_ins):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    s, te, qe = C.c_int32(), C.c_int32(), C.c_int32()
+    lib().orc_brute_enum_local(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins,
+                               C.addressof(s), C.addressof(te), C.addressof(qe))
+    return s.value, te.value, qe.value
