@@ -11,5 +11,7 @@ struct BatchedTiming {
 
 int run_batched(bwg_ctx *c, bwg_batch *b, BatchedTiming *t);
 void release_batched(bwg_ctx *c, bwg_batch *b);
+int run_batched_ksw(bwg_ctx *c, bwg_batch *b, BatchedTiming *t);
+void release_batched_ksw(bwg_ctx *c, bwg_batch *b);
 
 }  // namespace bwg

@@ -1,0 +1,21 @@
+// Launchers of the batched-form kernels.
+#pragma once
+#include "bwg_internal.h"
+
+// Odd K values (window columns per lane) the group kernel is instantiated for; 16*K columns.
+#define BWG_EXT_GROUP_KS BWG_K(3) BWG_K(5) BWG_K(7) BWG_K(9) BWG_K(11) BWG_K(13) BWG_K(15) BWG_K(17) BWG_K(21) BWG_K(25) BWG_K(33)
+
+namespace bwg {
+
+int ext_group_words(int qcap);
+int ext_group_blocks_per_cu(int K, int qcap);
+hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const ExtJob *jobs,
+                            const uint32_t *order, uint32_t count, uint32_t *head,
+                            const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid);
+int ext_thread_blocks_per_cu(int qcap);
+hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const ExtJob *jobs,
+                             const uint32_t *order, uint32_t count, uint32_t *head,
+                             const ExtScoring &sc, bwg_ext_result *out, int grid);
+hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n);
+
+}  // namespace bwg

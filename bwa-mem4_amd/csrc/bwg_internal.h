@@ -62,4 +62,12 @@ hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const ExtJob *
 hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const KswJob *jobs, uint32_t n,
                             const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out);
 
+// Same kernels over a compacted job list whose k-th result goes to out[out_index[k]].
+hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const ExtJob *jobs,
+                                    const uint32_t *out_index, uint32_t n, const ExtScoring &sc,
+                                    int2 *scratch, bwg_ext_result *out);
+hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const KswJob *jobs,
+                                    const uint32_t *out_index, uint32_t n, const KswScoring &sc,
+                                    uint8_t *scratch, bwg_ksw_result *out);
+
 }  // namespace bwg

@@ -41,7 +41,7 @@ struct bwg_ctx {
     hipStream_t stream = nullptr;
     hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
     std::mutex run_mu;  // one batch on the context stream at a time
-    int form = BWG_FORM_PLAIN;
+    int form = BWG_FORM_BATCHED;
     std::atomic<int64_t> launches{0}, jobs{0};
     int64_t last_kernel_ns = 0, last_main_ns = 0, last_main_launches = 0;
     bwg::DevicePool pool;
@@ -63,6 +63,8 @@ struct bwg_batch {
     void *d_plain_scratch = nullptr;
     size_t plain_scratch_bytes = 0;
     void *d_plain_jobs = nullptr;  // descriptors carrying the plain form's scratch offsets
+    void *ext_plan = nullptr;      // batched form: bins, order and queues (bwg_batched.cpp)
+    void *ksw_plan = nullptr;
 };
 
 namespace bwg {

@@ -1,0 +1,57 @@
+// Cross-lane primitives on 16-lane DPP rows (gfx9 DPP controls).  A "group" is one 16-lane DPP
+// row of a 64-wide wavefront; all reductions and scans here stay inside the row.
+#pragma once
+#include <hip/hip_runtime.h>
+
+namespace bwg {
+
+// DPP control codes (gfx9 encoding).
+enum : int {
+    kRowShr = 0x110,  // + n : lane l reads lane l-n of its row
+    kRowRor = 0x120,  // + n : lane l reads lane (l-n) mod 16 of its row
+};
+
+template <int CTRL, bool BOUND_ZERO>
+__device__ __forceinline__ int dpp(int old, int src) {
+    return __builtin_amdgcn_update_dpp(old, src, CTRL, 0xf, 0xf, BOUND_ZERO);
+}
+
+// Max over the 16 lanes of the row, result in every lane of the row.
+__device__ __forceinline__ int row_allmax(int x) {
+    x = max(x, dpp<kRowRor + 1, false>(x, x));
+    x = max(x, dpp<kRowRor + 2, false>(x, x));
+    x = max(x, dpp<kRowRor + 4, false>(x, x));
+    x = max(x, dpp<kRowRor + 8, false>(x, x));
+    return x;
+}
+__device__ __forceinline__ int row_allmin(int x) {
+    x = min(x, dpp<kRowRor + 1, false>(x, x));
+    x = min(x, dpp<kRowRor + 2, false>(x, x));
+    x = min(x, dpp<kRowRor + 4, false>(x, x));
+    x = min(x, dpp<kRowRor + 8, false>(x, x));
+    return x;
+}
+
+// Inclusive prefix max over the row for non-negative values (lanes shifted in from outside the
+// row read 0, the identity for non-negative inputs).
+__device__ __forceinline__ int row_scan_max_nonneg(int x) {
+    x = max(x, dpp<kRowShr + 1, true>(0, x));
+    x = max(x, dpp<kRowShr + 2, true>(0, x));
+    x = max(x, dpp<kRowShr + 4, true>(0, x));
+    x = max(x, dpp<kRowShr + 8, true>(0, x));
+    return x;
+}
+
+// Value of lane l-1 of the row; lane 0 of the row receives `fill`.
+__device__ __forceinline__ int row_shift_in(int x, int fill) { return dpp<kRowShr + 1, false>(fill, x); }
+
+// Order LDS accesses of the lanes of one wavefront: everything before is visible to every lane
+// of the wave after.  A wavefront's LDS operations execute in issue order, so this only has to
+// stop the compiler from moving accesses across it.
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+}  // namespace bwg

@@ -1,0 +1,424 @@
+// Batched form of the banded extension (ksw_extend2 semantics, crates/bwa-extend/src/sw.rs:203-493).
+//
+// Two persistent kernels, each draining a device work queue of length-binned jobs:
+//
+//  * ext_group_kernel<K>: one job per 16-lane DPP row ("group"), four jobs per wavefront.  Row i
+//    of the DP is evaluated in parallel over the live window [beg, end): lane g owns the K
+//    consecutive window columns beg + g*K .. beg + g*K + K-1 (window-relative blocking; K odd keeps
+//    the 16 lanes' LDS accesses in distinct banks).  The eh arrays of the reference live in LDS over
+//    absolute columns 0..qlen, so columns outside the window keep their stale values exactly as in
+//    the reference.  M and E depend only on the previous row; F is the max-plus recurrence
+//    F(j+1) = max(F(j) - e_ins, max(M(j) - oe_ins, 0)), evaluated in two passes: each lane runs it
+//    over its block from F_in = 0, the blocks are combined by an exclusive max-scan across the row
+//    (F(l) = max_{m<l} c_m - (l-1-m)*K*e_ins), and a second pass applies each lane's incoming F.
+//    Row maximum with the later-column tie rule is a max over keys (h << 13 | j).  The row
+//    epilogue (to-end score, z-drop, band tightening) is evaluated redundantly by the 16 lanes.
+//
+//  * ext_thread_kernel<QCAP>: one job per lane for short queries (qlen <= QCAP), the scalar
+//    recurrence with the eh arrays in LDS laid out [column][lane].
+#include "bwg_batched_kernels.h"
+#include "dpp.h"
+
+namespace bwg {
+namespace {
+
+constexpr int kKeyShift = 13;  // key = h << 13 | j, j < 8192
+constexpr int kKeyMask = (1 << kKeyShift) - 1;
+
+struct GroupJob {
+    // per-job constants (identical in the 16 lanes of the group)
+    const uint8_t *t;
+    int32_t qlen, tlen, h0, w;
+    uint32_t out_idx;
+    // row state
+    int32_t i, beg, end, best, best_i, best_j, gbest_i, gscore, max_off;
+    int32_t tcur;  // target code of row i
+};
+
+// Per-group LDS layout, in 32-bit words: eh pairs {H[j], E[j]} for j in 0..qcap (+1 spare), then
+// the query profile prof[j] (scores of target codes 0..3 against query[j], one byte each) and the
+// N-row scores prof4[j] (mat[4][query[j]], one byte each, packed four per word).
+__host__ __device__ __forceinline__ int group_words(int qcap) { return 2 * (qcap + 2) + (qcap + 1) + (qcap + 4) / 4; }
+
+template <int K>
+__global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restrict__ arena,
+                                                        const ExtJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ order,
+                                                        uint32_t count, uint32_t *__restrict__ head,
+                                                        ExtScoring sc, int qcap,
+                                                        bwg_ext_result *__restrict__ out) {
+    extern __shared__ int32_t lds[];
+    const int lane = threadIdx.x & 63;
+    const int g = lane & 15;  // lane within the group
+    const int gid = (threadIdx.x >> 4);  // group within the block
+    const int gw = group_words(qcap);
+    int32_t *eh = lds + gid * gw;                   // {H, E} pairs
+    uint32_t *prof = (uint32_t *)(eh + 2 * (qcap + 2));
+    int8_t *prof4 = (int8_t *)(prof + (qcap + 1));
+
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
+    const int32_t Ke = K * e_ins;
+
+    GroupJob J;
+    bool active = false, exhausted = false;
+    for (;;) {
+        // ---- fetch and initialise a job for idle groups ------------------------------------
+        if (!active && !exhausted) {
+            uint32_t slot = 0;
+            if (g == 0) slot = atomicAdd(head, 1u);
+            slot = __shfl(slot, lane & ~15, 64);
+            if (slot >= count) {
+                exhausted = true;
+            } else {
+                const ExtJob jb = jobs[order[slot]];
+                const uint8_t *q = arena + jb.q_off;
+                J.t = arena + jb.t_off;
+                J.qlen = (int32_t)jb.qlen;
+                J.tlen = (int32_t)jb.tlen;
+                J.h0 = jb.h0;
+                J.out_idx = order[slot];
+                J.w = clamp_band(sc.w, J.qlen, sc);
+                // row -1 (sw.rs:263-279): H[0] = h0, H[j] = max(h0 - oe_ins - (j-1)*e_ins, 0)
+                for (int j = g; j <= J.qlen + 1; j += 16) {
+                    int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
+                    eh[2 * j] = j <= J.qlen ? h : 0;
+                    eh[2 * j + 1] = 0;
+                }
+                for (int j = g; j < J.qlen; j += 16) {
+                    const int qc = q[j];
+                    uint32_t p = (uint32_t)(uint8_t)sc.mat[qc] | ((uint32_t)(uint8_t)sc.mat[5 + qc] << 8) |
+                                 ((uint32_t)(uint8_t)sc.mat[10 + qc] << 16) |
+                                 ((uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
+                    prof[j] = p;
+                    prof4[j] = sc.mat[20 + qc];
+                }
+                J.i = 0;
+                J.beg = 0;
+                J.end = J.qlen;
+                J.best = J.h0;
+                J.best_i = J.best_j = J.gbest_i = J.gscore = -1;
+                J.max_off = 0;
+                J.tcur = J.tlen > 0 ? J.t[0] : 0;
+                active = J.tlen > 0;
+                if (!active && g == 0) {
+                    bwg_ext_result r = {J.h0, 0, 0, 0, -1, 0};
+                    out[J.out_idx] = r;
+                }
+            }
+        }
+        wave_lds_sync();
+        // the wavefront exits once none of its four groups has work left
+        if (__ballot(active || !exhausted) == 0) break;
+        if (!active) continue;
+
+        // ---- one DP row ------------------------------------------------------------------------
+        const int32_t i = J.i;
+        int32_t beg = J.beg, end = J.end;
+        if (beg < i - J.w) beg = i - J.w;
+        if (end > i + J.w + 1) end = i + J.w + 1;
+        if (end > J.qlen) end = J.qlen;
+        const int32_t h1_init = beg == 0 ? max(J.h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
+        const int32_t tcode = J.tcur;
+        const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
+        const bool trow_n = tcode >= 4;
+        const uint32_t tsh = (uint32_t)tcode * 8u;
+
+        const int32_t base = beg + g * K;   // first window column of this lane
+        const int32_t nval = min(max(end - base, 0), K);
+        int32_t mm[K], ee[K], gg[K];
+        int32_t grun = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            mm[k] = 0;
+            ee[k] = 0;
+            gg[k] = grun;
+            if (k < nval) {
+                const int j = base + k;
+                const int2 v = *(const int2 *)(eh + 2 * j);
+                const int32_t s = trow_n ? (int32_t)prof4[j] : (int32_t)__builtin_amdgcn_sbfe(prof[j], tsh, 8);
+                const int32_t m = v.x != 0 ? v.x + s : 0;
+                mm[k] = m;
+                ee[k] = v.y;
+                grun = max(grun - e_ins, max(m - oe_ins, 0));
+            }
+        }
+        // F entering this lane's block: exclusive max-scan of U_l = c_l + l*K*e_ins over the row
+        const int32_t U = grun + g * Ke;
+        const int32_t incl = row_scan_max_nonneg(U);
+        const int32_t excl = row_shift_in(incl, -Ke);
+        const int32_t f_in = excl - (g - 1) * Ke;
+
+        int32_t key = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            if (k < nval) {
+                const int j = base + k;
+                const int32_t f = max(f_in - k * e_ins, gg[k]);
+                const int32_t h = max(max(mm[k], ee[k]), f);
+                key = max(key, (h << kKeyShift) | j);
+                const int32_t e = max(max(ee[k] - e_del, mm[k] - oe_del), 0);
+                eh[2 * j + 1] = e;      // E(i+1, j)
+                eh[2 * j + 2] = h;      // H(i, j) is the diagonal of column j+1 next row
+            }
+        }
+        if (g == 0) {
+            eh[2 * beg] = h1_init;      // H(i, beg-1)
+            eh[2 * end + 1] = 0;        // E at the sentinel column
+        }
+        wave_lds_sync();
+        key = row_allmax(key);
+        const int32_t row_max = key >> kKeyShift;
+        const int32_t mj = key & kKeyMask;
+        const int32_t h1_last = eh[2 * end];  // H(i, end-1), or h1_init for an empty window
+
+        bool done = false;
+        const int32_t j_exit = beg < end ? end : beg;
+        if (j_exit == J.qlen && J.gscore <= h1_last) {
+            J.gbest_i = i;
+            J.gscore = h1_last;
+        }
+        if (row_max == 0) {
+            done = true;
+        } else if (row_max > J.best) {
+            J.best = row_max;
+            J.best_i = i;
+            J.best_j = mj;
+            J.max_off = max(J.max_off, abs(mj - i));
+        } else if (sc.zdrop > 0) {
+            const int32_t di = i - J.best_i, dj = mj - J.best_j;
+            if (di > dj) {
+                if (J.best - row_max - (di - dj) * e_del > sc.zdrop) done = true;
+            } else if (J.best - row_max - (dj - di) * e_ins > sc.zdrop) {
+                done = true;
+            }
+        }
+        if (!done) {
+            // band tightening (sw.rs:464-480): first live column in [beg, end), last live column
+            // scanning down from end; 16 columns per probe
+            int32_t nb = beg;
+            for (;;) {
+                const int j = nb + g;
+                bool live = false;
+                if (j < end) live = (eh[2 * j] | eh[2 * j + 1]) != 0;
+                const int32_t cand = row_allmin(live ? j : 0x7fffffff);
+                if (cand != 0x7fffffff) { nb = cand; break; }
+                nb += 16;
+                if (nb >= end) { nb = end; break; }
+            }
+            int32_t ne = end;
+            for (;;) {
+                const int j = ne - g;
+                bool live = false;
+                if (j >= nb) live = (eh[2 * j] | eh[2 * j + 1]) != 0;
+                const int32_t cand = row_allmax(live ? j : -1);
+                if (cand >= 0) { ne = cand; break; }
+                ne -= 16;
+                if (ne < nb) { ne = nb - 1; break; }
+            }
+            J.beg = nb;
+            J.end = ne + 2 < J.qlen ? ne + 2 : J.qlen;
+            J.i = i + 1;
+            J.tcur = tnext;
+            if (J.i >= J.tlen) done = true;
+        }
+        if (done) {
+            if (g == 0) {
+                bwg_ext_result r;
+                r.score = J.best;
+                r.qle = J.best_j + 1;
+                r.tle = J.best_i + 1;
+                r.gtle = J.gbest_i + 1;
+                r.gscore = J.gscore;
+                r.max_off = J.max_off;
+                out[J.out_idx] = r;
+            }
+            active = false;
+        }
+    }
+}
+
+// ---- short queries: one job per lane --------------------------------------------------------
+template <int QCAP>
+__global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restrict__ arena,
+                                                        const ExtJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ order,
+                                                        uint32_t count, uint32_t *__restrict__ head,
+                                                        ExtScoring sc,
+                                                        bwg_ext_result *__restrict__ out) {
+    __shared__ int2 eh[(QCAP + 1) * 64];
+    __shared__ uint32_t prof[QCAP * 64];
+    __shared__ int8_t prof4[QCAP * 64];
+    const int lane = threadIdx.x;
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
+    for (;;) {
+        uint32_t slot = 0;
+        if (lane == 0) slot = atomicAdd(head, 64u);
+        slot = __shfl(slot, 0, 64);
+        if (slot >= count) break;
+        const uint32_t my = slot + lane;
+        if (my < count) {
+            const uint32_t idx = order[my];
+            const ExtJob jb = jobs[idx];
+            const uint8_t *q = arena + jb.q_off;
+            const uint8_t *t = arena + jb.t_off;
+            const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
+            const int32_t w = clamp_band(sc.w, qlen, sc);
+            for (int j = 0; j <= qlen; ++j)
+                eh[j * 64 + lane] = make_int2(j == 0 ? h0 : max(h0 - oe_ins - (j - 1) * e_ins, 0), 0);
+            for (int j = 0; j < qlen; ++j) {
+                const int qc = q[j];
+                prof[j * 64 + lane] = (uint32_t)(uint8_t)sc.mat[qc] | ((uint32_t)(uint8_t)sc.mat[5 + qc] << 8) |
+                                      ((uint32_t)(uint8_t)sc.mat[10 + qc] << 16) |
+                                      ((uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
+                prof4[j * 64 + lane] = sc.mat[20 + qc];
+            }
+            int32_t best = h0, best_i = -1, best_j = -1, gbest_i = -1, gscore = -1, max_off = 0;
+            int32_t beg = 0, end = qlen;
+            int32_t tnext = tlen > 0 ? t[0] : 0;
+            for (int32_t i = 0; i < tlen; ++i) {
+                const int32_t tcode = tnext;
+                if (i + 1 < tlen) tnext = t[i + 1];
+                const bool trow_n = tcode >= 4;
+                const uint32_t tsh = (uint32_t)tcode * 8u;
+                int32_t f = 0, key = 0;
+                if (beg < i - w) beg = i - w;
+                if (end > i + w + 1) end = i + w + 1;
+                if (end > qlen) end = qlen;
+                int32_t h1 = beg == 0 ? max(h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
+                int32_t j = beg;
+                for (; j < end; ++j) {
+                    const int2 v = eh[j * 64 + lane];
+                    const int32_t s = trow_n ? (int32_t)prof4[j * 64 + lane]
+                                             : (int32_t)__builtin_amdgcn_sbfe(prof[j * 64 + lane], tsh, 8);
+                    const int32_t m = v.x != 0 ? v.x + s : 0;
+                    const int32_t h = max(max(m, v.y), f);
+                    const int32_t e = max(max(v.y - e_del, m - oe_del), 0);
+                    eh[j * 64 + lane] = make_int2(h1, e);
+                    h1 = h;
+                    key = max(key, (h << kKeyShift) | j);
+                    f = max(f - e_ins, max(m - oe_ins, 0));
+                }
+                eh[end * 64 + lane] = make_int2(h1, 0);
+                const int32_t row_max = key >> kKeyShift, mj = key & kKeyMask;
+                if (j == qlen && gscore <= h1) {
+                    gbest_i = i;
+                    gscore = h1;
+                }
+                if (row_max == 0) break;
+                if (row_max > best) {
+                    best = row_max;
+                    best_i = i;
+                    best_j = mj;
+                    max_off = max(max_off, abs(mj - i));
+                } else if (sc.zdrop > 0) {
+                    const int32_t di = i - best_i, dj = mj - best_j;
+                    if (di > dj) {
+                        if (best - row_max - (di - dj) * e_del > sc.zdrop) break;
+                    } else if (best - row_max - (dj - di) * e_ins > sc.zdrop) {
+                        break;
+                    }
+                }
+                int32_t a = beg;
+                while (a < end) {
+                    const int2 v = eh[a * 64 + lane];
+                    if ((v.x | v.y) != 0) break;
+                    ++a;
+                }
+                beg = a;
+                int32_t z = end;
+                while (z >= beg) {
+                    const int2 v = eh[z * 64 + lane];
+                    if ((v.x | v.y) != 0) break;
+                    --z;
+                }
+                end = z + 2 < qlen ? z + 2 : qlen;
+            }
+            bwg_ext_result r;
+            r.score = best;
+            r.qle = best_j + 1;
+            r.tle = best_i + 1;
+            r.gtle = gbest_i + 1;
+            r.gscore = gscore;
+            r.max_off = max_off;
+            out[idx] = r;
+        }
+    }
+}
+
+template <int K>
+hipError_t launch_group(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, const uint32_t *order,
+                        uint32_t count, uint32_t *head, const ExtScoring &sc, int qcap,
+                        bwg_ext_result *out, int grid) {
+    const size_t lds = (size_t)16 * group_words(qcap) * sizeof(int32_t);  // 16 groups per block
+    hipLaunchKernelGGL(ext_group_kernel<K>, dim3(grid), dim3(256), lds, st, arena, jobs, order, count,
+                       head, sc, qcap, out);
+    return hipGetLastError();
+}
+
+template <int K>
+int group_occupancy(int qcap) {
+    int blocks = 0;
+    const size_t lds = (size_t)16 * group_words(qcap) * sizeof(int32_t);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K>, 256, lds) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+template <int QCAP>
+int thread_occupancy() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_thread_kernel<QCAP>, 64, 0) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+}  // namespace
+
+int ext_group_words(int qcap) { return 2 * (qcap + 2) + (qcap + 1) + (qcap + 4) / 4; }
+
+int ext_group_blocks_per_cu(int K, int qcap) {
+    switch (K) {
+#define BWG_K(k) case k: return group_occupancy<k>(qcap);
+        BWG_EXT_GROUP_KS
+#undef BWG_K
+    default: return 1;
+    }
+}
+
+hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const ExtJob *jobs,
+                            const uint32_t *order, uint32_t count, uint32_t *head,
+                            const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid) {
+    switch (K) {
+#define BWG_K(k) case k: return launch_group<k>(st, arena, jobs, order, count, head, sc, qcap, out, grid);
+        BWG_EXT_GROUP_KS
+#undef BWG_K
+    default: return hipErrorInvalidValue;
+    }
+}
+
+int ext_thread_blocks_per_cu(int qcap) {
+    switch (qcap) {
+    case 16: return thread_occupancy<16>();
+    case 32: return thread_occupancy<32>();
+    default: return 1;
+    }
+}
+
+hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const ExtJob *jobs,
+                             const uint32_t *order, uint32_t count, uint32_t *head,
+                             const ExtScoring &sc, bwg_ext_result *out, int grid) {
+    switch (qcap) {
+    case 16:
+        hipLaunchKernelGGL(ext_thread_kernel<16>, dim3(grid), dim3(64), 0, st, arena, jobs, order, count, head, sc, out);
+        break;
+    case 32:
+        hipLaunchKernelGGL(ext_thread_kernel<32>, dim3(grid), dim3(64), 0, st, arena, jobs, order, count, head, sc, out);
+        break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace bwg

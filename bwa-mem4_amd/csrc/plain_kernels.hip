@@ -14,6 +14,7 @@ __global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restric
                                                         const ExtJob *__restrict__ jobs,
                                                         uint32_t n, ExtScoring sc,
                                                         int2 *__restrict__ scratch,
+                                                        const uint32_t *__restrict__ out_index,
                                                         bwg_ext_result *__restrict__ out) {
     __shared__ int8_t smat[32];
     if (threadIdx.x < 25) smat[threadIdx.x] = sc.mat[threadIdx.x];
@@ -105,7 +106,7 @@ __global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restric
     r.gtle = gbest_i + 1;
     r.gscore = gscore;
     r.max_off = max_off;
-    out[k] = r;
+    out[out_index ? out_index[k] : k] = r;
 }
 
 // One local-SW pass (ksw_local_fwd) for one thread.  REV reads the query prefix [0, qlen_real)
@@ -156,6 +157,7 @@ __global__ __launch_bounds__(256) void ksw_plain_kernel(const uint8_t *__restric
                                                         const KswJob *__restrict__ jobs,
                                                         uint32_t n, KswScoring sc,
                                                         uint8_t *__restrict__ scratch,
+                                                        const uint32_t *__restrict__ out_index,
                                                         bwg_ksw_result *__restrict__ out) {
     __shared__ int8_t smat[32];
     if (threadIdx.x < 25) smat[threadIdx.x] = sc.mat[threadIdx.x];
@@ -167,8 +169,9 @@ __global__ __launch_bounds__(256) void ksw_plain_kernel(const uint8_t *__restric
     bwg_ksw_result r;
     r.score = 0;
     r.qb = r.qe = r.tb = r.te = r.score2 = r.te2 = -1;
+    const uint32_t ok = out_index ? out_index[k] : k;
     if (qlen == 0 || tlen == 0) {
-        out[k] = r;
+        out[ok] = r;
         return;
     }
     const uint8_t *q = arena + J.q_off;
@@ -226,7 +229,12 @@ __global__ __launch_bounds__(256) void ksw_plain_kernel(const uint8_t *__restric
             r.qb = r.qe - rqe;
         }
     }
-    out[k] = r;
+    out[ok] = r;
+}
+
+__global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = 0;
 }
 
 }  // namespace
@@ -236,7 +244,23 @@ hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const ExtJob *
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
     hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
-                       sc, scratch, out);
+                       sc, scratch, (const uint32_t *)nullptr, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const ExtJob *jobs,
+                                    const uint32_t *out_index, uint32_t n, const ExtScoring &sc,
+                                    int2 *scratch, bwg_ext_result *out) {
+    if (n == 0) return hipSuccess;
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+                       sc, scratch, out_index, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n);
     return hipGetLastError();
 }
 
@@ -245,7 +269,17 @@ hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const KswJob *
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
     hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
-                       sc, scratch, out);
+                       sc, scratch, (const uint32_t *)nullptr, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const KswJob *jobs,
+                                    const uint32_t *out_index, uint32_t n, const KswScoring &sc,
+                                    uint8_t *scratch, bwg_ksw_result *out) {
+    if (n == 0) return hipSuccess;
+    const uint32_t bs = 256;
+    hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+                       sc, scratch, out_index, out);
     return hipGetLastError();
 }
 
