@@ -3,6 +3,8 @@
 #include "bwg_internal.h"
 
 // Odd K values (window columns per lane) the group kernel is instantiated for; 16*K columns.
+// Even K values (padded query columns per lane) the rescue group kernel is instantiated for.
+#define BWG_KSW_GROUP_KS BWG_K(2) BWG_K(4) BWG_K(6) BWG_K(8) BWG_K(10) BWG_K(12) BWG_K(14) BWG_K(16)
 #define BWG_EXT_GROUP_KS BWG_K(3) BWG_K(5) BWG_K(7) BWG_K(9) BWG_K(11) BWG_K(13) BWG_K(15) BWG_K(17) BWG_K(21) BWG_K(25) BWG_K(33)
 
 namespace bwg {
@@ -17,5 +19,12 @@ hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, con
                              const uint32_t *order, uint32_t count, uint32_t *head,
                              const ExtScoring &sc, bwg_ext_result *out, int grid);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n);
+
+int ksw_group_blocks_per_cu(int K, bool rev);
+hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const KswJob *jobs,
+                            const uint32_t *order, uint32_t count, uint32_t *head, const KswScoring &sc,
+                            bwg_ksw_result *out, int32_t *rowmax, int grid);
+hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t *list, uint32_t count,
+                              const KswScoring &sc, const int32_t *rowmax, bwg_ksw_result *out);
 
 }  // namespace bwg

@@ -1,0 +1,283 @@
+// Batched form of mate rescue (batched_ksw_align2 semantics, crates/bwa-neon/src/matesw.rs:299-497;
+// per-pass recurrence of ksw_local_fwd, crates/bwa-extend/src/sw.rs:921-1075).
+//
+// ksw_group_kernel<K, REV>: one job per 16-lane DPP row, four jobs per wavefront, persistent over a
+// device work queue.  Lane g owns the K padded query columns g*K .. g*K+K-1 and keeps H(i-1, .) and
+// E(i, .) of those columns in registers; there is no band, so the whole padded row is live every
+// row.  Within a row the gap-in-target recurrence F(j+1) = max(F(j)-e_ins, H(j)-oe_ins, 0) with
+// H(j) = max(D(j), F(j)), D(j) = max(H(i-1,j-1)+s, E(i,j), 0) reduces, because o_ins >= 0 makes
+// F(j)-oe_ins <= F(j)-e_ins, to F(j+1) = max(F(j)-e_ins, D(j)-oe_ins, 0): a max-plus recurrence
+// over D, which depends on the previous row only.  It is evaluated as in the extension kernel: a
+// per-lane pass from F = 0, an exclusive max-scan across the row, and a second pass that applies
+// the incoming F.  The row maximum with the first-column tie rule is a max over keys
+// (h << 13 | 8191-j).
+//
+// The forward pass (REV = false) records each row's maximum for the suboptimal tracker and the
+// first row/column reaching the best score; ksw_tracker_kernel then replays the row maxima through
+// the tracker.  The reverse start-recovery pass (REV = true) runs on the reversed prefixes
+// query[..=qe], target[..=te], read back to front in place, padded by its own lane rule, and stops
+// at the first row whose maximum reaches the forward score.
+#include "bwg_batched_kernels.h"
+#include "dpp.h"
+
+namespace bwg {
+namespace {
+
+constexpr int kKeyShift = 13;
+constexpr int kKeyMask = (1 << kKeyShift) - 1;
+
+template <int K, bool REV>
+__global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restrict__ arena,
+                                                        const KswJob *__restrict__ jobs,
+                                                        const uint32_t *__restrict__ order,
+                                                        uint32_t count, uint32_t *__restrict__ head,
+                                                        KswScoring sc, bwg_ksw_result *__restrict__ out,
+                                                        int32_t *__restrict__ rowmax) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane & 15;
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
+    const int32_t Ke = K * e_ins;
+
+    bool active = false, exhausted = false;
+    // per-job state
+    const uint8_t *t = nullptr;
+    int32_t tlen = 0, endsc = 0, i = 0, gmax = 0, te = -1, qe = -1, tnext = 0;
+    uint32_t idx = 0;
+    int32_t *rm = nullptr;
+    uint32_t prof[K], p4w[(K + 3) / 4];
+    int32_t km[K], kc[K], hrow[K], erow[K];
+
+    for (;;) {
+        if (!active && !exhausted) {
+            uint32_t slot = 0;
+            if (g == 0) slot = atomicAdd(head, 1u);
+            slot = __shfl(slot, lane & ~15, 64);
+            if (slot >= count) {
+                exhausted = true;
+            } else {
+                idx = order[slot];
+                const KswJob jb = jobs[idx];
+                const uint8_t *q = arena + jb.q_off;
+                t = arena + jb.t_off;
+                int32_t qlen = (int32_t)jb.qlen;
+                tlen = (int32_t)jb.tlen;
+                endsc = 0x7fffffff;
+                bool run = qlen > 0 && tlen > 0;
+                if (REV) {
+                    const bwg_ksw_result f = out[idx];
+                    run = f.score >= sc.minsc && f.qe >= 0;
+                    qlen = min(f.qe + 1, qlen);
+                    tlen = f.te + 1;
+                    endsc = f.score;
+                    te = f.te;  // forward coordinates, kept for the start computation
+                    qe = f.qe;
+                } else {
+                    rm = rowmax + jb.aux0;
+                    if (!run && g == 0) {
+                        bwg_ksw_result r;
+                        r.score = 0;
+                        r.qb = r.qe = r.tb = r.te = r.score2 = r.te2 = -1;
+                        out[idx] = r;
+                    }
+                }
+                if (run) {
+                    const int32_t qpad = ksw_padded(qlen, sc.max_sc);
+#pragma unroll
+                    for (int k = 0; k < K; ++k) {
+                        const int j = g * K + k;
+                        uint32_t p = 0;
+                        int32_t n4 = 0;
+                        if (j < qlen) {
+                            const int qc = REV ? q[qlen - 1 - j] : q[j];
+                            p = (uint32_t)(uint8_t)sc.mat[qc] | ((uint32_t)(uint8_t)sc.mat[5 + qc] << 8) |
+                                ((uint32_t)(uint8_t)sc.mat[10 + qc] << 16) | ((uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
+                            n4 = (uint8_t)sc.mat[20 + qc];
+                        }
+                        prof[k] = p;  // padding columns score 0 against every target base
+                        if ((k & 3) == 0) p4w[k >> 2] = 0;
+                        p4w[k >> 2] |= (uint32_t)n4 << (8 * (k & 3));
+                        km[k] = j < qpad ? (1 << kKeyShift) : 0;  // columns past the padded query are
+                        kc[k] = j < qpad ? kKeyMask - j : 0;      // not part of the DP: keyed out
+                        hrow[k] = 0;
+                        erow[k] = 0;
+                    }
+                    i = 0;
+                    gmax = 0;
+                    if (!REV) {
+                        te = -1;
+                        qe = -1;
+                    }
+                    tnext = REV ? t[tlen - 1] : t[0];
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active || !exhausted) == 0) break;
+        if (!active) continue;
+
+        // ---- one DP row ----------------------------------------------------------------------
+        const int32_t tc = tnext;
+        if (i + 1 < tlen) tnext = REV ? t[tlen - 2 - i] : t[i + 1];
+        const bool trow_n = tc >= 4;
+        const uint32_t tsh = (uint32_t)tc * 8u;
+        int32_t hd = row_shift_in(hrow[K - 1], 0);  // H(i-1, g*K-1); 0 left of column 0
+        int32_t dd[K], gg[K];
+        int32_t grun = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int32_t s = trow_n ? (int32_t)(int8_t)(p4w[k >> 2] >> (8 * (k & 3)))
+                                     : (int32_t)__builtin_amdgcn_sbfe(prof[k], tsh, 8);
+            const int32_t d = max(max(hd + s, erow[k]), 0);
+            hd = hrow[k];
+            dd[k] = d;
+            gg[k] = grun;
+            grun = max(max(grun - e_ins, d - oe_ins), 0);
+        }
+        const int32_t U = grun + g * Ke;
+        const int32_t incl = row_scan_max_nonneg(U);
+        const int32_t excl = row_shift_in(incl, -Ke);
+        const int32_t f_in = excl - (g - 1) * Ke;
+        int32_t key = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            const int32_t f = max(f_in - k * e_ins, gg[k]);
+            const int32_t h = max(dd[k], f);
+            key = max(key, (int32_t)(__umul24((uint32_t)h, (uint32_t)km[k]) + (uint32_t)kc[k]));
+            erow[k] = max(max(erow[k] - e_del, h - oe_del), 0);
+            hrow[k] = h;
+        }
+        key = row_allmax(key);
+        const int32_t imax = key >> kKeyShift;
+        if (!REV && g == 0) rm[i] = imax;
+        bool done = false;
+        if (imax > gmax) {
+            gmax = imax;
+            if (REV) {
+                if (gmax >= endsc) {
+                    // first row reaching the forward score: the alignment starts rte/rqe back
+                    if (g == 0 && gmax == endsc) {
+                        const int32_t rqe = kKeyMask - (key & kKeyMask);
+                        out[idx].tb = te - i;
+                        out[idx].qb = qe - rqe;
+                    }
+                    done = true;
+                }
+            } else {
+                te = i;
+                qe = kKeyMask - (key & kKeyMask);
+            }
+        }
+        ++i;
+        if (i >= tlen) done = true;
+        if (done) {
+            if (!REV && g == 0) {
+                bwg_ksw_result r;
+                r.score = gmax;
+                r.qb = -1;
+                r.qe = te >= 0 ? qe : -1;
+                r.tb = -1;
+                r.te = te;
+                r.score2 = -1;
+                r.te2 = -1;
+                out[idx] = r;
+            }
+            active = false;
+        }
+    }
+}
+
+// score2/te2 from the recorded row maxima (crates/bwa-extend/src/sw.rs:806-851): replay
+// push_row's merge rule and keep the best closed entry outside the exclusion window around te.
+__global__ __launch_bounds__(256) void ksw_tracker_kernel(const KswJob *__restrict__ jobs,
+                                                          const uint32_t *__restrict__ list,
+                                                          uint32_t count, KswScoring sc,
+                                                          const int32_t *__restrict__ rowmax,
+                                                          bwg_ksw_result *__restrict__ out) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k >= count) return;
+    const uint32_t idx = list[k];
+    const KswJob jb = jobs[idx];
+    bwg_ksw_result r = out[idx];
+    const int32_t gmax = r.score, te = r.te;
+    int32_t s2 = -1, t2 = -1;
+    if (gmax > 0 && jb.qlen > 0) {
+        const int32_t *rm = rowmax + jb.aux0;
+        const int32_t half = (gmax + sc.max_sc - 1) / sc.max_sc;
+        const int32_t lo = te - half, hi = te + half;
+        int32_t cs = 0, cr = -1;
+        bool open = false;
+        for (int32_t i = 0; i < (int32_t)jb.tlen; ++i) {
+            const int32_t v = rm[i];
+            if (v < sc.minsc) continue;
+            if (open && cr + 1 == i) {
+                if (cs < v) {
+                    cs = v;
+                    cr = i;
+                }
+                continue;
+            }
+            if (open && (cr < lo || cr > hi) && cs > s2) {
+                s2 = cs;
+                t2 = cr;
+            }
+            cs = v;
+            cr = i;
+            open = true;
+        }
+        if (open && (cr < lo || cr > hi) && cs > s2) {
+            s2 = cs;
+            t2 = cr;
+        }
+    }
+    out[idx].score2 = s2;
+    out[idx].te2 = t2;
+}
+
+template <int K, bool REV>
+int occ() {
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ksw_group_kernel<K, REV>, 256, 0) != hipSuccess)
+        return 1;
+    return blocks > 0 ? blocks : 1;
+}
+
+}  // namespace
+
+int ksw_group_blocks_per_cu(int K, bool rev) {
+    switch (K) {
+#define BWG_K(k) case k: return rev ? occ<k, true>() : occ<k, false>();
+        BWG_KSW_GROUP_KS
+#undef BWG_K
+    default: return 1;
+    }
+}
+
+hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const KswJob *jobs,
+                            const uint32_t *order, uint32_t count, uint32_t *head, const KswScoring &sc,
+                            bwg_ksw_result *out, int32_t *rowmax, int grid) {
+    switch (K) {
+#define BWG_K(k)                                                                                          \
+    case k:                                                                                               \
+        if (rev)                                                                                          \
+            hipLaunchKernelGGL((ksw_group_kernel<k, true>), dim3(grid), dim3(256), 0, st, arena, jobs,    \
+                               order, count, head, sc, out, rowmax);                                      \
+        else                                                                                              \
+            hipLaunchKernelGGL((ksw_group_kernel<k, false>), dim3(grid), dim3(256), 0, st, arena, jobs,   \
+                               order, count, head, sc, out, rowmax);                                      \
+        return hipGetLastError();
+        BWG_KSW_GROUP_KS
+#undef BWG_K
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t *list, uint32_t count,
+                              const KswScoring &sc, const int32_t *rowmax, bwg_ksw_result *out) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(ksw_tracker_kernel, dim3((count + 255) / 256), dim3(256), 0, st, jobs, list, count,
+                       sc, rowmax, out);
+    return hipGetLastError();
+}
+
+}  // namespace bwg
