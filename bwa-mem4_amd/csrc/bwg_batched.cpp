@@ -25,7 +25,8 @@ enum BinKind { kThread = 0, kGroup = 1, kPlain = 2 };
 struct Bin {
     int kind = kGroup;
     int param = 0;  // thread: qcap; group: K
-    int qcap = 0;   // group: LDS sizing (max qlen in the bin)
+    int qcap = 0;   // group: bin key part (0: one bin per K)
+    int qcap_max = 0;  // group: LDS sizing, the longest query of the bin rounded up to 32
     uint32_t offset = 0, count = 0;
     double work = 0;
 };
@@ -46,11 +47,7 @@ int odd_k_for(int width) {
     return -1;
 }
 
-int qcap_class(int q) {
-    int c = 64;
-    while (c < q) c <<= 1;
-    return c;
-}
+int qcap_class(int q) { return std::max(32, (q + 31) / 32 * 32); }
 
 ExtPlan *build_ext_plan(bwg_ctx *c, bwg_batch *b, int *rc) {
     const size_t n = b->n;
@@ -85,7 +82,8 @@ ExtPlan *build_ext_plan(bwg_ctx *c, bwg_batch *b, int *rc) {
             const int K = odd_k_for(width);
             const int qc = qcap_class(q);
             if (K < 0 || qc > tu.ext_group_max_qcap) bi = find_bin(kPlain, 0, 0);
-            else bi = find_bin(kGroup, K, qc);
+            else bi = find_bin(kGroup, K, 0);
+            if (bi >= 0 && bins[bi].kind == kGroup) bins[bi].qcap_max = std::max(bins[bi].qcap_max, qc);
         }
         key[k] = bi;
         bins[bi].count++;
@@ -180,13 +178,13 @@ int run_batched(bwg_ctx *c, bwg_batch *b, BatchedTiming *t) {
                 e = hipEventRecord(c->ev[2], c->stream);
                 main_open = true;
             }
-            const int per_cu = ext_group_blocks_per_cu(bn.param, bn.qcap);
+            const int per_cu = ext_group_blocks_per_cu(bn.param, bn.qcap_max);
             const uint32_t groups_needed = bn.count;
             int grid = std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ext_group_oversub, (groups_needed + 15) / 16);
             grid = std::max(grid, 1);
             if (e == hipSuccess)
                 e = launch_ext_group(c->stream, bn.param, b->d_arena, (const ExtJob *)b->d_jobs,
-                                     p->d_order + bn.offset, bn.count, p->d_heads + r, b->esc, bn.qcap,
+                                     p->d_order + bn.offset, bn.count, p->d_heads + r, b->esc, bn.qcap_max,
                                      out, grid);
             ++launches;
             ++main_launches;

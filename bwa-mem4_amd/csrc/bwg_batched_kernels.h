@@ -9,7 +9,7 @@
 
 namespace bwg {
 
-int ext_group_words(int qcap);
+size_t ext_group_lds_bytes(int K, int qcap);
 int ext_group_blocks_per_cu(int K, int qcap);
 hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const ExtJob *jobs,
                             const uint32_t *order, uint32_t count, uint32_t *head,

@@ -107,20 +107,22 @@ KswPlan *build_plan(bwg_ctx *c, bwg_batch *b, int *rc) {
         *rc = bwg_map_hip(e);
         return p;
     }
-    // row-maximum slots for the fast-path jobs (indexed by job), scratch for the fallback
+    // Row-maximum slots for the fast-path jobs: tiles of 64 consecutive jobs of the order array,
+    // row-major within a tile ([row][64]), so the tracker kernel's 64 lanes read consecutive words.
+    // A job's slot for row i is aux0 + 64*i.
     uint64_t rows = 0;
     std::vector<KswJob> plain;
     for (auto &bn : p->bins) {
-        for (uint32_t r = 0; r < bn.count; ++r) {
-            const uint32_t j = order[bn.offset + r];
-            if (bn.K) {
-                all[j].aux0 = (uint32_t)rows;
-                rows += all[j].tlen;
-            } else {
-                plain.push_back(all[j]);
-            }
-        }
         if (bn.K) p->n_fast += bn.count;
+        else
+            for (uint32_t r = 0; r < bn.count; ++r) plain.push_back(all[order[bn.offset + r]]);
+    }
+    for (uint32_t t0 = 0; t0 < p->n_fast; t0 += 64) {
+        uint32_t tmax = 0;
+        const uint32_t t1 = std::min<uint32_t>(p->n_fast, t0 + 64);
+        for (uint32_t r = t0; r < t1; ++r) tmax = std::max(tmax, all[order[r]].tlen);
+        for (uint32_t r = t0; r < t1; ++r) all[order[r]].aux0 = (uint32_t)(rows + (r - t0));
+        rows += 64ull * tmax;
     }
     if (rows >= (1ull << 32)) {
         *rc = BWG_E2BIG;

@@ -35,10 +35,18 @@ struct GroupJob {
     int32_t tcur;  // target code of row i
 };
 
-// Per-group LDS layout, in 32-bit words: eh pairs {H[j], E[j]} for j in 0..qcap (+1 spare), then
-// the query profile prof[j] (scores of target codes 0..3 against query[j], one byte each) and the
-// N-row scores prof4[j] (mat[4][query[j]], one byte each, packed four per word).
-__host__ __device__ __forceinline__ int group_words(int qcap) { return 2 * (qcap + 2) + (qcap + 1) + (qcap + 4) / 4; }
+// Block LDS layout (bytes): the eh pairs {H[j], E[j]} of every group, (qcap+2) columns each; then
+// every group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes);
+// then slack.  A lane always loads all K columns of its block, including columns past the window
+// (their values are not used); those loads may run past the group's own region into the next
+// one, and the slack keeps the last group's loads inside the allocation.
+__host__ __device__ __forceinline__ int eh_bytes(int qcap) { return 8 * (qcap + 2); }
+__host__ __device__ __forceinline__ int plane_bytes(int qcap) { return 5 * qcap; }
+__host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap, int K) {
+    return (size_t)groups * (eh_bytes(qcap) + plane_bytes(qcap)) + 16 * K * 8 + 64;
+}
+
+constexpr int kGroupsPerBlock = 16;
 
 template <int K>
 __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restrict__ arena,
@@ -49,12 +57,10 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                                                         bwg_ext_result *__restrict__ out) {
     extern __shared__ int32_t lds[];
     const int lane = threadIdx.x & 63;
-    const int g = lane & 15;  // lane within the group
-    const int gid = (threadIdx.x >> 4);  // group within the block
-    const int gw = group_words(qcap);
-    int32_t *eh = lds + gid * gw;                   // {H, E} pairs
-    uint32_t *prof = (uint32_t *)(eh + 2 * (qcap + 2));
-    int8_t *prof4 = (int8_t *)(prof + (qcap + 1));
+    const int g = lane & 15;             // lane within the group
+    const int gid = threadIdx.x >> 4;    // group within the block
+    int32_t *eh = (int32_t *)((char *)lds + gid * eh_bytes(qcap));
+    int8_t *planes = (int8_t *)lds + kGroupsPerBlock * eh_bytes(qcap) + gid * plane_bytes(qcap);
 
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
@@ -81,17 +87,13 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 J.w = clamp_band(sc.w, J.qlen, sc);
                 // row -1 (sw.rs:263-279): H[0] = h0, H[j] = max(h0 - oe_ins - (j-1)*e_ins, 0)
                 for (int j = g; j <= J.qlen + 1; j += 16) {
-                    int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
-                    eh[2 * j] = j <= J.qlen ? h : 0;
-                    eh[2 * j + 1] = 0;
+                    const int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
+                    *(int2 *)(eh + 2 * j) = make_int2(j <= J.qlen ? h : 0, 0);
                 }
                 for (int j = g; j < J.qlen; j += 16) {
                     const int qc = q[j];
-                    uint32_t p = (uint32_t)(uint8_t)sc.mat[qc] | ((uint32_t)(uint8_t)sc.mat[5 + qc] << 8) |
-                                 ((uint32_t)(uint8_t)sc.mat[10 + qc] << 16) |
-                                 ((uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
-                    prof[j] = p;
-                    prof4[j] = sc.mat[20 + qc];
+#pragma unroll
+                    for (int c = 0; c < 5; ++c) planes[c * qcap + j] = sc.mat[c * 5 + qc];
                 }
                 J.i = 0;
                 J.beg = 0;
@@ -119,29 +121,29 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         if (end > i + J.w + 1) end = i + J.w + 1;
         if (end > J.qlen) end = J.qlen;
         const int32_t h1_init = beg == 0 ? max(J.h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
-        const int32_t tcode = J.tcur;
         const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
-        const bool trow_n = tcode >= 4;
-        const uint32_t tsh = (uint32_t)tcode * 8u;
 
-        const int32_t base = beg + g * K;   // first window column of this lane
+        const int32_t base = beg + g * K;  // first window column of this lane
         const int32_t nval = min(max(end - base, 0), K);
-        int32_t mm[K], ee[K], gg[K];
+        const int2 *ehb = (const int2 *)eh + base;
+        const int8_t *sb = planes + J.tcur * qcap + base;
+        // pass 1: loads of all K columns are unconditional and independent, so they issue back to
+        // back; columns past the window compute values that are never used
+        int2 v[K];
+        int32_t s[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+            v[k] = ehb[k];
+            s[k] = sb[k];
+        }
+        int32_t mm[K], gg[K];
         int32_t grun = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            mm[k] = 0;
-            ee[k] = 0;
+            const int32_t m = v[k].x != 0 ? v[k].x + s[k] : 0;
+            mm[k] = m;
             gg[k] = grun;
-            if (k < nval) {
-                const int j = base + k;
-                const int2 v = *(const int2 *)(eh + 2 * j);
-                const int32_t s = trow_n ? (int32_t)prof4[j] : (int32_t)__builtin_amdgcn_sbfe(prof[j], tsh, 8);
-                const int32_t m = v.x != 0 ? v.x + s : 0;
-                mm[k] = m;
-                ee[k] = v.y;
-                grun = max(grun - e_ins, max(m - oe_ins, 0));
-            }
+            grun = max(max(grun - e_ins, m - oe_ins), 0);
         }
         // F entering this lane's block: exclusive max-scan of U_l = c_l + l*K*e_ins over the row
         const int32_t U = grun + g * Ke;
@@ -150,27 +152,29 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         const int32_t f_in = excl - (g - 1) * Ke;
 
         int32_t key = 0;
+        int2 *ehw = (int2 *)eh + base;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
+            const int32_t f = max(f_in - k * e_ins, gg[k]);
+            const int32_t h = max(max(mm[k], v[k].y), f);
+            const int32_t e = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
             if (k < nval) {
-                const int j = base + k;
-                const int32_t f = max(f_in - k * e_ins, gg[k]);
-                const int32_t h = max(max(mm[k], ee[k]), f);
-                key = max(key, (h << kKeyShift) | j);
-                const int32_t e = max(max(ee[k] - e_del, mm[k] - oe_del), 0);
-                eh[2 * j + 1] = e;      // E(i+1, j)
-                eh[2 * j + 2] = h;      // H(i, j) is the diagonal of column j+1 next row
+                key = max(key, (h << kKeyShift) | (base + k));
+                ehw[k].y = e;       // E(i+1, j)
+                ehw[k + 1].x = h;   // H(i, j): the diagonal of column j+1 next row
             }
         }
         if (g == 0) {
-            eh[2 * beg] = h1_init;      // H(i, beg-1)
-            eh[2 * end + 1] = 0;        // E at the sentinel column
+            eh[2 * beg] = h1_init;  // H(i, beg-1)
+            eh[2 * end + 1] = 0;    // E at the sentinel column
         }
         wave_lds_sync();
         key = row_allmax(key);
         const int32_t row_max = key >> kKeyShift;
         const int32_t mj = key & kKeyMask;
-        const int32_t h1_last = eh[2 * end];  // H(i, end-1), or h1_init for an empty window
+        const int2 at_end = *(const int2 *)(eh + 2 * end);  // H(i, end-1) (h1_init if empty)
+        const int2 at_beg = *(const int2 *)(eh + 2 * beg);
+        const int32_t h1_last = at_end.x;
 
         bool done = false;
         const int32_t j_exit = beg < end ? end : beg;
@@ -195,26 +199,36 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         }
         if (!done) {
             // band tightening (sw.rs:464-480): first live column in [beg, end), last live column
-            // scanning down from end; 16 columns per probe
+            // scanning down from end.  The edges are usually live, so test them first.
             int32_t nb = beg;
-            for (;;) {
-                const int j = nb + g;
-                bool live = false;
-                if (j < end) live = (eh[2 * j] | eh[2 * j + 1]) != 0;
-                const int32_t cand = row_allmin(live ? j : 0x7fffffff);
-                if (cand != 0x7fffffff) { nb = cand; break; }
-                nb += 16;
-                if (nb >= end) { nb = end; break; }
+            if (beg < end && (at_beg.x | at_beg.y) == 0) {
+                for (;;) {
+                    const int j = nb + g;
+                    bool live = false;
+                    if (j < end) {
+                        const int2 x = *(const int2 *)(eh + 2 * j);
+                        live = (x.x | x.y) != 0;
+                    }
+                    const int32_t cand = row_allmin(live ? j : 0x7fffffff);
+                    if (cand != 0x7fffffff) { nb = cand; break; }
+                    nb += 16;
+                    if (nb >= end) { nb = end; break; }
+                }
             }
             int32_t ne = end;
-            for (;;) {
-                const int j = ne - g;
-                bool live = false;
-                if (j >= nb) live = (eh[2 * j] | eh[2 * j + 1]) != 0;
-                const int32_t cand = row_allmax(live ? j : -1);
-                if (cand >= 0) { ne = cand; break; }
-                ne -= 16;
-                if (ne < nb) { ne = nb - 1; break; }
+            if ((at_end.x | at_end.y) == 0) {
+                for (;;) {
+                    const int j = ne - g;
+                    bool live = false;
+                    if (j >= nb) {
+                        const int2 x = *(const int2 *)(eh + 2 * j);
+                        live = (x.x | x.y) != 0;
+                    }
+                    const int32_t cand = row_allmax(live ? j : -1);
+                    if (cand >= 0) { ne = cand; break; }
+                    ne -= 16;
+                    if (ne < nb) { ne = nb - 1; break; }
+                }
             }
             J.beg = nb;
             J.end = ne + 2 < J.qlen ? ne + 2 : J.qlen;
@@ -246,9 +260,9 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                                                         uint32_t count, uint32_t *__restrict__ head,
                                                         ExtScoring sc,
                                                         bwg_ext_result *__restrict__ out) {
+    // [column][lane] layouts: lanes of a wavefront touch consecutive words
     __shared__ int2 eh[(QCAP + 1) * 64];
-    __shared__ uint32_t prof[QCAP * 64];
-    __shared__ int8_t prof4[QCAP * 64];
+    __shared__ int8_t planes[5 * QCAP * 64];  // planes[(c*QCAP + j)*64 + lane] = mat[c][query[j]]
     const int lane = threadIdx.x;
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
@@ -269,19 +283,15 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                 eh[j * 64 + lane] = make_int2(j == 0 ? h0 : max(h0 - oe_ins - (j - 1) * e_ins, 0), 0);
             for (int j = 0; j < qlen; ++j) {
                 const int qc = q[j];
-                prof[j * 64 + lane] = (uint32_t)(uint8_t)sc.mat[qc] | ((uint32_t)(uint8_t)sc.mat[5 + qc] << 8) |
-                                      ((uint32_t)(uint8_t)sc.mat[10 + qc] << 16) |
-                                      ((uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
-                prof4[j * 64 + lane] = sc.mat[20 + qc];
+#pragma unroll
+                for (int c = 0; c < 5; ++c) planes[(c * QCAP + j) * 64 + lane] = sc.mat[c * 5 + qc];
             }
             int32_t best = h0, best_i = -1, best_j = -1, gbest_i = -1, gscore = -1, max_off = 0;
             int32_t beg = 0, end = qlen;
             int32_t tnext = tlen > 0 ? t[0] : 0;
             for (int32_t i = 0; i < tlen; ++i) {
-                const int32_t tcode = tnext;
+                const int8_t *srow = planes + tnext * QCAP * 64 + lane;
                 if (i + 1 < tlen) tnext = t[i + 1];
-                const bool trow_n = tcode >= 4;
-                const uint32_t tsh = (uint32_t)tcode * 8u;
                 int32_t f = 0, key = 0;
                 if (beg < i - w) beg = i - w;
                 if (end > i + w + 1) end = i + w + 1;
@@ -290,8 +300,7 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                 int32_t j = beg;
                 for (; j < end; ++j) {
                     const int2 v = eh[j * 64 + lane];
-                    const int32_t s = trow_n ? (int32_t)prof4[j * 64 + lane]
-                                             : (int32_t)__builtin_amdgcn_sbfe(prof[j * 64 + lane], tsh, 8);
+                    const int32_t s = srow[j * 64];
                     const int32_t m = v.x != 0 ? v.x + s : 0;
                     const int32_t h = max(max(m, v.y), f);
                     const int32_t e = max(max(v.y - e_del, m - oe_del), 0);
@@ -322,15 +331,15 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                 }
                 int32_t a = beg;
                 while (a < end) {
-                    const int2 v = eh[a * 64 + lane];
-                    if ((v.x | v.y) != 0) break;
+                    const int2 x = eh[a * 64 + lane];
+                    if ((x.x | x.y) != 0) break;
                     ++a;
                 }
                 beg = a;
                 int32_t z = end;
                 while (z >= beg) {
-                    const int2 v = eh[z * 64 + lane];
-                    if ((v.x | v.y) != 0) break;
+                    const int2 x = eh[z * 64 + lane];
+                    if ((x.x | x.y) != 0) break;
                     --z;
                 }
                 end = z + 2 < qlen ? z + 2 : qlen;
@@ -351,7 +360,7 @@ template <int K>
 hipError_t launch_group(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, const uint32_t *order,
                         uint32_t count, uint32_t *head, const ExtScoring &sc, int qcap,
                         bwg_ext_result *out, int grid) {
-    const size_t lds = (size_t)16 * group_words(qcap) * sizeof(int32_t);  // 16 groups per block
+    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
     hipLaunchKernelGGL(ext_group_kernel<K>, dim3(grid), dim3(256), lds, st, arena, jobs, order, count,
                        head, sc, qcap, out);
     return hipGetLastError();
@@ -360,7 +369,7 @@ hipError_t launch_group(hipStream_t st, const uint8_t *arena, const ExtJob *jobs
 template <int K>
 int group_occupancy(int qcap) {
     int blocks = 0;
-    const size_t lds = (size_t)16 * group_words(qcap) * sizeof(int32_t);
+    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K>, 256, lds) != hipSuccess)
         return 1;
     return blocks > 0 ? blocks : 1;
@@ -376,7 +385,7 @@ int thread_occupancy() {
 
 }  // namespace
 
-int ext_group_words(int qcap) { return 2 * (qcap + 2) + (qcap + 1) + (qcap + 4) / 4; }
+size_t ext_group_lds_bytes(int K, int qcap) { return block_lds_bytes(kGroupsPerBlock, qcap, K); }
 
 int ext_group_blocks_per_cu(int K, int qcap) {
     switch (K) {

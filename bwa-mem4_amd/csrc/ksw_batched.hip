@@ -149,7 +149,7 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
         }
         key = row_allmax(key);
         const int32_t imax = key >> kKeyShift;
-        if (!REV && g == 0) rm[i] = imax;
+        if (!REV && g == 0) rm[64 * i] = imax;
         bool done = false;
         if (imax > gmax) {
             gmax = imax;
@@ -208,7 +208,7 @@ __global__ __launch_bounds__(256) void ksw_tracker_kernel(const KswJob *__restri
         int32_t cs = 0, cr = -1;
         bool open = false;
         for (int32_t i = 0; i < (int32_t)jb.tlen; ++i) {
-            const int32_t v = rm[i];
+            const int32_t v = rm[64 * i];
             if (v < sc.minsc) continue;
             if (open && cr + 1 == i) {
                 if (cs < v) {
