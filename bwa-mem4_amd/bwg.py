@@ -31,7 +31,7 @@ KSW_DTYPE = np.dtype([(n, "<i4") for n in KswAlignResult._fields])
 
 BWG_OK, BWG_EIO, BWG_E2BIG, BWG_EAGAIN, BWG_ENOMEM, BWG_ENODEV, BWG_EINVAL = 0, -5, -7, -11, -12, -19, -22
 FORM_PLAIN, FORM_BATCHED = 0, 1
-OPT_FORM = 1
+OPT_FORM, OPT_CONCURRENT_BINS, OPT_GRAPHS, OPT_EXT_THREAD_MAX_Q = 1, 2, 3, 4
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 
 # Every symbol include/bwg.h declares.
@@ -202,6 +202,9 @@ class Context:
 
     def set_form(self, form):
         _check(_lib().bwg_set_option(self.h, OPT_FORM, int(form)), "bwg_set_option(form)")
+
+    def set_option(self, key, value):
+        _check(_lib().bwg_set_option(self.h, int(key), int(value)), "bwg_set_option(%d)" % key)
 
     def stat(self, key):
         return int(_lib().bwg_get_stat(self.h, key))

@@ -92,15 +92,10 @@ int bwg_create(int device, bwg_ctx **out) {
     if (!c) return BWG_ENOMEM;
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
-        delete c;
+    if (hipSetDevice(device) != hipSuccess || c->ss.create() != BWG_OK) {
+        bwg_destroy(c);
         return BWG_EIO;
     }
-    for (auto &e : c->ev)
-        if (hipEventCreate(&e) != hipSuccess) {
-            bwg_destroy(c);
-            return BWG_EIO;
-        }
     *out = c;
     return BWG_OK;
 }
@@ -108,11 +103,8 @@ int bwg_create(int device, bwg_ctx **out) {
 void bwg_destroy(bwg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    c->ss.destroy();
     c->pool.release_all();
-    for (auto &e : c->ev)
-        if (e) (void)hipEventDestroy(e);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 
@@ -125,6 +117,17 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
         if (v != BWG_FORM_PLAIN && v != BWG_FORM_BATCHED) return BWG_EINVAL;
         c->form = (int)v;
         return BWG_OK;
+    case BWG_OPT_CONCURRENT_BINS:
+        c->tuning.concurrent_bins = v != 0;
+        return BWG_OK;
+    case BWG_OPT_GRAPHS:
+        c->tuning.use_graphs = v != 0;
+        return BWG_OK;
+    case BWG_OPT_EXT_THREAD_MAX_Q:
+        if (v < 0 || v > 32) return BWG_EINVAL;
+        c->tuning.ext_thread_max_q16 = (int)std::min<int64_t>(v, 16);
+        c->tuning.ext_thread_max_q32 = (int)v;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -133,6 +136,9 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     if (!c) return -1;
     switch (key) {
     case BWG_OPT_FORM: return c->form;
+    case BWG_OPT_CONCURRENT_BINS: return c->tuning.concurrent_bins;
+    case BWG_OPT_GRAPHS: return c->tuning.use_graphs;
+    case BWG_OPT_EXT_THREAD_MAX_Q: return c->tuning.ext_thread_max_q32;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;
@@ -165,7 +171,11 @@ int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uin
     sc.end_bonus = p->end_bonus;
     sc.zdrop = p->zdrop;
     sc.max_sc = mat_max(p->mat);
-    return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, out);
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, c->ss.main, out);
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));  // the caller may free its buffers
+    return rc;
 }
 
 int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
@@ -186,7 +196,11 @@ int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uin
     sc.e_ins = p->e_ins;
     sc.minsc = p->minsc;
     sc.max_sc = p->max_sc;
-    return stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, out);
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    rc = stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, c->ss.main, out);
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
+    return rc;
 }
 
 int bwg_batch_run(bwg_ctx *c, bwg_batch *b) {

@@ -11,17 +11,17 @@ namespace bwg {
 
 size_t ext_group_lds_bytes(int K, int qcap);
 int ext_group_blocks_per_cu(int K, int qcap);
-hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const ExtJob *jobs,
+hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                             const uint32_t *order, uint32_t count, uint32_t *head,
                             const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid);
 int ext_thread_blocks_per_cu(int qcap);
-hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const ExtJob *jobs,
+hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                              const uint32_t *order, uint32_t count, uint32_t *head,
                              const ExtScoring &sc, bwg_ext_result *out, int grid);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n);
 
 int ksw_group_blocks_per_cu(int K, bool rev);
-hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const KswJob *jobs,
+hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                             const uint32_t *order, uint32_t count, uint32_t *head, const KswScoring &sc,
                             bwg_ksw_result *out, int32_t *rowmax, int grid);
 hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t *list, uint32_t count,

@@ -9,9 +9,36 @@
 
 namespace bwg {
 
+// Target location: t_off indexes the batch arena, or, with kTFromRef set, a device-resident
+// reference; with kTReverse set the target is that window read back to front (a left extension's
+// target, or a reverse-strand window), i.e. target[i] = base[off + tlen - 1 - i].
+constexpr uint64_t kTFromRef = 1ull << 62;
+constexpr uint64_t kTReverse = 1ull << 63;
+constexpr uint64_t kTOffMask = kTFromRef - 1;
+
+struct TargetView {
+    const uint8_t *p;
+    int32_t step;
+    __host__ __device__ uint8_t operator[](int32_t i) const { return p[(ptrdiff_t)i * step]; }
+};
+__host__ __device__ inline TargetView target_view(const uint8_t *arena, const uint8_t *ref,
+                                                  uint64_t t_off, uint32_t tlen) {
+    const uint8_t *base = (t_off & kTFromRef) ? ref : arena;
+    const uint64_t o = t_off & kTOffMask;
+    TargetView v;
+    if (t_off & kTReverse) {
+        v.p = base + o + (tlen ? tlen - 1 : 0);
+        v.step = -1;
+    } else {
+        v.p = base + o;
+        v.step = 1;
+    }
+    return v;
+}
+
 // Job descriptor as the kernels read it: 32 bytes, one load per job.
 struct ExtJob {
-    uint64_t q_off, t_off;  // into the device arena
+    uint64_t q_off, t_off;  // q_off into the device arena; t_off as described above
     uint32_t qlen, tlen;
     int32_t h0;
     uint32_t aux;  // plain form: scratch slot offset in int2 units / 8; batched: original index
@@ -57,16 +84,16 @@ __host__ __device__ inline int32_t ksw_padded(int32_t qlen, int32_t max_sc) {
 
 // ---- launchers (defined in the .hip files) ----
 // Plain form: one thread per job, state in global memory.
-hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, uint32_t n,
-                            const ExtScoring &sc, int2 *scratch, bwg_ext_result *out);
-hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const KswJob *jobs, uint32_t n,
-                            const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out);
+hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
+                            uint32_t n, const ExtScoring &sc, int2 *scratch, bwg_ext_result *out);
+hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
+                            uint32_t n, const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out);
 
 // Same kernels over a compacted job list whose k-th result goes to out[out_index[k]].
-hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const ExtJob *jobs,
+hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const ExtScoring &sc,
                                     int2 *scratch, bwg_ext_result *out);
-hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const KswJob *jobs,
+hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const KswScoring &sc,
                                     uint8_t *scratch, bwg_ksw_result *out);
 

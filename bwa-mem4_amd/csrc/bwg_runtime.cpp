@@ -1,11 +1,10 @@
-// Host runtime: device memory pool, batch staging, launch planning and result copies.
+// Host runtime: device memory pool, stream sets, batch staging, the plain form's launches, result
+// copies.  The batched form's planning and launches are in bwg_batched.cpp.
 #include "bwg_runtime.h"
 
 #include <algorithm>
 #include <cstring>
 #include <new>
-
-#include "bwg_batched.h"
 
 namespace bwg {
 
@@ -21,12 +20,12 @@ size_t DevicePool::round(size_t b) {
 }
 
 void *DevicePool::alloc(size_t bytes) {
-    size_t sz = round(bytes);
+    const size_t sz = round(bytes);
     std::lock_guard<std::mutex> g(mu_);
     auto it = free_.lower_bound(sz);
     if (it != free_.end() && it->first <= sz + (sz >> 1)) {
         void *p = it->second;
-        size_t got = it->first;
+        const size_t got = it->first;
         free_.erase(it);
         cached_ -= got;
         live_[p] = got;
@@ -71,20 +70,63 @@ void DevicePool::release_all() {
 }
 
 // ---------------------------------------------------------------------------------------------
+// StreamSet
+// ---------------------------------------------------------------------------------------------
+int StreamSet::create() {
+    if (hipStreamCreateWithFlags(&main, hipStreamNonBlocking) != hipSuccess) return BWG_EIO;
+    if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return BWG_EIO;
+    for (int k = 0; k < kAuxStreams; ++k) {
+        if (hipStreamCreateWithFlags(&aux[k], hipStreamNonBlocking) != hipSuccess) return BWG_EIO;
+        if (hipEventCreateWithFlags(&join[k], hipEventDisableTiming) != hipSuccess) return BWG_EIO;
+    }
+    return BWG_OK;
+}
+
+void StreamSet::destroy() {
+    if (main) (void)hipStreamSynchronize(main);
+    for (int k = 0; k < kAuxStreams; ++k) {
+        if (aux[k]) {
+            (void)hipStreamSynchronize(aux[k]);
+            (void)hipStreamDestroy(aux[k]);
+        }
+        if (join[k]) (void)hipEventDestroy(join[k]);
+        aux[k] = nullptr;
+        join[k] = nullptr;
+    }
+    if (fork) (void)hipEventDestroy(fork);
+    if (main) (void)hipStreamDestroy(main);
+    main = nullptr;
+    fork = nullptr;
+}
+
+// ---------------------------------------------------------------------------------------------
 // staging
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-int stage_common(bwg_ctx *c, bwg_batch *b, const uint8_t *arena, size_t arena_len, size_t job_bytes,
-                 const void *jobs_host, size_t out_bytes) {
+bwg_batch *new_batch(int kind, size_t n) {
+    bwg_batch *b = new (std::nothrow) bwg_batch();
+    if (!b) return nullptr;
+    b->kind = kind;
+    b->n = n;
+    if (hipEventCreate(&b->ev_start) != hipSuccess || hipEventCreate(&b->ev_end) != hipSuccess) {
+        if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+        delete b;
+        return nullptr;
+    }
+    return b;
+}
+
+int stage_buffers(bwg_ctx *c, bwg_batch *b, const uint8_t *arena, size_t arena_len, const void *jobs,
+                  size_t job_bytes, size_t out_bytes, hipStream_t st) {
+    b->arena_len = arena_len;
     b->d_arena = (uint8_t *)c->pool.alloc(arena_len + 64);
-    b->d_jobs = c->pool.alloc(job_bytes);
-    b->d_out = c->pool.alloc(out_bytes);
+    b->d_jobs = c->pool.alloc(std::max<size_t>(job_bytes, 64));
+    b->d_out = c->pool.alloc(std::max<size_t>(out_bytes, 64));
     if (!b->d_arena || !b->d_jobs || !b->d_out) return BWG_ENOMEM;
-    hipError_t e = hipMemcpyAsync(b->d_arena, arena, arena_len, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess)
-        e = hipMemcpyAsync(b->d_jobs, jobs_host, job_bytes, hipMemcpyHostToDevice, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = hipSuccess;
+    if (arena_len) e = hipMemcpyAsync(b->d_arena, arena, arena_len, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && job_bytes) e = hipMemcpyAsync(b->d_jobs, jobs, job_bytes, hipMemcpyHostToDevice, st);
     return bwg_map_hip(e);
 }
 
@@ -92,29 +134,25 @@ int stage_common(bwg_ctx *c, bwg_batch *b, const uint8_t *arena, size_t arena_le
 
 int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, const int32_t *h0, bwg_batch **out) {
+              const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
+              bwg_batch **out) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    bwg_batch *b = new (std::nothrow) bwg_batch();
+    bwg_batch *b = new_batch(0, n);
     if (!b) return BWG_ENOMEM;
-    b->kind = 0;
-    b->n = n;
     b->esc = sc;
-    b->arena_len = arena_len;
-    b->q_len.assign(q_len, q_len + n);
-    b->t_len.assign(t_len, t_len + n);
-    b->h0.assign(h0, h0 + n);
-    std::vector<ExtJob> jobs(n);
+    b->d_ref = ref;
+    b->hext.resize(n);
     for (size_t k = 0; k < n; ++k) {
-        jobs[k].q_off = q_off[k];
-        jobs[k].t_off = t_off[k];
-        jobs[k].qlen = q_len[k];
-        jobs[k].tlen = t_len[k];
-        jobs[k].h0 = h0[k];
-        jobs[k].aux = (uint32_t)k;
+        ExtJob &j = b->hext[k];
+        j.q_off = q_off[k];
+        j.t_off = t_off[k];
+        j.qlen = q_len[k];
+        j.tlen = t_len[k];
+        j.h0 = h0[k];
+        j.aux = (uint32_t)k;
     }
-    std::lock_guard<std::mutex> g(c->run_mu);
-    int rc = stage_common(c, b, arena, arena_len, std::max<size_t>(1, n) * sizeof(ExtJob),
-                          jobs.data(), std::max<size_t>(1, n) * sizeof(bwg_ext_result));
+    int rc = stage_buffers(c, b, arena, arena_len, b->hext.data(), n * sizeof(ExtJob),
+                           n * sizeof(bwg_ext_result), st);
     if (rc) {
         destroy_batch(c, b);
         return rc;
@@ -125,28 +163,24 @@ int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, 
 
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, bwg_batch **out) {
+              const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    bwg_batch *b = new (std::nothrow) bwg_batch();
+    bwg_batch *b = new_batch(1, n);
     if (!b) return BWG_ENOMEM;
-    b->kind = 1;
-    b->n = n;
     b->ksc = sc;
-    b->arena_len = arena_len;
-    b->q_len.assign(q_len, q_len + n);
-    b->t_len.assign(t_len, t_len + n);
-    std::vector<KswJob> jobs(n);
+    b->d_ref = ref;
+    b->hksw.resize(n);
     for (size_t k = 0; k < n; ++k) {
-        jobs[k].q_off = q_off[k];
-        jobs[k].t_off = t_off[k];
-        jobs[k].qlen = q_len[k];
-        jobs[k].tlen = t_len[k];
-        jobs[k].aux0 = (uint32_t)k;
-        jobs[k].aux1 = 0;
+        KswJob &j = b->hksw[k];
+        j.q_off = q_off[k];
+        j.t_off = t_off[k];
+        j.qlen = q_len[k];
+        j.tlen = t_len[k];
+        j.aux0 = (uint32_t)k;
+        j.aux1 = 0;
     }
-    std::lock_guard<std::mutex> g(c->run_mu);
-    int rc = stage_common(c, b, arena, arena_len, std::max<size_t>(1, n) * sizeof(KswJob),
-                          jobs.data(), std::max<size_t>(1, n) * sizeof(bwg_ksw_result));
+    int rc = stage_buffers(c, b, arena, arena_len, b->hksw.data(), n * sizeof(KswJob),
+                           n * sizeof(bwg_ksw_result), st);
     if (rc) {
         destroy_batch(c, b);
         return rc;
@@ -155,105 +189,123 @@ int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, 
     return BWG_OK;
 }
 
+size_t batch_device_bytes(int kind, size_t n, size_t arena_len, const uint32_t *q_len,
+                          const uint32_t *t_len, int32_t max_sc) {
+    size_t bytes = DevicePool::round(arena_len + 64) + 2 * DevicePool::round(std::max<size_t>(64, n * 32));
+    bytes += 2 * DevicePool::round(std::max<size_t>(64, n * 4)) + 4096;  // order array, queue heads
+    if (kind == 1) {
+        // row maxima: tiles of 64 jobs x longest target of the tile, bounded by n * max target
+        uint32_t tmax = 0;
+        for (size_t k = 0; k < n; ++k) tmax = std::max(tmax, t_len[k]);
+        bytes += DevicePool::round(std::max<size_t>(64, ((n + 63) / 64) * 64 * (size_t)tmax * 4));
+    }
+    (void)q_len;
+    (void)max_sc;
+    return bytes;
+}
+
 // ---------------------------------------------------------------------------------------------
 // plain form
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-int prepare_plain(bwg_ctx *c, bwg_batch *b) {
+int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
     if (b->d_plain_jobs) return BWG_OK;
     const size_t n = b->n;
     size_t total = 0;
+    hipError_t e;
     if (b->kind == 0) {
         // {H,E} pairs over absolute columns 0..qlen per job
-        std::vector<ExtJob> jobs(n);
-        hipError_t e = hipMemcpy(jobs.data(), b->d_jobs, n * sizeof(ExtJob), hipMemcpyDeviceToHost);
-        if (e != hipSuccess) return bwg_map_hip(e);
+        std::vector<ExtJob> jobs = b->hext;
         for (size_t k = 0; k < n; ++k) {
             jobs[k].aux = (uint32_t)total;
             total += (size_t)jobs[k].qlen + 1;
         }
         if (total >= (1ull << 32)) return BWG_E2BIG;
-        b->plain_scratch_bytes = total * sizeof(int2);
-        b->d_plain_scratch = c->pool.alloc(b->plain_scratch_bytes);
+        b->d_plain_scratch = c->pool.alloc(total * sizeof(int2));
         b->d_plain_jobs = c->pool.alloc(n * sizeof(ExtJob));
         if (!b->d_plain_scratch || !b->d_plain_jobs) return BWG_ENOMEM;
-        e = hipMemcpy(b->d_plain_jobs, jobs.data(), n * sizeof(ExtJob), hipMemcpyHostToDevice);
+        e = hipMemcpyAsync(b->d_plain_jobs, jobs.data(), n * sizeof(ExtJob), hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) e = hipStreamSynchronize(st);
         return bwg_map_hip(e);
     }
     // rescue: {H,E} pairs for the padded query (+16 for the reverse pass's own padding) and one
     // int32 row maximum per target row; slots are 16-byte aligned
-    std::vector<KswJob> jobs(n);
-    hipError_t e = hipMemcpy(jobs.data(), b->d_jobs, n * sizeof(KswJob), hipMemcpyDeviceToHost);
-    if (e != hipSuccess) return bwg_map_hip(e);
+    std::vector<KswJob> jobs = b->hksw;
     for (size_t k = 0; k < n; ++k) {
-        size_t bytes = ((size_t)ksw_padded((int32_t)jobs[k].qlen, b->ksc.max_sc) + 16) * 8 +
-                       (size_t)jobs[k].tlen * 4;
+        const size_t bytes = ((size_t)ksw_padded((int32_t)jobs[k].qlen, b->ksc.max_sc) + 16) * 8 +
+                             (size_t)jobs[k].tlen * 4;
         jobs[k].aux0 = (uint32_t)(total / 16);
         total += (bytes + 15) & ~(size_t)15;
     }
     if (total / 16 >= (1ull << 32)) return BWG_E2BIG;
-    b->plain_scratch_bytes = total;
     b->d_plain_scratch = c->pool.alloc(std::max<size_t>(16, total));
     b->d_plain_jobs = c->pool.alloc(n * sizeof(KswJob));
     if (!b->d_plain_scratch || !b->d_plain_jobs) return BWG_ENOMEM;
-    e = hipMemcpy(b->d_plain_jobs, jobs.data(), n * sizeof(KswJob), hipMemcpyHostToDevice);
+    e = hipMemcpyAsync(b->d_plain_jobs, jobs.data(), n * sizeof(KswJob), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return bwg_map_hip(e);
 }
 
 }  // namespace
 
+int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
+    if (b->n == 0) return BWG_OK;
+    if (form == BWG_FORM_PLAIN) {
+        int rc = prepare_plain(c, b, ss.main);
+        if (rc) return rc;
+        hipError_t e = hipEventRecord(b->ev_start, ss.main);
+        if (e == hipSuccess) {
+            if (b->kind == 0)
+                e = launch_ext_plain(ss.main, b->d_arena, b->d_ref, (const ExtJob *)b->d_plain_jobs,
+                                     (uint32_t)b->n, b->esc, (int2 *)b->d_plain_scratch,
+                                     (bwg_ext_result *)b->d_out);
+            else
+                e = launch_ksw_plain(ss.main, b->d_arena, b->d_ref, (const KswJob *)b->d_plain_jobs,
+                                     (uint32_t)b->n, b->ksc, (uint8_t *)b->d_plain_scratch,
+                                     (bwg_ksw_result *)b->d_out);
+        }
+        if (e == hipSuccess) e = hipEventRecord(b->ev_end, ss.main);
+        b->last_launches = 1;
+        return bwg_map_hip(e);
+    }
+    int64_t launches = 0;
+    int rc = enqueue_batched(c, b, ss, &launches);
+    b->last_launches = launches;
+    return rc;
+}
+
 int run_batch(bwg_ctx *c, bwg_batch *b) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     std::lock_guard<std::mutex> g(c->run_mu);
     if (b->n == 0) return BWG_OK;
-    const int form = c->form;
-    hipError_t e = hipSuccess;
-    int64_t launches = 0;
-    if (form == BWG_FORM_PLAIN) {
-        int rc = prepare_plain(c, b);
-        if (rc) return rc;
-        e = hipEventRecord(c->ev[0], c->stream);
-        if (e == hipSuccess) {
-            if (b->kind == 0)
-                e = launch_ext_plain(c->stream, b->d_arena, (const ExtJob *)b->d_plain_jobs,
-                                     (uint32_t)b->n, b->esc, (int2 *)b->d_plain_scratch,
-                                     (bwg_ext_result *)b->d_out);
-            else
-                e = launch_ksw_plain(c->stream, b->d_arena, (const KswJob *)b->d_plain_jobs,
-                                     (uint32_t)b->n, b->ksc, (uint8_t *)b->d_plain_scratch,
-                                     (bwg_ksw_result *)b->d_out);
-            launches = 1;
-        }
-        if (e == hipSuccess) e = hipEventRecord(c->ev[1], c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-        if (e != hipSuccess) return bwg_map_hip(e);
-        float ms = 0.f;
-        (void)hipEventElapsedTime(&ms, c->ev[0], c->ev[1]);
-        c->last_kernel_ns = c->last_main_ns = (int64_t)(ms * 1e6);
-        c->last_main_launches = launches;
-    } else {
-        BatchedTiming t;
-        int rc = run_batched(c, b, &t);
-        if (rc) return rc;
-        launches = t.launches;
-        c->last_kernel_ns = t.total_ns;
-        c->last_main_ns = t.main_ns;
-        c->last_main_launches = t.main_launches;
-    }
-    c->launches += launches;
+    int rc = enqueue_batch(c, b, c->form, c->ss);
+    if (rc) return rc;
+    hipError_t e = hipStreamSynchronize(c->ss.main);
+    if (e != hipSuccess) return bwg_map_hip(e);
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev_start, b->ev_end);
+    c->last_kernel_ns = c->last_main_ns = (int64_t)(ms * 1e6);
+    c->last_main_launches = b->last_launches;
+    c->launches += b->last_launches;
     c->jobs += (int64_t)b->n;
     return BWG_OK;
+}
+
+int enqueue_results(bwg_ctx *c, bwg_batch *b, void *out, hipStream_t st) {
+    (void)c;
+    if (b->n == 0) return BWG_OK;
+    const size_t bytes = b->n * (b->kind == 0 ? sizeof(bwg_ext_result) : sizeof(bwg_ksw_result));
+    return bwg_map_hip(hipMemcpyAsync(out, b->d_out, bytes, hipMemcpyDeviceToHost, st));
 }
 
 int fetch_results(bwg_ctx *c, bwg_batch *b, void *out) {
     if (b->n == 0) return BWG_OK;
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    size_t bytes = b->n * (b->kind == 0 ? sizeof(bwg_ext_result) : sizeof(bwg_ksw_result));
     std::lock_guard<std::mutex> g(c->run_mu);
-    hipError_t e = hipMemcpyAsync(out, b->d_out, bytes, hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
-    return bwg_map_hip(e);
+    int rc = enqueue_results(c, b, out, c->ss.main);
+    if (rc) return rc;
+    return bwg_map_hip(hipStreamSynchronize(c->ss.main));
 }
 
 void destroy_batch(bwg_ctx *c, bwg_batch *b) {
@@ -264,6 +316,8 @@ void destroy_batch(bwg_ctx *c, bwg_batch *b) {
     c->pool.free(b->d_plain_scratch);
     c->pool.free(b->d_plain_jobs);
     release_batched(c, b);
+    if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+    if (b->ev_end) (void)hipEventDestroy(b->ev_end);
     delete b;
 }
 

@@ -1,4 +1,4 @@
-// Host-side runtime structures: the context, device memory pool and staged batches.
+// Host-side runtime structures: the context, device memory pool, stream sets and staged batches.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -14,6 +14,7 @@ namespace bwg {
 
 constexpr uint32_t kMaxQlen = 4096;
 constexpr uint32_t kMaxTlen = 65535;
+constexpr int kAuxStreams = 3;
 
 // Caching device allocator: freed blocks are kept per rounded size and reused, so a steady
 // stream of batches does not call hipMalloc/hipFree on the hot path.
@@ -24,13 +25,34 @@ class DevicePool {
     void release_all();
     size_t bytes_in_use() const { return in_use_; }
     size_t bytes_cached() const { return cached_; }
+    static size_t round(size_t b);
 
   private:
-    static size_t round(size_t b);
     std::mutex mu_;
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
     size_t in_use_ = 0, cached_ = 0;
+};
+
+// A main stream plus helper streams used to run independent kernels of one batch concurrently
+// (fork after a main-stream event, join back before the batch's end event).
+struct StreamSet {
+    hipStream_t main = nullptr;
+    hipStream_t aux[kAuxStreams] = {nullptr, nullptr, nullptr};
+    hipEvent_t fork = nullptr;
+    hipEvent_t join[kAuxStreams] = {nullptr, nullptr, nullptr};
+    int create();
+    void destroy();
+};
+
+struct Tuning {
+    int ext_thread_max_q16 = 16;   // queries up to this length run one job per lane, 16-column LDS
+    int ext_thread_max_q32 = 32;   // ... 32-column LDS
+    int ext_group_max_qcap = 256;  // longest query the 16-lane group kernel keeps in LDS
+    int ext_group_oversub = 1;     // persistent grid = oversub * occupancy * CUs
+    int ksw_group_oversub = 1;
+    bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
+    bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph
 };
 
 }  // namespace bwg
@@ -38,10 +60,10 @@ class DevicePool {
 struct bwg_ctx {
     int device = 0;
     int num_cus = 256;
-    hipStream_t stream = nullptr;
-    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
-    std::mutex run_mu;  // one batch on the context stream at a time
+    bwg::StreamSet ss;           // the context's own streams (synchronous entry points)
+    std::mutex run_mu;           // one synchronous batch on the context streams at a time
     int form = BWG_FORM_BATCHED;
+    bwg::Tuning tuning;
     std::atomic<int64_t> launches{0}, jobs{0};
     int64_t last_kernel_ns = 0, last_main_ns = 0, last_main_launches = 0;
     bwg::DevicePool pool;
@@ -52,31 +74,55 @@ struct bwg_batch {
     size_t n = 0;
     bwg::ExtScoring esc{};
     bwg::KswScoring ksc{};
-    uint8_t *d_arena = nullptr;
+    // host copies of the job descriptors, used to plan the launches
+    std::vector<bwg::ExtJob> hext;
+    std::vector<bwg::KswJob> hksw;
+    // device buffers
+    uint8_t *d_arena = nullptr;  // query bytes, and target bytes unless targets come from a reference
     size_t arena_len = 0;
-    void *d_jobs = nullptr;  // ExtJob[n] or KswJob[n] in arrival order
-    void *d_out = nullptr;   // bwg_ext_result[n] or bwg_ksw_result[n]
-    // host copies used to plan the launches
-    std::vector<uint32_t> q_len, t_len;
-    std::vector<int32_t> h0;
-    // per-form device state, created on first run in that form
+    const uint8_t *d_ref = nullptr;  // device-resident reference for targets given as windows
+    void *d_jobs = nullptr;
+    void *d_out = nullptr;
+    // plain form state, created on first plain run
     void *d_plain_scratch = nullptr;
-    size_t plain_scratch_bytes = 0;
-    void *d_plain_jobs = nullptr;  // descriptors carrying the plain form's scratch offsets
-    void *ext_plan = nullptr;      // batched form: bins, order and queues (bwg_batched.cpp)
-    void *ksw_plan = nullptr;
+    void *d_plain_jobs = nullptr;
+    // batched form state (bwg_batched.cpp)
+    void *plan = nullptr;
+    hipGraphExec_t graph = nullptr;
+    int batched_runs = 0;
+    // timing of the last run
+    hipEvent_t ev_start = nullptr, ev_end = nullptr;
+    int64_t last_launches = 0;
 };
 
 namespace bwg {
+
+// Stage a batch: copy arena and descriptors to the device (asynchronously on `st`).  `ref` (may be
+// null) is a device reference array for jobs whose targets are reference windows (bit 62 of
+// t_off; bit 63 reads the window back to front).
 int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, const int32_t *h0, bwg_batch **out);
+              const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
+              bwg_batch **out);
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, bwg_batch **out);
+              const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out);
+// Enqueue the batch's kernels on the stream set (no host synchronisation).  b->ev_start/ev_end
+// bracket them on ss.main.
+int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss);
+// Synchronous run on the context streams: enqueue, wait, update the context statistics.
 int run_batch(bwg_ctx *c, bwg_batch *b);
+int enqueue_results(bwg_ctx *c, bwg_batch *b, void *out, hipStream_t st);
 int fetch_results(bwg_ctx *c, bwg_batch *b, void *out);
 void destroy_batch(bwg_ctx *c, bwg_batch *b);
+// Device bytes a batch of this description holds once staged and planned (admission control).
+size_t batch_device_bytes(int kind, size_t n, size_t arena_len, const uint32_t *q_len,
+                          const uint32_t *t_len, int32_t max_sc);
+
+// batched form (bwg_batched.cpp)
+int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches);
+void release_batched(bwg_ctx *c, bwg_batch *b);
+
 }  // namespace bwg
 
 int bwg_map_hip(hipError_t e);

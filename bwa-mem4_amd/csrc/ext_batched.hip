@@ -27,7 +27,7 @@ constexpr int kKeyMask = (1 << kKeyShift) - 1;
 
 struct GroupJob {
     // per-job constants (identical in the 16 lanes of the group)
-    const uint8_t *t;
+    TargetView t;
     int32_t qlen, tlen, h0, w;
     uint32_t out_idx;
     // row state
@@ -50,6 +50,7 @@ constexpr int kGroupsPerBlock = 16;
 
 template <int K>
 __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restrict__ arena,
+                                                        const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ order,
                                                         uint32_t count, uint32_t *__restrict__ head,
@@ -79,7 +80,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             } else {
                 const ExtJob jb = jobs[order[slot]];
                 const uint8_t *q = arena + jb.q_off;
-                J.t = arena + jb.t_off;
+                J.t = target_view(arena, ref, jb.t_off, jb.tlen);
                 J.qlen = (int32_t)jb.qlen;
                 J.tlen = (int32_t)jb.tlen;
                 J.h0 = jb.h0;
@@ -255,6 +256,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
 // ---- short queries: one job per lane --------------------------------------------------------
 template <int QCAP>
 __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restrict__ arena,
+                                                        const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ order,
                                                         uint32_t count, uint32_t *__restrict__ head,
@@ -276,7 +278,7 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
             const uint32_t idx = order[my];
             const ExtJob jb = jobs[idx];
             const uint8_t *q = arena + jb.q_off;
-            const uint8_t *t = arena + jb.t_off;
+            const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
             const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
             const int32_t w = clamp_band(sc.w, qlen, sc);
             for (int j = 0; j <= qlen; ++j)
@@ -357,11 +359,11 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 }
 
 template <int K>
-hipError_t launch_group(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, const uint32_t *order,
+hipError_t launch_group(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs, const uint32_t *order,
                         uint32_t count, uint32_t *head, const ExtScoring &sc, int qcap,
                         bwg_ext_result *out, int grid) {
     const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
-    hipLaunchKernelGGL(ext_group_kernel<K>, dim3(grid), dim3(256), lds, st, arena, jobs, order, count,
+    hipLaunchKernelGGL(ext_group_kernel<K>, dim3(grid), dim3(256), lds, st, arena, ref, jobs, order, count,
                        head, sc, qcap, out);
     return hipGetLastError();
 }
@@ -396,11 +398,11 @@ int ext_group_blocks_per_cu(int K, int qcap) {
     }
 }
 
-hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const ExtJob *jobs,
+hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                             const uint32_t *order, uint32_t count, uint32_t *head,
                             const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid) {
     switch (K) {
-#define BWG_K(k) case k: return launch_group<k>(st, arena, jobs, order, count, head, sc, qcap, out, grid);
+#define BWG_K(k) case k: return launch_group<k>(st, arena, ref, jobs, order, count, head, sc, qcap, out, grid);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return hipErrorInvalidValue;
@@ -415,15 +417,15 @@ int ext_thread_blocks_per_cu(int qcap) {
     }
 }
 
-hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const ExtJob *jobs,
+hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                              const uint32_t *order, uint32_t count, uint32_t *head,
                              const ExtScoring &sc, bwg_ext_result *out, int grid) {
     switch (qcap) {
     case 16:
-        hipLaunchKernelGGL(ext_thread_kernel<16>, dim3(grid), dim3(64), 0, st, arena, jobs, order, count, head, sc, out);
+        hipLaunchKernelGGL(ext_thread_kernel<16>, dim3(grid), dim3(64), 0, st, arena, ref, jobs, order, count, head, sc, out);
         break;
     case 32:
-        hipLaunchKernelGGL(ext_thread_kernel<32>, dim3(grid), dim3(64), 0, st, arena, jobs, order, count, head, sc, out);
+        hipLaunchKernelGGL(ext_thread_kernel<32>, dim3(grid), dim3(64), 0, st, arena, ref, jobs, order, count, head, sc, out);
         break;
     default: return hipErrorInvalidValue;
     }

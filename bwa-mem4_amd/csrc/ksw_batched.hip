@@ -28,6 +28,7 @@ constexpr int kKeyMask = (1 << kKeyShift) - 1;
 
 template <int K, bool REV>
 __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restrict__ arena,
+                                                        const uint8_t *__restrict__ ref,
                                                         const KswJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ order,
                                                         uint32_t count, uint32_t *__restrict__ head,
@@ -41,7 +42,7 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
 
     bool active = false, exhausted = false;
     // per-job state
-    const uint8_t *t = nullptr;
+    TargetView t = {nullptr, 1};
     int32_t tlen = 0, endsc = 0, i = 0, gmax = 0, te = -1, qe = -1, tnext = 0;
     uint32_t idx = 0;
     int32_t *rm = nullptr;
@@ -59,7 +60,7 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
                 idx = order[slot];
                 const KswJob jb = jobs[idx];
                 const uint8_t *q = arena + jb.q_off;
-                t = arena + jb.t_off;
+                t = target_view(arena, ref, jb.t_off, jb.tlen);
                 int32_t qlen = (int32_t)jb.qlen;
                 tlen = (int32_t)jb.tlen;
                 endsc = 0x7fffffff;
@@ -253,17 +254,17 @@ int ksw_group_blocks_per_cu(int K, bool rev) {
     }
 }
 
-hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const KswJob *jobs,
+hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                             const uint32_t *order, uint32_t count, uint32_t *head, const KswScoring &sc,
                             bwg_ksw_result *out, int32_t *rowmax, int grid) {
     switch (K) {
 #define BWG_K(k)                                                                                          \
     case k:                                                                                               \
         if (rev)                                                                                          \
-            hipLaunchKernelGGL((ksw_group_kernel<k, true>), dim3(grid), dim3(256), 0, st, arena, jobs,    \
+            hipLaunchKernelGGL((ksw_group_kernel<k, true>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
                                order, count, head, sc, out, rowmax);                                      \
         else                                                                                              \
-            hipLaunchKernelGGL((ksw_group_kernel<k, false>), dim3(grid), dim3(256), 0, st, arena, jobs,   \
+            hipLaunchKernelGGL((ksw_group_kernel<k, false>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
                                order, count, head, sc, out, rowmax);                                      \
         return hipGetLastError();
         BWG_KSW_GROUP_KS

@@ -11,6 +11,7 @@ namespace bwg {
 namespace {
 
 __global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restrict__ arena,
+                                                        const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
                                                         uint32_t n, ExtScoring sc,
                                                         int2 *__restrict__ scratch,
@@ -23,7 +24,7 @@ __global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restric
     if (k >= n) return;
     const ExtJob J = jobs[k];
     const uint8_t *q = arena + J.q_off;
-    const uint8_t *t = arena + J.t_off;
+    const TargetView t = target_view(arena, ref, J.t_off, J.tlen);
     const int32_t qlen = (int32_t)J.qlen, tlen = (int32_t)J.tlen, h0 = J.h0;
     int2 *eh = scratch + J.aux;  // eh[j] = {H, E} over absolute columns 0..qlen
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
@@ -114,7 +115,7 @@ __global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restric
 // batched_ksw_align2 without materialising the reversed copies.  `he` holds {H(i-1,j), E(i,j)}
 // for the padded columns; `rowmax` (optional) receives each row's maximum for the tracker.
 template <bool REV>
-__device__ void local_pass(const uint8_t *q, int32_t qlen_real, int32_t qpad, const uint8_t *t,
+__device__ void local_pass(const uint8_t *q, int32_t qlen_real, int32_t qpad, const TargetView &t,
                            int32_t tlen, const int8_t *smat, const KswScoring &sc, int32_t endsc,
                            int2 *he, int32_t *rowmax, int32_t &gmax, int32_t &te, int32_t &qe,
                            int32_t &last_row) {
@@ -154,6 +155,7 @@ __device__ void local_pass(const uint8_t *q, int32_t qlen_real, int32_t qpad, co
 }
 
 __global__ __launch_bounds__(256) void ksw_plain_kernel(const uint8_t *__restrict__ arena,
+                                                        const uint8_t *__restrict__ ref,
                                                         const KswJob *__restrict__ jobs,
                                                         uint32_t n, KswScoring sc,
                                                         uint8_t *__restrict__ scratch,
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(256) void ksw_plain_kernel(const uint8_t *__restric
         return;
     }
     const uint8_t *q = arena + J.q_off;
-    const uint8_t *t = arena + J.t_off;
+    const TargetView t = target_view(arena, ref, J.t_off, J.tlen);
     const int32_t qpad = ksw_padded(qlen, sc.max_sc);
     uint8_t *base = scratch + (size_t)J.aux0 * 16;
     int2 *he = (int2 *)base;
@@ -239,21 +241,21 @@ __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
 
 }  // namespace
 
-hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const ExtJob *jobs, uint32_t n,
-                            const ExtScoring &sc, int2 *scratch, bwg_ext_result *out) {
+hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
+                            uint32_t n, const ExtScoring &sc, int2 *scratch, bwg_ext_result *out) {
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
-    hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+    hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, ref, jobs, n,
                        sc, scratch, (const uint32_t *)nullptr, out);
     return hipGetLastError();
 }
 
-hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const ExtJob *jobs,
+hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const ExtScoring &sc,
                                     int2 *scratch, bwg_ext_result *out) {
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
-    hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+    hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, ref, jobs, n,
                        sc, scratch, out_index, out);
     return hipGetLastError();
 }
@@ -264,21 +266,21 @@ hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n) {
     return hipGetLastError();
 }
 
-hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const KswJob *jobs, uint32_t n,
-                            const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out) {
+hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
+                            uint32_t n, const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out) {
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
-    hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+    hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, ref, jobs, n,
                        sc, scratch, (const uint32_t *)nullptr, out);
     return hipGetLastError();
 }
 
-hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const KswJob *jobs,
+hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const KswScoring &sc,
                                     uint8_t *scratch, bwg_ksw_result *out) {
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
-    hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, jobs, n,
+    hipLaunchKernelGGL(ksw_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, ref, jobs, n,
                        sc, scratch, out_index, out);
     return hipGetLastError();
 }

@@ -96,6 +96,9 @@ int bwg_ksw_align2_batch(bwg_ctx *ctx, const bwg_ksw_params *p, size_t n, const 
 enum { BWG_FORM_PLAIN = 0, BWG_FORM_BATCHED = 1 };
 enum {
     BWG_OPT_FORM = 1,
+    BWG_OPT_CONCURRENT_BINS = 2,  /* batched: run a batch's independent bins concurrently (default 1) */
+    BWG_OPT_GRAPHS = 3,           /* batched: replay repeated runs of a batch from a hipGraph (default 1) */
+    BWG_OPT_EXT_THREAD_MAX_Q = 4, /* batched: longest query run one job per lane (0..32, default 32) */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */

@@ -179,3 +179,49 @@ def test_backend_mirror_api(gpu_ctx):
                               5, mat, 6, 1, 6, 1, 4, 1)
     assert (r[0].score, r[0].qb, r[0].qe, r[0].tb, r[0].te) == (8, 0, 7, 0, 7)
     assert be.extend_batch([], 5, mat, 6, 1, 6, 1, 100, 0, 100) == []
+
+
+@pytest.mark.parametrize("kind", ["ext", "resc"])
+def test_graph_replays_match_eager(gpu_ctx, kind):
+    """The first batched run launches eagerly; later runs replay a captured hipGraph.  Every replay,
+    with other eager device work and synchronisations in between, must equal the eager result and
+    the oracle."""
+    _say("graph_replays %s" % kind)
+    mat = bwg.default_mat()
+    if kind == "ext":
+        batch = bwg.generate(bwg.WORKLOAD_EXT_TAIL, 20000, seed=5)
+        params = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+        want = ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100)
+        other = bwg.generate(bwg.WORKLOAD_EXT_SMALL, 3000, seed=6)
+        oparams = params
+    else:
+        batch = bwg.generate(bwg.WORKLOAD_RESCUE, 3000, seed=5)
+        params = bwg.ksw_params(5, mat, 6, 1, 6, 1, 19, 1)
+        want = ob.align2_batch(batch, mat, 6, 1, 6, 1, 19, 1)
+        other = bwg.generate(bwg.WORKLOAD_EXT_SMALL, 3000, seed=6)
+        oparams = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    st = gpu_ctx.stage(batch, params)
+    results = []
+    for r in range(5):
+        st.run()
+        results.append(st.results())
+        gpu_ctx.extend_arrays(other, oparams)  # unrelated eager work between replays
+    st.close()
+    for r, got in enumerate(results):
+        bad = np.nonzero(got != want)[0]
+        assert bad.size == 0, "run %d: first mismatch job %d" % (r, bad[0])
+
+
+def test_serial_and_concurrent_bins_agree(gpu_ctx):
+    mat = bwg.default_mat()
+    batch = bwg.generate(bwg.WORKLOAD_EXT_TAIL, 20000, seed=8)
+    params = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    outs = []
+    for conc in (0, 1):
+        gpu_ctx.set_option(bwg.OPT_CONCURRENT_BINS, conc)
+        outs.append(gpu_ctx.extend_arrays(batch, params))
+    gpu_ctx.set_option(bwg.OPT_CONCURRENT_BINS, 1)
+    assert np.array_equal(outs[0], outs[1])
+    assert np.array_equal(outs[0], ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100))
