@@ -71,6 +71,9 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
     ap.add_argument("--only", default="ext,resc", help="comma list of workloads to time")
+    ap.add_argument("--serve-rates", default="2e5,2e6,2e7,5e7",
+                    help="offered loads (jobs/s) of the serving sweep; empty to skip")
+    ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -121,7 +124,7 @@ def main():
             log("rescue %s: staging" % fname)
             st = ctx.stage(resc, ksw_p)
             st.run()
-            per, kern, main, nl = time_steps(ctx, st, max(1, args.steps // 2), 1)
+            per, kern, main, nl = time_steps(ctx, st, max(1, args.steps // 2) if form == bwg.FORM_BATCHED else 2, 1)
             outs["resc_" + fname] = st.results()
             st.close()
             res["resc_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
@@ -130,6 +133,23 @@ def main():
             pass
         except bwg.BwgError as e:
             log("form %s unavailable: %s" % (fname, e))
+    serving = []
+    rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
+    if rates:
+        ctx.set_form(bwg.FORM_BATCHED)
+        srv = bwg.Server(ctx, hbm_budget_bytes=64 << 30, slots=3)
+        for rate in rates:
+            r = bwg.serve_run(ctx, srv, rate, producers=8, duration_s=args.serve_seconds, seed=42,
+                              use_ref=True, pool_chunks=64, verify=True)
+            r["offered_jobs_per_s"] = rate
+            serving.append(r)
+            log("serving at %.3g jobs/s: p50 %.2f ms p99 %.2f ms, completed %.3g jobs/s, %d chunks, "
+                "%d mismatched" % (rate, r["p50_ms"], r["p99_ms"], r["jobs_per_s"], r["chunks"],
+                                   r["mismatched_chunks"]))
+            if r["mismatched_chunks"] or r["failed_chunks"]:
+                log("serving results differ from the chunks run alone")
+                sys.exit(1)
+        srv.close()
     head = "ext_batched" if "ext_batched" in res else "ext_plain"
     if head not in res:
         log("no extension timing requested; nothing to report")
@@ -179,6 +199,12 @@ def main():
             r = res["resc_" + f]
             line["rescue_%s_jobs_per_s" % f] = args.rescue_jobs / r["step_s"]
             line["rescue_%s_gcell_s" % f] = float(np.sum(resc.q_len.astype(np.int64) * resc.t_len)) / r["step_s"] / 1e9
+    if serving:
+        at_cfg = min(serving, key=lambda r: abs(r["offered_jobs_per_s"] - 2e5))
+        line["p50_chunk_latency_ms"] = at_cfg["p50_ms"]
+        line["p99_chunk_latency_ms"] = at_cfg["p99_ms"]
+        line["serving"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
+                           for r in serving]
     if cpu:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

@@ -16,7 +16,7 @@ ROOT = os.path.dirname(HERE)
 BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
-HIP_SOURCES = ["plain_kernels.hip", "ext_batched.hip", "ksw_batched.hip", "bwg_batched.cpp", "bwg_runtime.cpp", "bwg_api.cpp"]
+HIP_SOURCES = ["plain_kernels.hip", "ext_batched.hip", "ksw_batched.hip", "bwg_batched.cpp", "bwg_runtime.cpp", "bwg_api.cpp", "bwg_server.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
@@ -48,7 +48,8 @@ def build(verbose=False):
     os.replace(tmp, LIB)
     tmpw = WORKLOAD_LIB + ".tmp"
     _run(["g++", "-O3", "-std=c++17", "-fPIC", "-shared", "-pthread",
-          os.path.join(CSRC, "workload.cpp"), "-o", tmpw])
+          os.path.join(CSRC, "workload.cpp"), os.path.join(CSRC, "serve_driver.cpp"), "-o", tmpw,
+          "-L" + HERE, "-lbwg", "-Wl,-rpath,$ORIGIN"])
     os.replace(tmpw, WORKLOAD_LIB)
     if verbose:
         for out, _ in results:

@@ -39,6 +39,8 @@ EXPORTED_SYMBOLS = (
     "bwg_create", "bwg_destroy", "bwg_name", "bwg_strerror", "bwg_extend_batch", "bwg_extend",
     "bwg_ksw_align2_batch", "bwg_set_option", "bwg_get_stat", "bwg_batch_stage_ext",
     "bwg_batch_stage_ksw", "bwg_batch_run", "bwg_batch_results", "bwg_batch_destroy",
+    "bwg_server_create", "bwg_server_set_reference", "bwg_server_submit", "bwg_server_poll",
+    "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy",
 )
 
 
@@ -365,3 +367,150 @@ def default_mat(a=1, b=4):
         for j in range(4):
             m[i, j] = a if i == j else -b
     return m.reshape(-1)
+
+
+# ---- serving -------------------------------------------------------------------------------------
+CHUNK_EXTEND, CHUNK_RESCUE = 0, 1
+TARGET_FROM_REF, TARGET_REVERSE = 1 << 62, 1 << 63
+
+
+class Chunk(C.Structure):
+    _fields_ = [("kind", C.c_int32), ("ext", C.c_void_p), ("ksw", C.c_void_p), ("n", C.c_size_t),
+                ("arena", C.c_void_p), ("arena_len", C.c_size_t), ("q_off", C.c_void_p),
+                ("q_len", C.c_void_p), ("t_off", C.c_void_p), ("t_len", C.c_void_p),
+                ("h0", C.c_void_p), ("out", C.c_void_p)]
+
+
+class ServerCounters(C.Structure):
+    _fields_ = [("chunks", C.c_uint64), ("jobs", C.c_uint64), ("epochs", C.c_uint64),
+                ("peak_device_bytes", C.c_uint64), ("seconds", C.c_double), ("p50_ms", C.c_double),
+                ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double)]
+
+
+def _server_lib():
+    L = _lib()
+    if not hasattr(L, "_server_bound"):
+        L.bwg_server_create.restype = C.c_int
+        L.bwg_server_create.argtypes = [_P, C.c_uint64, C.c_int, _P]
+        L.bwg_server_set_reference.restype = C.c_int
+        L.bwg_server_set_reference.argtypes = [_P, _P, C.c_uint64]
+        L.bwg_server_submit.restype = C.c_int
+        L.bwg_server_submit.argtypes = [_P, _P, _P]
+        L.bwg_server_poll.restype = C.c_int
+        L.bwg_server_poll.argtypes = [_P, C.c_uint64]
+        L.bwg_server_wait.restype = C.c_int
+        L.bwg_server_wait.argtypes = [_P, C.c_uint64]
+        L.bwg_server_stats.restype = C.c_int
+        L.bwg_server_stats.argtypes = [_P, _P, C.c_int]
+        L.bwg_server_destroy.restype = None
+        L.bwg_server_destroy.argtypes = [_P]
+        L._server_bound = True
+    return L
+
+
+class Server:
+    """Continuous-batching server on one context (bwg_server_* in include/bwg.h)."""
+
+    def __init__(self, ctx, hbm_budget_bytes=64 << 30, slots=3):
+        L = _server_lib()
+        h = C.c_void_p()
+        _check(L.bwg_server_create(ctx.h, int(hbm_budget_bytes), int(slots), C.byref(h)), "bwg_server_create")
+        self.h, self.ctx = h, ctx
+        self._live = {}
+
+    def set_reference(self, ref):
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        _check(_server_lib().bwg_server_set_reference(self.h, ref.ctypes.data, ref.size), "bwg_server_set_reference")
+
+    def submit(self, batch, params, t_off=None):
+        """Queue a JobBatch; t_off overrides the batch's target offsets (e.g. genome windows with
+        TARGET_FROM_REF).  Returns a ticket; the arrays stay referenced until wait()."""
+        is_ext = isinstance(params, ExtParams)
+        out = np.zeros(len(batch), dtype=EXT_DTYPE if is_ext else KSW_DTYPE)
+        t = np.ascontiguousarray(batch.t_off if t_off is None else t_off, dtype=np.uint64)
+        ch = Chunk(CHUNK_EXTEND if is_ext else CHUNK_RESCUE,
+                   C.addressof(params) if is_ext else None, None if is_ext else C.addressof(params),
+                   len(batch), batch.arena.ctypes.data, batch.arena.size, batch.q_off.ctypes.data,
+                   batch.q_len.ctypes.data, t.ctypes.data, batch.t_len.ctypes.data,
+                   batch.h0.ctypes.data, out.ctypes.data)
+        tk = C.c_uint64()
+        _check(_server_lib().bwg_server_submit(self.h, C.byref(ch), C.byref(tk)), "bwg_server_submit")
+        self._live[tk.value] = (batch, params, t, out)
+        return tk.value
+
+    def wait(self, ticket):
+        rc = _server_lib().bwg_server_wait(self.h, ticket)
+        _, _, _, out = self._live.pop(ticket)
+        _check(rc, "bwg_server_wait")
+        return out
+
+    def counters(self, reset=False):
+        c = ServerCounters()
+        _check(_server_lib().bwg_server_stats(self.h, C.byref(c), 1 if reset else 0), "bwg_server_stats")
+        return {f: getattr(c, f) for f, _ in ServerCounters._fields_}
+
+    def close(self):
+        if self.h:
+            _server_lib().bwg_server_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def genome(seed=42):
+    """The genome every generated job of `seed` is drawn from (codes 0..4)."""
+    W = _wlib()
+    W.bwgw_genome_len.restype = C.c_uint64
+    W.bwgw_genome.restype = None
+    W.bwgw_genome.argtypes = [C.c_uint64, C.c_void_p]
+    g = np.zeros(int(W.bwgw_genome_len()), dtype=np.uint8)
+    W.bwgw_genome(seed, g.ctypes.data)
+    return g
+
+
+def generate_windows(kind, n, seed=42, genome_seed=None):
+    """Like generate(), drawing jobs with `seed` from the genome of `genome_seed`; also returns each
+    job's target as a genome window (TARGET_FROM_REF | start, plus TARGET_REVERSE for left
+    extensions)."""
+    W = _wlib()
+    W.bwgw_generate_on.restype = C.c_void_p
+    W.bwgw_generate_on.argtypes = [C.c_int, C.c_uint64, C.c_uint64, C.c_uint64]
+    W.bwgw_copy_windows.restype = None
+    W.bwgw_copy_windows.argtypes = [C.c_void_p, C.c_void_p]
+    h = W.bwgw_generate_on(kind, seed if genome_seed is None else genome_seed, seed, n)
+    try:
+        n = int(W.bwgw_count(h))
+        arena = np.zeros(int(W.bwgw_arena_len(h)) + 1, dtype=np.uint8)
+        q_off = np.zeros(n, dtype=np.uint64)
+        q_len = np.zeros(n, dtype=np.uint32)
+        t_off = np.zeros(n, dtype=np.uint64)
+        t_len = np.zeros(n, dtype=np.uint32)
+        h0 = np.zeros(n, dtype=np.int32)
+        kinds = np.zeros(n, dtype=np.uint8)
+        win = np.zeros(n, dtype=np.uint64)
+        W.bwgw_copy(h, arena.ctypes.data, q_off.ctypes.data, q_len.ctypes.data, t_off.ctypes.data,
+                    t_len.ctypes.data, h0.ctypes.data, kinds.ctypes.data)
+        W.bwgw_copy_windows(h, win.ctypes.data)
+    finally:
+        W.bwgw_free(h)
+    return JobBatch(arena, q_off, q_len, t_off, t_len, h0, kinds), win
+
+
+def serve_run(ctx, server, rate_jobs_s, producers=8, duration_s=5.0, seed=42, use_ref=True,
+              pool_chunks=64, verify=True):
+    """Poisson load from `producers` threads against `server` (the C++ driver in the workload
+    library).  Returns p50/p99/mean/max chunk latency (ms), completed jobs/s and counts."""
+    W = _wlib()
+    W.bwgw_serve_run.restype = C.c_int
+    W.bwgw_serve_run.argtypes = [_P, _P, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,
+                                 C.c_int, _P]
+    out = (C.c_double * 11)()
+    _check(W.bwgw_serve_run(ctx.h, server.h, float(rate_jobs_s), int(producers), float(duration_s), seed,
+                            1 if use_ref else 0, int(pool_chunks), 1 if verify else 0, out), "bwgw_serve_run")
+    keys = ("p50_ms", "p99_ms", "mean_ms", "max_ms", "jobs_per_s", "chunks", "jobs", "epochs",
+            "mismatched_chunks", "failed_chunks", "peak_device_bytes")
+    return dict(zip(keys, list(out)))

@@ -105,6 +105,8 @@ void sim_read(Rng &r, const Genome &g, size_t pos, int len, double sub, double i
 struct Job {
     std::vector<uint8_t> q, t;
     int32_t h0 = 0;
+    uint64_t win = 0;   // target as a genome window: start position
+    bool rev = false;   // ... read back to front
 };
 
 // One extension job: query = read[s, s+qlen) (right extension) or the reverse of read[s-qlen, s)
@@ -126,11 +128,15 @@ void make_ext_job(Rng &r, const Genome &g, int read_len, int qlen, double sub, d
         int64_t rp = ref_of[(size_t)s];
         for (int k = 0; k < qlen; ++k) job.q[(size_t)k] = read[(size_t)(s + k)];
         for (int k = 0; k < tlen; ++k) job.t[(size_t)k] = g.seq[(size_t)(rp + k)];
+        job.win = (uint64_t)rp;
+        job.rev = false;
     } else {
         int s = (int)r.uniform(qlen, read_len);  // seed starts at read position s
         int64_t rp = s < read_len ? ref_of[(size_t)s] : ref_of[(size_t)read_len - 1] + 1;
         for (int k = 0; k < qlen; ++k) job.q[(size_t)k] = read[(size_t)(s - 1 - k)];
         for (int k = 0; k < tlen; ++k) job.t[(size_t)k] = g.seq[(size_t)(rp - 1 - k)];
+        job.win = (uint64_t)(rp - tlen);
+        job.rev = true;
     }
     job.h0 = (int32_t)r.uniform(h0_lo, h0_hi);
 }
@@ -159,6 +165,8 @@ void make_resc_job(Rng &r, const Genome &g, Job &job) {
     }
     for (int k = 0; k < L; ++k) job.t[(size_t)k] = g.seq[ws + (size_t)k];
     job.h0 = 0;
+    job.win = ws;
+    job.rev = false;
 }
 
 int draw_tail_qlen(Rng &r) {  // 70% < 30, 25% 30..150, 5% 150..250
@@ -176,6 +184,7 @@ int draw_read_len(Rng &r, int qlen) {  // 100/150/250 at 40/40/20%, at least qle
 
 struct Batch {
     std::vector<uint8_t> arena;
+    std::vector<uint64_t> t_win;  // target as a genome window: start | 1<<62 (| 1<<63 reversed)
     std::vector<uint64_t> q_off, t_off;
     std::vector<uint32_t> q_len, t_len;
     std::vector<int32_t> h0;
@@ -184,9 +193,9 @@ struct Batch {
 
 // kind: 1 = plain extension (U[1,130] from 150 bp reads), 2 = rescue, 3 = long-tail extension,
 // 4 = serving mix (80% long-tail extension / 20% rescue).
-Batch *generate(int kind, uint64_t seed, size_t n) {
+Batch *generate(int kind, uint64_t genome_seed, uint64_t seed, size_t n) {
     static const size_t kGenome = 8u << 20;
-    Genome g(seed, kGenome, 0.001);
+    Genome g(genome_seed, kGenome, 0.001);
     std::vector<Job> jobs(n);
     std::vector<uint8_t> kinds(n, 0);
     unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
@@ -229,6 +238,7 @@ Batch *generate(int kind, uint64_t seed, size_t n) {
     b->q_len.resize(n);
     b->t_len.resize(n);
     b->h0.resize(n);
+    b->t_win.resize(n);
     b->kind = kinds;
     size_t o = 0;
     for (size_t k = 0; k < n; ++k) {
@@ -242,6 +252,7 @@ Batch *generate(int kind, uint64_t seed, size_t n) {
         if (!j.t.empty()) std::memcpy(&b->arena[o], j.t.data(), j.t.size());
         o += j.t.size();
         b->h0[k] = j.h0;
+        b->t_win[k] = j.win | (1ull << 62) | (j.rev ? (1ull << 63) : 0);
     }
     return b;
 }
@@ -250,7 +261,11 @@ Batch *generate(int kind, uint64_t seed, size_t n) {
 
 extern "C" {
 
-void *bwgw_generate(int kind, uint64_t seed, uint64_t n) { return generate(kind, seed, (size_t)n); }
+void *bwgw_generate(int kind, uint64_t seed, uint64_t n) { return generate(kind, seed, seed, (size_t)n); }
+// Jobs drawn with `seed` from the genome of `genome_seed` (batches that share one genome).
+void *bwgw_generate_on(int kind, uint64_t genome_seed, uint64_t seed, uint64_t n) {
+    return generate(kind, genome_seed, seed, (size_t)n);
+}
 uint64_t bwgw_arena_len(const void *h) { return ((const Batch *)h)->arena.size(); }
 uint64_t bwgw_count(const void *h) { return ((const Batch *)h)->q_len.size(); }
 void bwgw_copy(const void *h, uint8_t *arena, uint64_t *q_off, uint32_t *q_len, uint64_t *t_off,
@@ -266,5 +281,19 @@ void bwgw_copy(const void *h, uint8_t *arena, uint64_t *q_off, uint32_t *q_len, 
     if (kind) std::memcpy(kind, b->kind.data(), n);
 }
 void bwgw_free(void *h) { delete (Batch *)h; }
+
+// Target windows of the jobs of a generated batch, as genome coordinates with the reference flag
+// (and the reverse flag for left extensions) set: the form a server with the genome resident takes.
+void bwgw_copy_windows(const void *h, uint64_t *t_win) {
+    const Batch *b = (const Batch *)h;
+    if (t_win) std::memcpy(t_win, b->t_win.data(), b->t_win.size() * 8);
+}
+
+// The genome the generator draws every job of `seed` from.
+uint64_t bwgw_genome_len() { return 8u << 20; }
+void bwgw_genome(uint64_t seed, uint8_t *out) {
+    Genome g(seed, 8u << 20, 0.001);
+    std::memcpy(out, g.seq.data(), g.seq.size());
+}
 
 }  // extern "C"

@@ -126,6 +126,54 @@ int bwg_batch_run(bwg_ctx *ctx, bwg_batch *b);
 int bwg_batch_results(bwg_ctx *ctx, bwg_batch *b, void *out);
 void bwg_batch_destroy(bwg_ctx *ctx, bwg_batch *b);
 
+/* ---- serving ----------------------------------------------------------------------------------
+ * Continuous batching of alignment work from concurrent producers.  No counterpart in the
+ * reference: its callers run extension and rescue per rayon chunk
+ * (crates/bwa-cli/src/cmd_mem.rs:2209-2216, crates/bwa-mem/src/pe.rs:1090-1100); a server lets those
+ * callers submit their chunks from every worker thread and have them merged into device-sized
+ * epochs.  A chunk holds jobs of one kind sharing one scoring.  Its inputs must stay valid and
+ * unmodified, and its `out` array must stay allocated, until bwg_server_wait / bwg_server_poll
+ * report completion.  Targets index the chunk arena, or, with BWG_TARGET_FROM_REF set in t_off,
+ * the server's device-resident reference (bwg_server_set_reference); BWG_TARGET_REVERSE then
+ * reads that reference window back to front (left extensions). */
+#define BWG_TARGET_FROM_REF (1ull << 62)
+#define BWG_TARGET_REVERSE (1ull << 63)
+enum { BWG_CHUNK_EXTEND = 0, BWG_CHUNK_RESCUE = 1 };
+typedef struct {
+    int32_t kind;                /* BWG_CHUNK_EXTEND or BWG_CHUNK_RESCUE */
+    const bwg_ext_params *ext;   /* scoring, kind == BWG_CHUNK_EXTEND */
+    const bwg_ksw_params *ksw;   /* scoring, kind == BWG_CHUNK_RESCUE */
+    size_t n;
+    const uint8_t *arena;
+    size_t arena_len;
+    const uint64_t *q_off;
+    const uint32_t *q_len;
+    const uint64_t *t_off;
+    const uint32_t *t_len;
+    const int32_t *h0;           /* extension only */
+    void *out;                   /* n bwg_ext_result or bwg_ksw_result */
+} bwg_chunk;
+typedef struct {
+    uint64_t chunks, jobs, epochs, peak_device_bytes;
+    double seconds, p50_ms, p99_ms, mean_ms, max_ms;  /* chunk latency: submit to results available */
+} bwg_server_counters;
+typedef struct bwg_server bwg_server;
+/* hbm_budget_bytes bounds the device memory of the epochs in flight; n_slots (1..8) epochs can be
+ * on the device at once. */
+int bwg_server_create(bwg_ctx *ctx, uint64_t hbm_budget_bytes, int n_slots, bwg_server **out);
+/* Upload the reference (codes 0..4) that BWG_TARGET_FROM_REF targets index; only while idle. */
+int bwg_server_set_reference(bwg_server *s, const uint8_t *ref, uint64_t len);
+/* Validate and queue a chunk (any thread).  BWG_E2BIG if it alone exceeds the budget. */
+int bwg_server_submit(bwg_server *s, const bwg_chunk *chunk, uint64_t *ticket);
+/* BWG_EAGAIN while the chunk is pending, else its final status (and the ticket is released). */
+int bwg_server_poll(bwg_server *s, uint64_t ticket);
+/* Block until the chunk completes; returns its status and releases the ticket. */
+int bwg_server_wait(bwg_server *s, uint64_t ticket);
+/* Counters and chunk-latency percentiles since creation or the last reset. */
+int bwg_server_stats(bwg_server *s, bwg_server_counters *out, int reset);
+/* Drain every queued chunk, then stop. */
+void bwg_server_destroy(bwg_server *s);
+
 #ifdef __cplusplus
 }
 #endif
