@@ -71,7 +71,7 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
     ap.add_argument("--only", default="ext,resc", help="comma list of workloads to time")
-    ap.add_argument("--serve-rates", default="2e5,2e6,2e7,5e7",
+    ap.add_argument("--serve-rates", default="2e5,2e6,1e7,3e7,6e7",
                     help="offered loads (jobs/s) of the serving sweep; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0)
     ap.add_argument("--calib-copy-mb", type=int, default=0,

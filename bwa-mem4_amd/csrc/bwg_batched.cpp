@@ -81,12 +81,27 @@ int widest_pad(int qlen, int max_sc) {
     return w;
 }
 
+// Monotone 16-bit key of a job's nominal work: exponent and the 11 bits below the leading one.
+inline uint32_t work_key(uint64_t w) {
+    if (w == 0) return 0;
+    if (w >= (1ull << 31)) w = (1ull << 31) - 1;
+    const uint32_t e = 31 - (uint32_t)__builtin_clz((uint32_t)w);
+    const uint32_t m = (uint32_t)((w << (31 - e)) >> 20) & 0x7ff;
+    return e * 2048 + m + 1;
+}
+
 // Group jobs into bins by key, order the bins by stage then descending work and each bin's jobs
-// by descending work; returns the order array, bins get their offsets.
+// by descending work; returns the order array, bins get their offsets.  Two stable counting-sort
+// passes (work key descending, then bin), O(n) for any batch size.
 template <class KeyFn, class WorkFn>
 std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, WorkFn work_of) {
     std::vector<int> key(n);
-    for (size_t k = 0; k < n; ++k) key[k] = key_of(k);
+    std::vector<uint16_t> wk(n);
+    for (size_t k = 0; k < n; ++k) {
+        key[k] = key_of(k);
+        const uint64_t w = work_of(k);
+        wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
+    }
     for (size_t k = 0; k < n; ++k) {
         bins[key[k]].count++;
         bins[key[k]].work += (double)work_of(k);
@@ -108,12 +123,18 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
         b.offset = off;
         off += b.count;
     }
+    // pass 1: by descending work
+    std::vector<uint32_t> hist(65537, 0), tmp(n);
+    for (size_t k = 0; k < n; ++k) hist[(size_t)wk[k] + 1]++;
+    for (size_t v = 1; v < hist.size(); ++v) hist[v] += hist[v - 1];
+    for (size_t k = 0; k < n; ++k) tmp[hist[wk[k]]++] = (uint32_t)k;
+    // pass 2: stable by bin
     std::vector<uint32_t> order(n), cur(sorted.size());
     for (size_t r = 0; r < sorted.size(); ++r) cur[r] = sorted[r].offset;
-    for (size_t k = 0; k < n; ++k) order[cur[remap[key[k]]]++] = (uint32_t)k;
-    for (auto &b : sorted)
-        std::stable_sort(order.begin() + b.offset, order.begin() + b.offset + b.count,
-                         [&](uint32_t a, uint32_t c) { return work_of(a) > work_of(c); });
+    for (size_t r = 0; r < n; ++r) {
+        const uint32_t k = tmp[r];
+        order[cur[remap[key[k]]]++] = k;
+    }
     bins.swap(sorted);
     return order;
 }

@@ -1,14 +1,15 @@
 // Serving loop: chunks of alignment work from concurrent producers, continuously batched onto the
 // device under an HBM budget.
 //
-// Producers call bwg_server_submit from any thread; the chunk is validated there and queued.  A
-// dispatcher thread forms epochs: it takes the queued chunks in arrival order, merging chunks of
-// the same kind and scoring, up to a job cap and as far as the HBM budget admits, packs their
-// inputs into a pinned staging buffer, stages them and enqueues the batched-form kernels and the
-// result copy on one of several slots (each slot its own streams), so a new epoch's copies and
-// kernels overlap the epochs already on the device.  A completion thread waits for epochs in
-// launch order, scatters results into the producers' output arrays, records per-chunk latency
-// (submit to results available) and returns the epoch's memory to the budget.
+// Producers call bwg_server_submit from any thread; the chunk is validated there and queued.  Each
+// of several slots (its own streams, pinned staging buffers and worker thread) repeatedly forms an
+// epoch: it takes the queued chunks in arrival order, merging chunks of the same kind and scoring,
+// up to a job cap and as far as the HBM budget admits; packs their inputs into its pinned buffer;
+// stages them and enqueues the batched-form kernels and the result copy; waits for them; scatters
+// the results into the producers' output arrays, records per-chunk latency (submit to results
+// available) and returns the epoch's memory to the budget.  With several slots the host
+// preparation of one epoch, the device work of another and the completion of a third overlap, and
+// requests that arrive while epochs run join the next epoch a slot forms.
 #include <algorithm>
 #include <chrono>
 #include <condition_variable>
@@ -71,30 +72,31 @@ int32_t mat_max(const int8_t *mat) {
 struct bwg_server {
     bwg_ctx *ctx = nullptr;
     uint64_t budget = 0;
-    uint64_t max_epoch_jobs = 1u << 20;
+    uint64_t max_epoch_jobs = 1u << 19;
     std::vector<Slot> slots;
     uint8_t *d_ref = nullptr;
     uint64_t ref_len = 0;
 
     std::mutex mu;
-    std::condition_variable cv_dispatch, cv_done, cv_complete;
+    std::condition_variable cv_dispatch, cv_complete;
     std::deque<Chunk *> pending;
-    std::deque<Epoch *> inflight;
     std::unordered_map<uint64_t, Chunk *> chunks;
     uint64_t next_ticket = 1;
     uint64_t in_flight_bytes = 0, peak_bytes = 0;
     bool stop = false;
-    bool dispatcher_done = false;  // set by the dispatcher as it exits (after its last epoch is queued)
-    std::thread dispatcher, completer;
+    std::vector<std::thread> workers;
 
     // statistics
     std::vector<double> lat_ms;
     uint64_t n_chunks = 0, n_jobs = 0, n_epochs = 0;
     Clock::time_point stats_t0 = Clock::now();
 
-    void dispatch_loop();
-    void complete_loop();
+    void slot_loop(int slot);
     int launch_epoch(Epoch *ep);
+    bool pending_empty_hint() {
+        std::lock_guard<std::mutex> g(mu);
+        return pending.empty();
+    }
 };
 
 namespace {
@@ -233,32 +235,20 @@ int bwg_server::launch_epoch(Epoch *ep) {
     return bwg_map_hip(hipEventRecord(sl.done, sl.ss.main));
 }
 
-void bwg_server::dispatch_loop() {
+void bwg_server::slot_loop(int si) {
     (void)hipSetDevice(ctx->device);
+    Slot &sl = slots[(size_t)si];
     for (;;) {
-        Epoch *ep = new Epoch;
+        Epoch ep;
+        ep.slot = si;
         {
             std::unique_lock<std::mutex> g(mu);
-            int free_slot = -1;
-            for (;;) {
-                free_slot = -1;
-                for (size_t k = 0; k < slots.size(); ++k)
-                    if (!slots[k].busy) {
-                        free_slot = (int)k;
-                        break;
-                    }
-                if (stop && pending.empty()) break;
-                if (!pending.empty() && free_slot >= 0 && in_flight_bytes + pending.front()->bytes <= budget) break;
-                cv_dispatch.wait(g);
-            }
-            if (stop && pending.empty()) {
-                delete ep;
-                dispatcher_done = true;
-                cv_done.notify_all();
-                return;
-            }
+            cv_dispatch.wait(g, [&] {
+                return (stop && pending.empty()) ||
+                       (!pending.empty() && in_flight_bytes + pending.front()->bytes <= budget);
+            });
+            if (pending.empty()) return;  // stopping and drained
             // take chunks in arrival order while they merge into this epoch and fit the budget
-            ep->slot = free_slot;
             uint64_t jobs = 0;
             Chunk *first[2] = {nullptr, nullptr};
             while (!pending.empty()) {
@@ -266,67 +256,43 @@ void bwg_server::dispatch_loop() {
                 const int kind = k->c.kind == BWG_CHUNK_EXTEND ? 0 : 1;
                 if (first[kind] && !same_params(first[kind], k)) break;
                 if (jobs && jobs + k->c.n > max_epoch_jobs) break;
-                if (in_flight_bytes + ep->bytes + k->bytes > budget) break;
+                if (in_flight_bytes + ep.bytes + k->bytes > budget) break;
                 if (!first[kind]) first[kind] = k;
                 pending.pop_front();
-                ep->chunks[kind].push_back(k);
-                ep->bytes += k->bytes;
+                ep.chunks[kind].push_back(k);
+                ep.bytes += k->bytes;
                 jobs += k->c.n;
             }
-            in_flight_bytes += ep->bytes;
+            in_flight_bytes += ep.bytes;
             peak_bytes = std::max(peak_bytes, in_flight_bytes);
-            slots[ep->slot].busy = true;
-        }
-        ep->rc = launch_epoch(ep);
-        {
-            std::lock_guard<std::mutex> g(mu);
-            inflight.push_back(ep);
             ++n_epochs;
         }
-        cv_done.notify_all();
-    }
-}
-
-void bwg_server::complete_loop() {
-    (void)hipSetDevice(ctx->device);
-    for (;;) {
-        Epoch *ep = nullptr;
-        {
-            std::unique_lock<std::mutex> g(mu);
-            cv_done.wait(g, [&] { return !inflight.empty() || dispatcher_done; });
-            if (inflight.empty()) return;  // the dispatcher has exited and every epoch is done
-            ep = inflight.front();
-            inflight.pop_front();
-        }
-        Slot &sl = slots[ep->slot];
-        int rc = ep->rc;
+        if (!pending_empty_hint()) cv_dispatch.notify_one();  // let another slot start the next epoch
+        int rc = launch_epoch(&ep);
         if (!rc) rc = bwg_map_hip(hipEventSynchronize(sl.done));
         else (void)hipStreamSynchronize(sl.ss.main);
         const auto now = Clock::now();
         for (int kind = 0; kind < 2; ++kind) {
-            size_t off = ep->out_off[kind];
+            size_t off = ep.out_off[kind];
             const size_t rsz = kind == 0 ? sizeof(bwg_ext_result) : sizeof(bwg_ksw_result);
-            for (auto *k : ep->chunks[kind]) {
+            for (auto *k : ep.chunks[kind]) {
                 if (!rc && k->c.n) std::memcpy(k->c.out, sl.pin_out + off, k->c.n * rsz);
                 off += k->c.n * rsz;
             }
-            if (ep->batch[kind]) bwg::destroy_batch(ctx, ep->batch[kind]);
+            if (ep.batch[kind]) bwg::destroy_batch(ctx, ep.batch[kind]);
         }
         {
             std::lock_guard<std::mutex> g(mu);
             for (int kind = 0; kind < 2; ++kind)
-                for (auto *k : ep->chunks[kind]) {
+                for (auto *k : ep.chunks[kind]) {
                     k->status = rc;
                     lat_ms.push_back(std::chrono::duration<double, std::milli>(now - k->t_submit).count());
                     ++n_chunks;
                     n_jobs += k->c.n;
+                    ctx->jobs += (int64_t)k->c.n;
                 }
-            in_flight_bytes -= ep->bytes;
-            sl.busy = false;
-            for (int kind = 0; kind < 2; ++kind)
-                for (auto *k : ep->chunks[kind]) ctx->jobs += (int64_t)k->c.n;
+            in_flight_bytes -= ep.bytes;
         }
-        delete ep;
         cv_complete.notify_all();
         cv_dispatch.notify_all();
     }
@@ -353,8 +319,7 @@ int bwg_server_create(bwg_ctx *ctx, uint64_t hbm_budget_bytes, int n_slots, bwg_
             return BWG_EIO;
         }
     }
-    s->dispatcher = std::thread([s] { s->dispatch_loop(); });
-    s->completer = std::thread([s] { s->complete_loop(); });
+    for (int k = 0; k < n_slots; ++k) s->workers.emplace_back([s, k] { s->slot_loop(k); });
     *out = s;
     return BWG_OK;
 }
@@ -364,7 +329,7 @@ int bwg_server_set_reference(bwg_server *s, const uint8_t *ref, uint64_t len) {
     for (uint64_t k = 0; k < len; ++k)
         if (ref[k] > 4) return BWG_EINVAL;
     std::unique_lock<std::mutex> g(s->mu);
-    if (!s->pending.empty() || !s->inflight.empty()) return BWG_EAGAIN;  // only while idle
+    if (!s->pending.empty() || s->in_flight_bytes) return BWG_EAGAIN;  // only while idle
     if (hipSetDevice(s->ctx->device) != hipSuccess) return BWG_EIO;
     if (s->d_ref) (void)hipFree(s->d_ref);
     s->d_ref = nullptr;
@@ -471,10 +436,8 @@ void bwg_server_destroy(bwg_server *s) {
         s->stop = true;
     }
     s->cv_dispatch.notify_all();
-    s->cv_done.notify_all();
-    if (s->dispatcher.joinable()) s->dispatcher.join();
-    s->cv_done.notify_all();
-    if (s->completer.joinable()) s->completer.join();
+    for (auto &w : s->workers)
+        if (w.joinable()) w.join();
     (void)hipSetDevice(s->ctx->device);
     for (auto &sl : s->slots) {
         sl.ss.destroy();

@@ -16,6 +16,9 @@
 //
 //  * ext_thread_kernel<QCAP>: one job per lane for short queries (qlen <= QCAP), the scalar
 //    recurrence with the eh arrays in LDS laid out [column][lane].
+#include <algorithm>
+#include <atomic>
+
 #include "bwg_batched_kernels.h"
 #include "dpp.h"
 
@@ -368,21 +371,33 @@ hipError_t launch_group(hipStream_t st, const uint8_t *arena, const uint8_t *ref
     return hipGetLastError();
 }
 
+// Occupancy queries are cached: they depend only on the kernel and its LDS size.
 template <int K>
 int group_occupancy(int qcap) {
+    static std::atomic<int> cache[130];
+    const int slot = std::min(qcap / 32, 129);
+    int v = cache[slot].load(std::memory_order_relaxed);
+    if (v > 0) return v;
     int blocks = 0;
     const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K>, 256, lds) != hipSuccess)
-        return 1;
-    return blocks > 0 ? blocks : 1;
+        blocks = 1;
+    v = blocks > 0 ? blocks : 1;
+    cache[slot].store(v, std::memory_order_relaxed);
+    return v;
 }
 
 template <int QCAP>
 int thread_occupancy() {
+    static std::atomic<int> cache{0};
+    int v = cache.load(std::memory_order_relaxed);
+    if (v > 0) return v;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_thread_kernel<QCAP>, 64, 0) != hipSuccess)
-        return 1;
-    return blocks > 0 ? blocks : 1;
+        blocks = 1;
+    v = blocks > 0 ? blocks : 1;
+    cache.store(v, std::memory_order_relaxed);
+    return v;
 }
 
 }  // namespace

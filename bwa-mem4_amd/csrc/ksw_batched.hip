@@ -17,6 +17,8 @@
 // the tracker.  The reverse start-recovery pass (REV = true) runs on the reversed prefixes
 // query[..=qe], target[..=te], read back to front in place, padded by its own lane rule, and stops
 // at the first row whose maximum reaches the forward score.
+#include <atomic>
+
 #include "bwg_batched_kernels.h"
 #include "dpp.h"
 
@@ -237,10 +239,15 @@ __global__ __launch_bounds__(256) void ksw_tracker_kernel(const KswJob *__restri
 
 template <int K, bool REV>
 int occ() {
+    static std::atomic<int> cache{0};
+    int v = cache.load(std::memory_order_relaxed);
+    if (v > 0) return v;
     int blocks = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ksw_group_kernel<K, REV>, 256, 0) != hipSuccess)
-        return 1;
-    return blocks > 0 ? blocks : 1;
+        blocks = 1;
+    v = blocks > 0 ? blocks : 1;
+    cache.store(v, std::memory_order_relaxed);
+    return v;
 }
 
 }  // namespace
