@@ -59,6 +59,21 @@ def cpu_baseline(ext_batch, mat, n_threads):
     return out, cells, dt
 
 
+def traffic_from_profile(jobs):
+    """HBM bytes per batch of the long-tail extension workload, from the committed rocprofv3
+    counter passes (FETCH_SIZE and WRITE_SIZE, each scaled by the copy-kernel calibration) of the
+    same workload; None when that summary is absent or was taken at another batch size."""
+    path = os.path.join(ROOT, "profiles", "ext_batched_longtail_1M_traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("jobs") != jobs:
+        return None
+    return t.get("bytes_per_batch")
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -70,10 +85,13 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
-    ap.add_argument("--only", default="ext,resc", help="comma list of workloads to time")
-    ap.add_argument("--serve-rates", default="2e5,2e6,1e7,3e7,6e7",
+    ap.add_argument("--only", default="ext,small,resc,e2e", help="comma list of workloads to time")
+    ap.add_argument("--serve-rates", default="5e4,2e5,2e6,1e7,3e7,6e7",
                     help="offered loads (jobs/s) of the serving sweep; empty to skip")
-    ap.add_argument("--serve-seconds", type=float, default=3.0)
+    ap.add_argument("--serve-seconds", type=float, default=3.0,
+                    help="seconds per offered load (longer at low loads, for at least ~40 chunks)")
+    ap.add_argument("--small-jobs", type=int, default=100_000,
+                    help="jobs of the small plain-extension workload (0 to skip)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -85,18 +103,18 @@ def main():
     mat = bwg.default_mat()
     ext_p = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
     ksw_p = bwg.ksw_params(5, mat, 6, 1, 6, 1, 20, 1)
-    log("generating %d long-tail extension jobs and %d rescue jobs" % (args.jobs, args.rescue_jobs))
+    log("generating %d long-tail extension jobs, %d small extension jobs and %d rescue jobs"
+        % (args.jobs, args.small_jobs, args.rescue_jobs))
     ext = bwg.generate(bwg.WORKLOAD_EXT_TAIL, args.jobs, seed=42)
     resc = bwg.generate(bwg.WORKLOAD_RESCUE, args.rescue_jobs, seed=42)
+    small = bwg.generate(bwg.WORKLOAD_EXT_SMALL, args.small_jobs, seed=42) if args.small_jobs else None
 
-    if args.calib_copy_mb:
-        import torch
-        src = torch.ones(args.calib_copy_mb << 18, dtype=torch.int32, device="cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log("calibration copy: %d MiB read + %d MiB written" % (args.calib_copy_mb, args.calib_copy_mb))
     ctx = bwg.Context(0)
+    copy_gbs = None
+    if args.calib_copy_mb:
+        copy_gbs = ctx.hbm_copy(args.calib_copy_mb << 20, reps=1)
+        log("calibration copy: hbm_copy_kernel, %d MiB read + %d MiB written per launch, %.0f GB/s"
+            % (args.calib_copy_mb, args.calib_copy_mb, copy_gbs))
     res = {}
     outs = {}
     forms = [f for f in ((bwg.FORM_BATCHED, "batched"), (bwg.FORM_PLAIN, "plain")) if f[1] in args.forms.split(",")]
@@ -119,6 +137,18 @@ def main():
         except bwg.BwgError as e:
             log("form %s unavailable: %s" % (fname, e))
         try:
+            if "small" not in only or small is None:
+                raise KeyError
+            st = ctx.stage(small, ext_p)
+            st.run()
+            per, kern, main, nl = time_steps(ctx, st, args.steps, args.warmup)
+            outs["small_" + fname] = st.results()
+            st.close()
+            res["small_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
+            log("small extension %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
+        except KeyError:
+            pass
+        try:
             if "resc" not in only:
                 raise KeyError
             log("rescue %s: staging" % fname)
@@ -133,13 +163,26 @@ def main():
             pass
         except bwg.BwgError as e:
             log("form %s unavailable: %s" % (fname, e))
+    e2e = None
+    if "e2e" in only and "ext" in only:
+        # PCIe-inclusive: the synchronous entry point from host buffers (stage, run, copy back)
+        ctx.set_form(bwg.FORM_BATCHED)
+        ctx.extend_arrays(ext, ext_p)
+        t0 = time.perf_counter()
+        reps = 3
+        for _ in range(reps):
+            ctx.extend_arrays(ext, ext_p)
+        e2e = args.jobs * reps / (time.perf_counter() - t0)
+        log("extension from host buffers (PCIe-inclusive): %.3g jobs/s" % e2e)
     serving = []
     rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
     if rates:
         ctx.set_form(bwg.FORM_BATCHED)
         srv = bwg.Server(ctx, hbm_budget_bytes=64 << 30, slots=3)
         for rate in rates:
-            r = bwg.serve_run(ctx, srv, rate, producers=8, duration_s=args.serve_seconds, seed=42,
+            # chunks average ~15k jobs: run long enough at low loads for ~40 of them
+            secs = max(args.serve_seconds, min(15.0, 40 * 15000.0 / rate))
+            r = bwg.serve_run(ctx, srv, rate, producers=8, duration_s=secs, seed=42,
                               use_ref=True, pool_chunks=64, verify=True)
             r["offered_jobs_per_s"] = rate
             serving.append(r)
@@ -156,9 +199,12 @@ def main():
         return
     if "ext_batched" in outs and "ext_plain" in outs:
         assert np.array_equal(outs["ext_batched"], outs["ext_plain"]), "forms disagree"
+    if "small_batched" in outs and "small_plain" in outs:
+        assert np.array_equal(outs["small_batched"], outs["small_plain"]), "forms disagree"
 
     cpu = None
     cells = None
+    small_cells = None
     if not args.no_cpu:
         log("cpu oracle over the extension batch on %d threads" % args.cpu_threads)
         cpu_out, cells, cpu_dt = cpu_baseline(ext, mat, args.cpu_threads)
@@ -172,6 +218,13 @@ def main():
         cpu = {"value": args.jobs / cpu_dt, "unit": "jobs/s", "cores": args.cpu_threads,
                "kind": "port", "gcups": cells / cpu_dt / 1e9,
                "sample": "all %d long-tail extension jobs, scalar C oracle, static partition" % args.jobs}
+        small_key = "small_batched" if "small_batched" in outs else "small_plain"
+        if small_key in outs:
+            s_out, small_cells, _ = cpu_baseline(small, mat, args.cpu_threads)
+            ok = np.array_equal(s_out, outs[small_key])
+            log("small workload bit-exact vs oracle: %s" % ok)
+            if not ok:
+                sys.exit(1)
 
     h = res[head]
     jobs_s = args.jobs / h["step_s"]
@@ -189,11 +242,25 @@ def main():
         line["gcups_kernel"] = cells / h["kernel_s"] / 1e9
         line["cells"] = cells
         ops = cells * K_EXT / h["kernel_s"] / 1e12
+        # algorithmic HBM bytes: query and target bytes, descriptor, order entry and result per job
+        alg_bytes = float(np.sum(ext.q_len.astype(np.int64) + ext.t_len)) + args.jobs * (32 + 4 + 24)
         line["roofline"] = {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS,
-                            "unit": "Tops/s", "frac": ops / VALU_PEAK_TOPS, "traffic": None}
+                            "unit": "Tops/s", "frac": ops / VALU_PEAK_TOPS,
+                            "traffic": traffic_from_profile(args.jobs),
+                            "hbm": {"achieved": alg_bytes / h["kernel_s"] / 1e9, "peak": HBM_PEAK_GBPS,
+                                    "unit": "GB/s", "frac": alg_bytes / h["kernel_s"] / 1e9 / HBM_PEAK_GBPS,
+                                    "alg_bytes": alg_bytes}}
     if "ext_plain" in res:
         line["plain_jobs_per_s"] = args.jobs / res["ext_plain"]["step_s"]
         line["speedup_vs_plain"] = jobs_s / line["plain_jobs_per_s"]
+    if e2e:
+        line["e2e_jobs_per_s"] = e2e
+    for f in ("batched", "plain"):
+        if "small_" + f in res:
+            r = res["small_" + f]
+            line["small_%s_jobs_per_s" % f] = args.small_jobs / r["step_s"]
+            if cpu and small_cells:
+                line["small_%s_gcups" % f] = small_cells / r["step_s"] / 1e9
     for f in ("batched", "plain"):
         if "resc_" + f in res:
             r = res["resc_" + f]
@@ -205,6 +272,9 @@ def main():
         line["p99_chunk_latency_ms"] = at_cfg["p99_ms"]
         line["serving"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                            for r in serving]
+    rep = ctx.tune_report()
+    if rep:
+        line["tuning"] = rep.splitlines()
     if cpu:
         line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)

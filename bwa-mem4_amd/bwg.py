@@ -32,6 +32,7 @@ KSW_DTYPE = np.dtype([(n, "<i4") for n in KswAlignResult._fields])
 BWG_OK, BWG_EIO, BWG_E2BIG, BWG_EAGAIN, BWG_ENOMEM, BWG_ENODEV, BWG_EINVAL = 0, -5, -7, -11, -12, -19, -22
 FORM_PLAIN, FORM_BATCHED = 0, 1
 OPT_FORM, OPT_CONCURRENT_BINS, OPT_GRAPHS, OPT_EXT_THREAD_MAX_Q = 1, 2, 3, 4
+OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP = 5, 6, 7
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 
 # Every symbol include/bwg.h declares.
@@ -40,7 +41,7 @@ EXPORTED_SYMBOLS = (
     "bwg_ksw_align2_batch", "bwg_set_option", "bwg_get_stat", "bwg_batch_stage_ext",
     "bwg_batch_stage_ksw", "bwg_batch_run", "bwg_batch_results", "bwg_batch_destroy",
     "bwg_server_create", "bwg_server_set_reference", "bwg_server_submit", "bwg_server_poll",
-    "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy",
+    "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy", "bwg_hbm_copy", "bwg_tune_report",
 )
 
 
@@ -100,6 +101,10 @@ def _lib():
         L.bwg_batch_results.argtypes = [_P, _P, _P]
         L.bwg_batch_destroy.restype = None
         L.bwg_batch_destroy.argtypes = [_P, _P]
+        L.bwg_hbm_copy.restype = C.c_int
+        L.bwg_hbm_copy.argtypes = [_P, C.c_size_t, C.c_int, _P]
+        L.bwg_tune_report.restype = C.c_int
+        L.bwg_tune_report.argtypes = [_P, C.c_char_p, C.c_size_t]
         _L = L
     return _L
 
@@ -210,6 +215,21 @@ class Context:
 
     def stat(self, key):
         return int(_lib().bwg_get_stat(self.h, key))
+
+    def tune_report(self):
+        """The autotuner's table: one line per tuned length class, chosen mapping starred."""
+        n = _lib().bwg_tune_report(self.h, None, 0)
+        _check(min(n, 0), "bwg_tune_report")
+        buf = C.create_string_buffer(n + 1)
+        _lib().bwg_tune_report(self.h, buf, n + 1)
+        return buf.value.decode()
+
+    def hbm_copy(self, nbytes, reps=1):
+        """Copy `nbytes` between two device buffers `reps` times with the library's streaming copy
+        kernel; returns the rate in GB/s counting bytes read plus bytes written."""
+        gbs = C.c_double(0.0)
+        _check(_lib().bwg_hbm_copy(self.h, int(nbytes), int(reps), C.byref(gbs)), "bwg_hbm_copy")
+        return gbs.value
 
     def name(self):
         return _lib().bwg_name(self.h).decode()

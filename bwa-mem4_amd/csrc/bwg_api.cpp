@@ -128,8 +128,29 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
         c->tuning.ext_thread_max_q16 = (int)std::min<int64_t>(v, 16);
         c->tuning.ext_thread_max_q32 = (int)v;
         return BWG_OK;
+    case BWG_OPT_AUTOTUNE:
+        c->tuning.autotune = v != 0;
+        c->tuner.clear();
+        return BWG_OK;
+    case BWG_OPT_EXT_K_STEP:
+        c->tuning.ext_k_step = v != 0;
+        return BWG_OK;
+    case BWG_OPT_KSW_K_STEP:
+        c->tuning.ksw_k_step = v != 0;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
+}
+
+int bwg_tune_report(const bwg_ctx *c, char *buf, size_t len) {
+    if (!c) return BWG_EINVAL;
+    const std::string r = c->tuner.report();
+    if (buf && len) {
+        const size_t m = std::min(len - 1, r.size());
+        std::memcpy(buf, r.data(), m);
+        buf[m] = 0;
+    }
+    return (int)r.size();
 }
 
 int64_t bwg_get_stat(const bwg_ctx *c, int key) {
@@ -139,6 +160,9 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_CONCURRENT_BINS: return c->tuning.concurrent_bins;
     case BWG_OPT_GRAPHS: return c->tuning.use_graphs;
     case BWG_OPT_EXT_THREAD_MAX_Q: return c->tuning.ext_thread_max_q32;
+    case BWG_OPT_AUTOTUNE: return c->tuning.autotune;
+    case BWG_OPT_EXT_K_STEP: return c->tuning.ext_k_step;
+    case BWG_OPT_KSW_K_STEP: return c->tuning.ksw_k_step;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;
@@ -146,6 +170,35 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_STAT_LAST_MAIN_LAUNCHES: return c->last_main_launches;
     default: return -1;
     }
+}
+
+int bwg_hbm_copy(bwg_ctx *c, size_t bytes, int reps, double *gb_per_s) {
+    if (!c || bytes < 16 || bytes % 16 || reps < 1) return BWG_EINVAL;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    void *src = nullptr, *dst = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMalloc(&src, bytes);
+    if (e == hipSuccess) e = hipMalloc(&dst, bytes);
+    if (e == hipSuccess) e = hipMemsetAsync(src, 1, bytes, c->ss.main);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    // 8 workgroups per CU keep enough 16-byte loads in flight to saturate HBM
+    const uint32_t blocks = (uint32_t)std::max(1, c->num_cus) * 8;
+    if (e == hipSuccess) e = launch_hbm_copy(c->ss.main, src, dst, bytes, blocks);  // warm-up
+    if (e == hipSuccess) e = hipEventRecord(e0, c->ss.main);
+    for (int r = 0; r < reps && e == hipSuccess; ++r) e = launch_hbm_copy(c->ss.main, src, dst, bytes, blocks);
+    if (e == hipSuccess) e = hipEventRecord(e1, c->ss.main);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    float ms = 0.f;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess && gb_per_s) *gb_per_s = 2.0 * (double)bytes * reps / (ms * 1e-3) / 1e9;
+    c->launches += reps + 1;
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    if (src) (void)hipFree(src);
+    if (dst) (void)hipFree(dst);
+    return bwg_map_hip(e);
 }
 
 int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,

@@ -22,11 +22,10 @@
 
 #include "bwg_batched_kernels.h"
 #include "bwg_runtime.h"
+#include "bwg_tuner.h"
 
 namespace bwg {
 namespace {
-
-enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain };
 
 struct Item {
     int kind = 0;
@@ -55,11 +54,24 @@ struct Plan {
     std::vector<KswJob> ksw_plain_jobs;
 };
 
-int odd_k_for(int width) {
-    static const int ks[] = {3, 5, 7, 9, 11, 13, 15, 17, 21, 25, 33};
-    for (int k : ks)
-        if (16 * k >= width) return k;
+// Autotuning: a class is tuned once the context sees a batch with at least kTuneMinJobs of its
+// jobs; each candidate runs kTuneReps times on up to kTuneSample of them and keeps its best time.
+constexpr size_t kTuneMinJobs = 4096;
+constexpr size_t kTuneSample = 65536;
+constexpr int kTuneReps = 3;
+
+constexpr int kExtKs[] = {3, 5, 7, 9, 11, 13, 15, 17, 21, 25, 33};
+constexpr int kNumExtKs = (int)(sizeof(kExtKs) / sizeof(kExtKs[0]));
+
+int odd_k_index(int width) {
+    for (int r = 0; r < kNumExtKs; ++r)
+        if (16 * kExtKs[r] >= width) return r;
     return -1;
+}
+
+int odd_k_for(int width) {
+    const int r = odd_k_index(width);
+    return r < 0 ? -1 : kExtKs[r];
 }
 
 int qcap_class(int q) { return std::max(32, (q + 31) / 32 * 32); }
@@ -139,6 +151,117 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
     return order;
 }
 
+// Persistent grid of a bin: enough workgroups to fill every CU at the kernel's occupancy, but no
+// more than the bin has work for.
+int ext_grid(const bwg_ctx *c, int kind, int param, int qcap, uint32_t count) {
+    if (kind == kExtGroup) {
+        const int per_cu = ext_group_blocks_per_cu(param, qcap);
+        return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ext_group_oversub,
+                                                           ((int64_t)count + 15) / 16));
+    }
+    const int per_cu = ext_thread_blocks_per_cu(param);
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)count + 63) / 64));
+}
+
+int ksw_grid(const bwg_ctx *c, int K, bool rev, uint32_t count) {
+    const int per_cu = ksw_group_blocks_per_cu(K, rev);
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ksw_group_oversub,
+                                                       ((int64_t)count + 15) / 16));
+}
+
+// Length class of an extension job (see bwg_tuner.h) and its minimal group-kernel K.
+int ext_class_of(const ExtJob &j, const ExtScoring &sc, const Tuning &tu, int *kidx) {
+    const int q = (int)j.qlen;
+    const int64_t bound = (int64_t)j.h0 + (int64_t)q * std::max(sc.max_sc, 0);
+    if (bound >= (1 << 18) || q >= 8192) return -1;
+    const int w = clamp_band(sc.w, q, sc);
+    *kidx = odd_k_index((int)std::min<int64_t>(q, 2 * (int64_t)w + 1));
+    if (q <= 32) return (q - 1) / 4;
+    if (*kidx < 0 || qcap_class(q) > tu.ext_group_max_qcap) return -1;
+    return kExtShortClasses + *kidx;
+}
+
+// Candidate mappings of a class, the default first.
+std::vector<TuneCandidate> ext_candidates(int cls, const Tuning &tu) {
+    std::vector<TuneCandidate> v;
+    auto add = [&](int kind, int param) {
+        TuneCandidate t;
+        t.kind = kind;
+        t.param = param;
+        v.push_back(t);
+    };
+    if (cls < kExtShortClasses) {
+        const int qmax = 4 * cls + 4;
+        if (qmax <= 16) add(kExtThread, 16);
+        add(kExtThread, 32);
+        add(kExtGroup, kExtKs[0]);
+    } else {
+        const int r = cls - kExtShortClasses;
+        add(kExtGroup, kExtKs[r]);
+        if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1]);
+    }
+    (void)tu;
+    return v;
+}
+
+// Time each candidate mapping of one class on a sample of the batch's own jobs (already resident
+// on the device) and record the fastest.  The sample kernels write the sampled jobs' true results,
+// which the batch's own launches overwrite with the same values.
+int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::vector<uint32_t> &members,
+                   hipStream_t st) {
+    std::lock_guard<std::mutex> g(c->tuner.tune_mu);
+    if (c->tuner.lookup(sig, cls, nullptr)) return BWG_OK;
+    const uint32_t S = (uint32_t)std::min<size_t>(members.size(), kTuneSample);
+    std::vector<uint32_t> idx(S);
+    int qcap = 32;
+    for (uint32_t r = 0; r < S; ++r) {
+        idx[r] = members[(size_t)r * members.size() / S];
+        qcap = std::max(qcap, qcap_class((int)b->hext[idx[r]].qlen));
+    }
+    TuneEntry ent;
+    ent.cands = ext_candidates(cls, c->tuning);
+    ent.sample = S;
+    uint32_t *d = (uint32_t *)c->pool.alloc(((size_t)S + 1) * 4);
+    if (!d) return BWG_ENOMEM;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipMemcpyAsync(d + 1, idx.data(), (size_t)S * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    std::vector<double> best(ent.cands.size(), 1e30);
+    for (int rep = 0; rep < kTuneReps && e == hipSuccess; ++rep) {
+        for (size_t k = 0; k < ent.cands.size() && e == hipSuccess; ++k) {
+            const TuneCandidate &cd = ent.cands[k];
+            e = launch_zero_u32(st, d, 1);
+            if (e == hipSuccess) e = hipEventRecord(e0, st);
+            const int grid = ext_grid(c, cd.kind, cd.param, qcap, S);
+            if (e == hipSuccess) {
+                if (cd.kind == kExtGroup)
+                    e = launch_ext_group(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S, d,
+                                         b->esc, qcap, (bwg_ext_result *)b->d_out, grid);
+                else
+                    e = launch_ext_thread(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S, d,
+                                          b->esc, (bwg_ext_result *)b->d_out, grid);
+            }
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess) best[k] = std::min(best[k], (double)ms * 1e6 / S);
+            c->launches += 2;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->pool.free(d);
+    if (e != hipSuccess) return bwg_map_hip(e);
+    for (size_t k = 0; k < best.size(); ++k) {
+        ent.cands[k].ns_per_job = best[k];
+        if (best[k] < best[ent.choice]) ent.choice = (int)k;
+    }
+    c->tuner.store(sig, cls, ent);
+    return BWG_OK;
+}
+
 int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
     const ExtScoring &sc = b->esc;
@@ -153,19 +276,49 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         bins.push_back(it);
         return (int)bins.size() - 1;
     };
+    // classify, then tune the classes this batch has enough jobs of and the table lacks
+    const int n_classes = kExtShortClasses + kNumExtKs;
+    std::vector<int8_t> cls(n), kidx(n);
+    std::vector<std::vector<uint32_t>> members(n_classes);
+    for (size_t k = 0; k < n; ++k) {
+        int ki = -1;
+        cls[k] = (int8_t)ext_class_of(b->hext[k], sc, tu, &ki);
+        kidx[k] = (int8_t)ki;
+        if (cls[k] >= 0) members[cls[k]].push_back((uint32_t)k);
+    }
+    const uint64_t sig = scoring_sig(sc);
+    std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0);
+    for (int cl = 0; cl < n_classes; ++cl) {
+        if (!tu.autotune) break;
+        if (members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+            const int rc = tune_ext_class(c, b, sig, cl, members[cl], st);
+            if (rc) return rc;
+        }
+        TuneEntry ent;
+        if (c->tuner.lookup(sig, cl, &ent)) {
+            choice_kind[cl] = ent.cands[ent.choice].kind;
+            choice_param[cl] = ent.cands[ent.choice].param;
+        }
+    }
     auto key_of = [&](size_t k) {
         const ExtJob &j = b->hext[k];
         const int q = (int)j.qlen;
-        const int64_t bound = (int64_t)j.h0 + (int64_t)q * std::max(sc.max_sc, 0);
-        if (bound >= (1 << 18) || q >= 8192) return find(kExtPlain, 0);
-        if (q <= tu.ext_thread_max_q16) return find(kExtThread, 16);
-        if (q <= tu.ext_thread_max_q32) return find(kExtThread, 32);
-        const int w = clamp_band(sc.w, q, sc);
-        const int K = odd_k_for((int)std::min<int64_t>(q, 2 * (int64_t)w + 1));
-        const int qc = qcap_class(q);
-        if (K < 0 || qc > tu.ext_group_max_qcap) return find(kExtPlain, 0);
-        const int bi = find(kExtGroup, K);
-        bins[bi].qcap = std::max(bins[bi].qcap, qc);
+        const int cl = cls[k];
+        if (cl < 0) return find(kExtPlain, 0);
+        int kind, param;
+        if (choice_kind[cl] >= 0) {
+            kind = choice_kind[cl];
+            param = choice_param[cl];
+        } else if (q <= tu.ext_thread_max_q16) {
+            kind = kExtThread, param = 16;
+        } else if (q <= tu.ext_thread_max_q32) {
+            kind = kExtThread, param = 32;
+        } else {
+            const int r = std::min(kidx[k] + (tu.ext_k_step ? 1 : 0), kNumExtKs - 1);
+            kind = kExtGroup, param = kExtKs[r];
+        }
+        const int bi = find(kind, param);
+        if (kind == kExtGroup) bins[bi].qcap = std::max(bins[bi].qcap, qcap_class(q));
         return bi;
     };
     auto work_of = [&](size_t k) { return (uint64_t)b->hext[k].qlen * (b->hext[k].tlen + 1); };
@@ -173,13 +326,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     for (size_t r = 0; r < bins.size(); ++r) {
         Item &it = bins[r];
         it.head = (uint32_t)r;
-        if (it.kind == kExtGroup) {
-            const int per_cu = ext_group_blocks_per_cu(it.param, it.qcap);
-            it.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ext_group_oversub,
-                                                                  (it.count + 15) / 16));
-        } else if (it.kind == kExtThread) {
-            const int per_cu = ext_thread_blocks_per_cu(it.param);
-            it.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, (it.count + 63) / 64));
+        if (it.kind == kExtGroup || it.kind == kExtThread) {
+            it.grid = ext_grid(c, it.kind, it.param, it.qcap, it.count);
         } else {
             // plain fallback: compact descriptors with per-job scratch offsets
             size_t total = 0;
@@ -207,6 +355,78 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     return bwg_map_hip(e);
 }
 
+// Time the rescue forward kernel at K and at the next K up on a sample of one class's jobs.  The
+// sample runs on its own descriptor copies (row-maximum slots tiled as in the batch plan) and writes
+// partial results to the first slots of the batch's output, which the batch's own launches
+// overwrite.
+int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::vector<uint32_t> &members,
+                   hipStream_t st) {
+    std::lock_guard<std::mutex> g(c->tuner.tune_mu);
+    if (c->tuner.lookup(sig, cls, nullptr)) return BWG_OK;
+    const uint32_t S = (uint32_t)std::min<size_t>(members.size(), kTuneSample / 2);
+    std::vector<KswJob> jobs(S);
+    std::vector<uint32_t> ident(S);
+    uint64_t rows = 0;
+    for (uint32_t t0 = 0; t0 < S; t0 += 64) {
+        const uint32_t t1 = std::min<uint32_t>(S, t0 + 64);
+        uint32_t tmax = 0;
+        for (uint32_t r = t0; r < t1; ++r) {
+            jobs[r] = b->hksw[members[(size_t)r * members.size() / S]];
+            jobs[r].aux0 = (uint32_t)(rows + (r - t0));
+            ident[r] = r;
+            tmax = std::max(tmax, jobs[r].tlen);
+        }
+        rows += 64ull * tmax;
+    }
+    TuneEntry ent;
+    ent.sample = S;
+    const int K = 2 * (cls + 1);
+    for (int k2 = K; k2 <= std::min(16, K + 2); k2 += 2) {
+        TuneCandidate t;
+        t.kind = kKswFwd;
+        t.param = k2;
+        ent.cands.push_back(t);
+    }
+    uint32_t *d = (uint32_t *)c->pool.alloc(((size_t)S + 1) * 4);
+    KswJob *dj = (KswJob *)c->pool.alloc((size_t)S * sizeof(KswJob));
+    int32_t *drm = (int32_t *)c->pool.alloc(std::max<uint64_t>(1, rows) * 4);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = (d && dj && drm) ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess) e = hipMemcpyAsync(d + 1, ident.data(), (size_t)S * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(dj, jobs.data(), (size_t)S * sizeof(KswJob), hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    std::vector<double> best(ent.cands.size(), 1e30);
+    for (int rep = 0; rep < kTuneReps && e == hipSuccess; ++rep) {
+        for (size_t k = 0; k < ent.cands.size() && e == hipSuccess; ++k) {
+            const int kk = ent.cands[k].param;
+            e = launch_zero_u32(st, d, 1);
+            if (e == hipSuccess) e = hipEventRecord(e0, st);
+            if (e == hipSuccess)
+                e = launch_ksw_group(st, kk, false, b->d_arena, b->d_ref, dj, d + 1, S, d, b->ksc,
+                                     (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, kk, false, S));
+            if (e == hipSuccess) e = hipEventRecord(e1, st);
+            if (e == hipSuccess) e = hipEventSynchronize(e1);
+            float ms = 0.f;
+            if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+            if (e == hipSuccess) best[k] = std::min(best[k], (double)ms * 1e6 / S);
+            c->launches += 2;
+        }
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    c->pool.free(d);
+    c->pool.free(dj);
+    c->pool.free(drm);
+    if (e != hipSuccess) return bwg_map_hip(e);
+    for (size_t k = 0; k < best.size(); ++k) {
+        ent.cands[k].ns_per_job = best[k];
+        if (best[k] < best[ent.choice]) ent.choice = (int)k;
+    }
+    c->tuner.store(sig, cls, ent);
+    return BWG_OK;
+}
+
 int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
     const KswScoring &sc = b->ksc;
@@ -222,9 +442,29 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         bins.push_back(it);
         return (int)bins.size() - 1;
     };
-    auto key_of = [&](size_t k) {
+    // class = the minimal K (index K/2-1); tune the classes with enough jobs
+    std::vector<int8_t> kmin(n);
+    std::vector<std::vector<uint32_t>> members(8);
+    for (size_t k = 0; k < n; ++k) {
         const int K = sc.o_ins >= 0 ? ksw_k_for(widest_pad((int)b->hksw[k].qlen, sc.max_sc)) : -1;
-        return K < 0 ? find(kKswPlain, 0) : find(kKswFwd, K);
+        kmin[k] = (int8_t)K;
+        if (K > 0) members[K / 2 - 1].push_back((uint32_t)k);
+    }
+    const uint64_t sig = scoring_sig(sc);
+    std::vector<int> chosen(8, 0);
+    for (int cl = 0; cl < 8; ++cl) {
+        chosen[cl] = tu.ksw_k_step ? std::min(16, 2 * (cl + 1) + 2) : 2 * (cl + 1);
+        if (!tu.autotune) continue;
+        if (members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+            const int rc = tune_ksw_class(c, b, sig, cl, members[cl], st);
+            if (rc) return rc;
+        }
+        TuneEntry ent;
+        if (c->tuner.lookup(sig, cl, &ent)) chosen[cl] = ent.cands[ent.choice].param;
+    }
+    auto key_of = [&](size_t k) {
+        const int K = kmin[k];
+        return K < 0 ? find(kKswPlain, 0) : find(kKswFwd, chosen[K / 2 - 1]);
     };
     auto work_of = [&](size_t k) { return (uint64_t)b->hksw[k].tlen; };
     p->order = bin_jobs(n, bins, key_of, work_of);
@@ -250,17 +490,13 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             Item f = it;
             f.stage = 0;
             f.head = (uint32_t)heads++;
-            const int gf = ksw_group_blocks_per_cu(it.param, false);
-            f.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)gf * c->num_cus * tu.ksw_group_oversub,
-                                                                 (it.count + 15) / 16));
+            f.grid = ksw_grid(c, it.param, false, it.count);
             items.push_back(f);
             Item r = it;
             r.kind = kKswRev;
             r.stage = 2;
             r.head = (uint32_t)heads++;
-            const int gr = ksw_group_blocks_per_cu(it.param, true);
-            r.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)gr * c->num_cus * tu.ksw_group_oversub,
-                                                                 (it.count + 15) / 16));
+            r.grid = ksw_grid(c, it.param, true, it.count);
             items.push_back(r);
         } else {
             Item pl = it;

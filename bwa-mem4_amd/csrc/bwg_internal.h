@@ -97,4 +97,7 @@ hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const 
                                     const uint32_t *out_index, uint32_t n, const KswScoring &sc,
                                     uint8_t *scratch, bwg_ksw_result *out);
 
+// Device-to-device copy of `bytes` (a multiple of 16) over `blocks` workgroups of 256 threads.
+hipError_t launch_hbm_copy(hipStream_t st, const void *src, void *dst, size_t bytes, uint32_t blocks);
+
 }  // namespace bwg

@@ -9,6 +9,7 @@
 #include <vector>
 
 #include "bwg_internal.h"
+#include "bwg_tuner.h"
 
 namespace bwg {
 
@@ -51,6 +52,9 @@ struct Tuning {
     int ext_group_max_qcap = 256;  // longest query the 16-lane group kernel keeps in LDS
     int ext_group_oversub = 1;     // persistent grid = oversub * occupancy * CUs
     int ksw_group_oversub = 1;
+    bool autotune = true;          // time candidate mappings per length class (bwg_tuner.h)
+    bool ext_k_step = false;       // untuned extension bins: one K above the minimal one
+    bool ksw_k_step = false;       // untuned rescue bins: one K above the minimal one
     bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
     bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph
 };
@@ -64,6 +68,7 @@ struct bwg_ctx {
     std::mutex run_mu;           // one synchronous batch on the context streams at a time
     int form = BWG_FORM_BATCHED;
     bwg::Tuning tuning;
+    bwg::Tuner tuner;
     std::atomic<int64_t> launches{0}, jobs{0};
     int64_t last_kernel_ns = 0, last_main_ns = 0, last_main_launches = 0;
     bwg::DevicePool pool;

@@ -266,6 +266,21 @@ hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n) {
     return hipGetLastError();
 }
 
+// Streaming copy of n 16-byte words, each read once and written once: the known-size workload that
+// hardware traffic counters (bytes fetched from / written to HBM per dispatch) are scaled against.
+__global__ void __launch_bounds__(256) hbm_copy_kernel(const uint4 *__restrict__ src, uint4 *__restrict__ dst,
+                                                       size_t n) {
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_hbm_copy(hipStream_t st, const void *src, void *dst, size_t bytes, uint32_t blocks) {
+    size_t n = bytes / sizeof(uint4);
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(hbm_copy_kernel, dim3(blocks), dim3(256), 0, st, (const uint4 *)src, (uint4 *)dst, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                             uint32_t n, const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out) {
     if (n == 0) return hipSuccess;

@@ -88,17 +88,26 @@ int bwg_ksw_align2_batch(bwg_ctx *ctx, const bwg_ksw_params *p, size_t n, const 
                          size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
                          const uint64_t *t_off, const uint32_t *t_len, bwg_ksw_result *out);
 
-/* ---- execution form and statistics ---------------------------------------------------------
+/* ---- execution form, tuning and statistics --------------------------------------------------
  * BWG_OPT_FORM selects how the kernels are mapped: BWG_FORM_PLAIN is one thread per job in
  * arrival order with per-job state in global memory and one launch per batch; BWG_FORM_BATCHED
  * (default) bins jobs by length and score range and runs each bin with the mapping the autotuner
- * picked.  Both give identical results. */
+ * picked.  Both give identical results.
+ *
+ * Autotuning (BWG_OPT_AUTOTUNE, default 1): the first batch with at least 4096 jobs of a length
+ * class times the class's candidate mappings on a sample of those jobs (one lane per job or 16
+ * lanes per job; the group kernel's columns per lane) and the context keeps the fastest for that
+ * scoring.  Setting the option clears the table.  With autotuning off, the mapping follows
+ * BWG_OPT_EXT_THREAD_MAX_Q and the K_STEP options, which force each candidate for tests. */
 enum { BWG_FORM_PLAIN = 0, BWG_FORM_BATCHED = 1 };
 enum {
     BWG_OPT_FORM = 1,
     BWG_OPT_CONCURRENT_BINS = 2,  /* batched: run a batch's independent bins concurrently (default 1) */
     BWG_OPT_GRAPHS = 3,           /* batched: replay repeated runs of a batch from a hipGraph (default 1) */
     BWG_OPT_EXT_THREAD_MAX_Q = 4, /* batched: longest query run one job per lane (0..32, default 32) */
+    BWG_OPT_AUTOTUNE = 5,         /* batched: time candidate mappings per length class (default 1) */
+    BWG_OPT_EXT_K_STEP = 6,       /* untuned extension group bins use one K above the minimal (default 0) */
+    BWG_OPT_KSW_K_STEP = 7,       /* untuned rescue bins use one K above the minimal (default 0) */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
@@ -107,6 +116,15 @@ enum {
 };
 int bwg_set_option(bwg_ctx *ctx, int key, int64_t value);
 int64_t bwg_get_stat(const bwg_ctx *ctx, int key);
+/* The autotuner's table as text, one line per tuned class with every candidate's time per job and
+ * the chosen one starred.  Writes at most len-1 bytes plus a terminating 0; returns the full
+ * length. */
+int bwg_tune_report(const bwg_ctx *ctx, char *buf, size_t len);
+
+/* Diagnostics, no counterpart in the reference: copy `bytes` (a multiple of 16) between two fresh
+ * device buffers `reps` times with a streaming kernel and report the copy rate, counting each byte
+ * read and each byte written.  Its known traffic calibrates hardware memory-traffic counters. */
+int bwg_hbm_copy(bwg_ctx *ctx, size_t bytes, int reps, double *gb_per_s);
 
 /* ---- device-resident batches ----------------------------------------------------------------
  * Stage a batch into device memory once (host-to-device copy), then run it any number of times

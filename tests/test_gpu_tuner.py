@@ -1,0 +1,95 @@
+"""The autotuner and every mapping it can choose: each candidate must give the oracle's results
+bit for bit, so the tuner's choice changes speed only."""
+import re
+import sys
+
+import numpy as np
+import pytest
+
+import bwg
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _first_bad(got, want):
+    bad = np.nonzero(got != want)[0]
+    return None if bad.size == 0 else (int(bad[0]), got[bad[0]], want[bad[0]])
+
+
+def test_forced_extension_mappings_agree(gpu_ctx):
+    mat = bwg.default_mat()
+    batch = bwg.generate(bwg.WORKLOAD_EXT_TAIL, 30000, seed=21)
+    params = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+    want = ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100)
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
+    try:
+        # thread kernel up to q=32 / up to q=16 / never; group kernels at the minimal K or one above
+        for tmax, kstep in [(32, 0), (16, 0), (0, 0), (32, 1), (0, 1)]:
+            _say("forced mapping thread_max_q=%d k_step=%d" % (tmax, kstep))
+            gpu_ctx.set_option(bwg.OPT_EXT_THREAD_MAX_Q, tmax)
+            gpu_ctx.set_option(bwg.OPT_EXT_K_STEP, kstep)
+            got = gpu_ctx.extend_arrays(batch, params)
+            assert _first_bad(got, want) is None, (tmax, kstep, _first_bad(got, want))
+    finally:
+        gpu_ctx.set_option(bwg.OPT_EXT_THREAD_MAX_Q, 32)
+        gpu_ctx.set_option(bwg.OPT_EXT_K_STEP, 0)
+        gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
+
+
+def test_forced_rescue_mappings_agree(gpu_ctx):
+    mat = bwg.default_mat()
+    batch = bwg.generate(bwg.WORKLOAD_RESCUE, 3000, seed=22)
+    params = bwg.ksw_params(5, mat, 6, 1, 6, 1, 20, 1)
+    want = ob.align2_batch(batch, mat, 6, 1, 6, 1, 20, 1)
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
+    try:
+        for kstep in (0, 1):
+            _say("forced rescue k_step=%d" % kstep)
+            gpu_ctx.set_option(bwg.OPT_KSW_K_STEP, kstep)
+            got = gpu_ctx.align2_arrays(batch, params)
+            assert _first_bad(got, want) is None, (kstep, _first_bad(got, want))
+    finally:
+        gpu_ctx.set_option(bwg.OPT_KSW_K_STEP, 0)
+        gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
+
+
+def test_autotuner_tunes_once_and_stays_exact():
+    _say("autotuner table")
+    ctx = bwg.Context(0)
+    try:
+        mat = bwg.default_mat()
+        ext = bwg.generate(bwg.WORKLOAD_EXT_TAIL, 200000, seed=23)
+        ep = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+        resc = bwg.generate(bwg.WORKLOAD_RESCUE, 5000, seed=23)
+        kp = bwg.ksw_params(5, mat, 6, 1, 6, 1, 20, 1)
+        assert ctx.tune_report() == ""
+        ext_want = ob.extend_batch(ext, mat, 6, 1, 6, 1, 100, 5, 100)
+        ext_got = ctx.extend_arrays(ext, ep)
+        assert _first_bad(ext_got, ext_want) is None
+        got = ctx.align2_arrays(resc, kp)
+        assert _first_bad(got, ob.align2_batch(resc, mat, 6, 1, 6, 1, 20, 1)) is None
+        rep = ctx.tune_report()
+        _say(rep)
+        lines = [l for l in rep.splitlines() if l]
+        assert any(l.startswith("extend qlen") for l in lines)   # short queries: lanes per job
+        assert any(l.startswith("extend window") for l in lines)  # group kernels: columns per lane
+        assert any(l.startswith("rescue") for l in lines)
+        for l in lines:
+            assert l.count("*") == 1, l
+            times = [float(x) for x in re.findall(r"([0-9.]+) ns/job", l)]
+            assert len(times) >= 2 and all(t > 0 for t in times), l
+            chosen = float(re.search(r"\*\S+ ([0-9.]+) ns/job", l).group(1))
+            assert chosen == min(times), l
+        # a second batch with the same scoring reuses the table and gives the same results
+        again = ctx.extend_arrays(ext, ep)
+        assert ctx.tune_report() == rep
+        assert np.array_equal(again, ext_got)
+    finally:
+        ctx.close()
