@@ -92,6 +92,8 @@ def main():
                     help="seconds per offered load (longer at low loads, for at least ~40 chunks)")
     ap.add_argument("--small-jobs", type=int, default=100_000,
                     help="jobs of the small plain-extension workload (0 to skip)")
+    ap.add_argument("--serial-bins", action="store_true",
+                    help="run a batch's bins one after another on one stream (for counter runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
     args = ap.parse_args()
@@ -110,6 +112,8 @@ def main():
     small = bwg.generate(bwg.WORKLOAD_EXT_SMALL, args.small_jobs, seed=42) if args.small_jobs else None
 
     ctx = bwg.Context(0)
+    if args.serial_bins:
+        ctx.set_option(bwg.OPT_CONCURRENT_BINS, 0)
     copy_gbs = None
     if args.calib_copy_mb:
         copy_gbs = ctx.hbm_copy(args.calib_copy_mb << 20, reps=1)

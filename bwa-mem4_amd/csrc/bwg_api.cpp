@@ -30,12 +30,6 @@ int check_mat(int32_t m, const int8_t *mat) {
     return m == 5 ? BWG_OK : BWG_EINVAL;
 }
 
-int32_t mat_max(const int8_t *mat) {
-    int32_t mx = mat[0];
-    for (int k = 1; k < 25; ++k) mx = std::max<int32_t>(mx, mat[k]);
-    return mx;
-}
-
 bool codes_ok(const uint8_t *p, size_t len) {
     uint8_t acc = 0;
     for (size_t k = 0; k < len; ++k) acc |= (uint8_t)(p[k] > 4);
@@ -138,6 +132,11 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
     case BWG_OPT_KSW_K_STEP:
         c->tuning.ksw_k_step = v != 0;
         return BWG_OK;
+    case BWG_OPT_EXT_SCORE_WIDTH:
+        if (v != 16 && v != 32) return BWG_EINVAL;
+        c->tuning.ext_force_wide = v == 32;
+        c->tuner.clear();
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -163,6 +162,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_AUTOTUNE: return c->tuning.autotune;
     case BWG_OPT_EXT_K_STEP: return c->tuning.ext_k_step;
     case BWG_OPT_KSW_K_STEP: return c->tuning.ksw_k_step;
+    case BWG_OPT_EXT_SCORE_WIDTH: return c->tuning.ext_force_wide ? 32 : 16;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;
@@ -214,16 +214,7 @@ int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uin
     if (rc) return rc;
     for (size_t k = 0; k < n; ++k)
         if (h0[k] <= 0 || q_len[k] < 1) return BWG_EINVAL;
-    ExtScoring sc{};
-    std::memcpy(sc.mat, p->mat, 25);
-    sc.o_del = p->o_del;
-    sc.e_del = p->e_del;
-    sc.o_ins = p->o_ins;
-    sc.e_ins = p->e_ins;
-    sc.w = p->w;
-    sc.end_bonus = p->end_bonus;
-    sc.zdrop = p->zdrop;
-    sc.max_sc = mat_max(p->mat);
+    ExtScoring sc = ext_scoring(*p);
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     std::lock_guard<std::mutex> g(c->run_mu);
     rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, c->ss.main, out);
@@ -241,14 +232,7 @@ int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uin
     if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
     rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
     if (rc) return rc;
-    KswScoring sc{};
-    std::memcpy(sc.mat, p->mat, 25);
-    sc.o_del = p->o_del;
-    sc.e_del = p->e_del;
-    sc.o_ins = p->o_ins;
-    sc.e_ins = p->e_ins;
-    sc.minsc = p->minsc;
-    sc.max_sc = p->max_sc;
+    KswScoring sc = ksw_scoring(*p);
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     std::lock_guard<std::mutex> g(c->run_mu);
     rc = stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, c->ss.main, out);

@@ -30,6 +30,7 @@ namespace {
 struct Item {
     int kind = 0;
     int param = 0;     // K, or the thread kernel's column capacity
+    bool narrow = false;  // extension: 16-bit column state
     int qcap = 0;      // extension group: LDS sizing
     int stage = 0;     // items of one stage run concurrently; stages run in order
     uint32_t offset = 0, count = 0, head = 0;
@@ -153,13 +154,13 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
 
 // Persistent grid of a bin: enough workgroups to fill every CU at the kernel's occupancy, but no
 // more than the bin has work for.
-int ext_grid(const bwg_ctx *c, int kind, int param, int qcap, uint32_t count) {
+int ext_grid(const bwg_ctx *c, int kind, int param, bool narrow, int qcap, uint32_t count) {
     if (kind == kExtGroup) {
         const int per_cu = ext_group_blocks_per_cu(param, qcap);
         return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ext_group_oversub,
                                                            ((int64_t)count + 15) / 16));
     }
-    const int per_cu = ext_thread_blocks_per_cu(param);
+    const int per_cu = ext_thread_blocks_per_cu(param, narrow);
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)count + 63) / 64));
 }
 
@@ -176,31 +177,33 @@ int ext_class_of(const ExtJob &j, const ExtScoring &sc, const Tuning &tu, int *k
     if (bound >= (1 << 18) || q >= 8192) return -1;
     const int w = clamp_band(sc.w, q, sc);
     *kidx = odd_k_index((int)std::min<int64_t>(q, 2 * (int64_t)w + 1));
-    if (q <= 32) return (q - 1) / 4;
-    if (*kidx < 0 || qcap_class(q) > tu.ext_group_max_qcap) return -1;
-    return kExtShortClasses + *kidx;
+    if (q > 32 && (*kidx < 0 || qcap_class(q) > tu.ext_group_max_qcap)) return -1;
+    if (bound >= 65536 || tu.ext_force_wide) return -2;
+    return q <= 32 ? (q - 1) / 4 : kExtShortClasses + *kidx;
 }
 
-// Candidate mappings of a class, the default first.
-std::vector<TuneCandidate> ext_candidates(int cls, const Tuning &tu) {
+// Candidate mappings of a class, the default first: lanes per job (thread or group kernel), the
+// group kernel's columns per lane (minimal K or the next), and the stored score width.
+std::vector<TuneCandidate> ext_candidates(int cls) {
     std::vector<TuneCandidate> v;
-    auto add = [&](int kind, int param) {
+    auto add = [&](int kind, int param, int width) {
         TuneCandidate t;
         t.kind = kind;
         t.param = param;
+        t.width = width;
         v.push_back(t);
     };
     if (cls < kExtShortClasses) {
         const int qmax = 4 * cls + 4;
-        if (qmax <= 16) add(kExtThread, 16);
-        add(kExtThread, 32);
-        add(kExtGroup, kExtKs[0]);
+        if (qmax <= 16) add(kExtThread, 16, 16);
+        add(kExtThread, 32, 16);
+        add(kExtGroup, kExtKs[0], 32);
+        add(kExtThread, qmax <= 16 ? 16 : 32, 32);
     } else {
         const int r = cls - kExtShortClasses;
-        add(kExtGroup, kExtKs[r]);
-        if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1]);
+        add(kExtGroup, kExtKs[r], 32);
+        if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1], 32);
     }
-    (void)tu;
     return v;
 }
 
@@ -219,7 +222,7 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
         qcap = std::max(qcap, qcap_class((int)b->hext[idx[r]].qlen));
     }
     TuneEntry ent;
-    ent.cands = ext_candidates(cls, c->tuning);
+    ent.cands = ext_candidates(cls);
     ent.sample = S;
     uint32_t *d = (uint32_t *)c->pool.alloc(((size_t)S + 1) * 4);
     if (!d) return BWG_ENOMEM;
@@ -233,14 +236,15 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
             const TuneCandidate &cd = ent.cands[k];
             e = launch_zero_u32(st, d, 1);
             if (e == hipSuccess) e = hipEventRecord(e0, st);
-            const int grid = ext_grid(c, cd.kind, cd.param, qcap, S);
+            const bool narrow = cd.width == 16;
+            const int grid = ext_grid(c, cd.kind, cd.param, narrow, qcap, S);
             if (e == hipSuccess) {
                 if (cd.kind == kExtGroup)
                     e = launch_ext_group(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S, d,
                                          b->esc, qcap, (bwg_ext_result *)b->d_out, grid);
                 else
-                    e = launch_ext_thread(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S, d,
-                                          b->esc, (bwg_ext_result *)b->d_out, grid);
+                    e = launch_ext_thread(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1,
+                                          S, d, b->esc, (bwg_ext_result *)b->d_out, grid);
             }
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -267,12 +271,13 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const ExtScoring &sc = b->esc;
     const Tuning &tu = c->tuning;
     std::vector<Item> bins;
-    auto find = [&](int kind, int param) {
+    auto find = [&](int kind, int param, bool narrow) {
         for (size_t k = 0; k < bins.size(); ++k)
-            if (bins[k].kind == kind && bins[k].param == param) return (int)k;
+            if (bins[k].kind == kind && bins[k].param == param && bins[k].narrow == narrow) return (int)k;
         Item it;
         it.kind = kind;
         it.param = param;
+        it.narrow = narrow;
         bins.push_back(it);
         return (int)bins.size() - 1;
     };
@@ -287,7 +292,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         if (cls[k] >= 0) members[cls[k]].push_back((uint32_t)k);
     }
     const uint64_t sig = scoring_sig(sc);
-    std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0);
+    std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0), choice_width(n_classes, 16);
     for (int cl = 0; cl < n_classes; ++cl) {
         if (!tu.autotune) break;
         if (members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
@@ -298,17 +303,20 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         if (c->tuner.lookup(sig, cl, &ent)) {
             choice_kind[cl] = ent.cands[ent.choice].kind;
             choice_param[cl] = ent.cands[ent.choice].param;
+            choice_width[cl] = ent.cands[ent.choice].width;
         }
     }
     auto key_of = [&](size_t k) {
         const ExtJob &j = b->hext[k];
         const int q = (int)j.qlen;
         const int cl = cls[k];
-        if (cl < 0) return find(kExtPlain, 0);
+        if (cl == -1) return find(kExtPlain, 0, false);
         int kind, param;
-        if (choice_kind[cl] >= 0) {
+        bool narrow = cl >= 0;
+        if (cl >= 0 && choice_kind[cl] >= 0) {
             kind = choice_kind[cl];
             param = choice_param[cl];
+            narrow = choice_width[cl] == 16;
         } else if (q <= tu.ext_thread_max_q16) {
             kind = kExtThread, param = 16;
         } else if (q <= tu.ext_thread_max_q32) {
@@ -317,7 +325,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             const int r = std::min(kidx[k] + (tu.ext_k_step ? 1 : 0), kNumExtKs - 1);
             kind = kExtGroup, param = kExtKs[r];
         }
-        const int bi = find(kind, param);
+        if (kind == kExtGroup) narrow = false;  // the group kernel keeps 32-bit state
+        const int bi = find(kind, param, narrow);
         if (kind == kExtGroup) bins[bi].qcap = std::max(bins[bi].qcap, qcap_class(q));
         return bi;
     };
@@ -327,7 +336,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         Item &it = bins[r];
         it.head = (uint32_t)r;
         if (it.kind == kExtGroup || it.kind == kExtThread) {
-            it.grid = ext_grid(c, it.kind, it.param, it.qcap, it.count);
+            it.grid = ext_grid(c, it.kind, it.param, it.narrow, it.qcap, it.count);
         } else {
             // plain fallback: compact descriptors with per-job scratch offsets
             size_t total = 0;
@@ -546,7 +555,7 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
         return launch_ext_group(st, it.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order + it.offset,
                                 it.count, p->d_heads + it.head, b->esc, it.qcap, (bwg_ext_result *)b->d_out, it.grid);
     case kExtThread:
-        return launch_ext_thread(st, it.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order + it.offset,
+        return launch_ext_thread(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order + it.offset,
                                  it.count, p->d_heads + it.head, b->esc, (bwg_ext_result *)b->d_out, it.grid);
     case kExtPlain:
         return launch_ext_plain_scatter(st, b->d_arena, b->d_ref, (const ExtJob *)p->d_plain_jobs,

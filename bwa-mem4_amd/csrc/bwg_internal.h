@@ -54,11 +54,45 @@ struct ExtScoring {
     int8_t mat[28];
     int32_t o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop;
     int32_t max_sc;
+    int32_t small_ext;  // e_del, e_ins < 2^14: gap products of row/column distances fit 24-bit multiplies
 };
 struct KswScoring {
     int8_t mat[28];
     int32_t o_del, e_del, o_ins, e_ins, minsc, max_sc;
 };
+
+inline int32_t mat_max(const int8_t *mat) {
+    int32_t mx = mat[0];
+    for (int k = 1; k < 25; ++k) mx = mat[k] > mx ? mat[k] : mx;
+    return mx;
+}
+
+inline ExtScoring ext_scoring(const bwg_ext_params &p) {
+    ExtScoring sc{};
+    for (int k = 0; k < 25; ++k) sc.mat[k] = p.mat[k];
+    sc.o_del = p.o_del;
+    sc.e_del = p.e_del;
+    sc.o_ins = p.o_ins;
+    sc.e_ins = p.e_ins;
+    sc.w = p.w;
+    sc.end_bonus = p.end_bonus;
+    sc.zdrop = p.zdrop;
+    sc.max_sc = mat_max(p.mat);
+    sc.small_ext = p.e_del < (1 << 14) && p.e_ins < (1 << 14);
+    return sc;
+}
+
+inline KswScoring ksw_scoring(const bwg_ksw_params &p) {
+    KswScoring sc{};
+    for (int k = 0; k < 25; ++k) sc.mat[k] = p.mat[k];
+    sc.o_del = p.o_del;
+    sc.e_del = p.e_del;
+    sc.o_ins = p.o_ins;
+    sc.e_ins = p.e_ins;
+    sc.minsc = p.minsc;
+    sc.max_sc = p.max_sc;
+    return sc;
+}
 
 // Integer closed form of ksw_extend2's f64 band clamp (crates/bwa-extend/src/sw.rs:288-308):
 // trunc((double)n/e + 1.0) floored at 1 equals (n <= 0 ? 1 : n/e + 1) for n = qlen*max_sc +

@@ -55,6 +55,7 @@ struct Tuning {
     bool autotune = true;          // time candidate mappings per length class (bwg_tuner.h)
     bool ext_k_step = false;       // untuned extension bins: one K above the minimal one
     bool ksw_k_step = false;       // untuned rescue bins: one K above the minimal one
+    bool ext_force_wide = false;   // extension: 32-bit column state even where 16 bits are exact
     bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
     bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph
 };

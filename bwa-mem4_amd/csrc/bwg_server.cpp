@@ -61,12 +61,6 @@ bool same_params(const Chunk *a, const Chunk *b) {
                                          : std::memcmp(&a->ksw, &b->ksw, sizeof(a->ksw)) == 0;
 }
 
-int32_t mat_max(const int8_t *mat) {
-    int32_t mx = mat[0];
-    for (int k = 1; k < 25; ++k) mx = std::max<int32_t>(mx, mat[k]);
-    return mx;
-}
-
 }  // namespace
 
 struct bwg_server {
@@ -164,28 +158,12 @@ static int stage_kind(bwg_server *s, Epoch *ep, int kind, size_t in_off) {
     int rc;
     if (kind == 0) {
         const bwg_ext_params &p = v[0]->ext;
-        bwg::ExtScoring sc{};
-        std::memcpy(sc.mat, p.mat, 25);
-        sc.o_del = p.o_del;
-        sc.e_del = p.e_del;
-        sc.o_ins = p.o_ins;
-        sc.e_ins = p.e_ins;
-        sc.w = p.w;
-        sc.end_bonus = p.end_bonus;
-        sc.zdrop = p.zdrop;
-        sc.max_sc = mat_max(p.mat);
+        bwg::ExtScoring sc = bwg::ext_scoring(p);
         rc = bwg::stage_ext(s->ctx, sc, n, arena, a, q_off.data(), q_len.data(), t_off.data(), t_len.data(), h0.data(),
                             s->d_ref, sl.ss.main, &ep->batch[0]);
     } else {
         const bwg_ksw_params &p = v[0]->ksw;
-        bwg::KswScoring sc{};
-        std::memcpy(sc.mat, p.mat, 25);
-        sc.o_del = p.o_del;
-        sc.e_del = p.e_del;
-        sc.o_ins = p.o_ins;
-        sc.e_ins = p.e_ins;
-        sc.minsc = p.minsc;
-        sc.max_sc = p.max_sc;
+        bwg::KswScoring sc = bwg::ksw_scoring(p);
         rc = bwg::stage_ksw(s->ctx, sc, n, arena, a, q_off.data(), q_len.data(), t_off.data(), t_len.data(), s->d_ref,
                             sl.ss.main, &ep->batch[1]);
     }

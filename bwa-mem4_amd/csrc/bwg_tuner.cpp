@@ -29,10 +29,10 @@ uint64_t scoring_sig(const KswScoring &sc) {
     return h;
 }
 
-const char *mapping_name(int kind, int param, char *buf, size_t len) {
+const char *mapping_name(int kind, int param, int width, char *buf, size_t len) {
     switch (kind) {
-    case kExtThread: std::snprintf(buf, len, "ext_thread<%d>", param); break;
-    case kExtGroup: std::snprintf(buf, len, "ext_group<%d>", param); break;
+    case kExtThread: std::snprintf(buf, len, "ext_thread<%d,i%d>", param, width); break;
+    case kExtGroup: std::snprintf(buf, len, "ext_group<%d,i%d>", param, width); break;
     case kKswFwd: std::snprintf(buf, len, "ksw_group<%d>", param); break;
     default: std::snprintf(buf, len, "plain"); break;
     }
@@ -77,7 +77,7 @@ std::string Tuner::report() const {
         for (size_t k = 0; k < e.cands.size() && n < (int)sizeof line - 80; ++k)
             n += std::snprintf(line + n, sizeof line - n, " %s%s %.3f ns/job",
                                (int)k == e.choice ? "*" : "",
-                               mapping_name(e.cands[k].kind, e.cands[k].param, name, sizeof name),
+                               mapping_name(e.cands[k].kind, e.cands[k].param, e.cands[k].width, name, sizeof name),
                                e.cands[k].ns_per_job);
         s += line;
         s += '\n';

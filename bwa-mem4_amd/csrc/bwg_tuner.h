@@ -19,6 +19,7 @@ enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev,
 struct TuneCandidate {
     int kind = 0;
     int param = 0;            // K, or the thread kernel's column capacity
+    int width = 32;           // bits of the stored DP state (16: narrow, 32: wide)
     double ns_per_job = 0.0;  // best of the timed repetitions
 };
 
@@ -44,12 +45,13 @@ class Tuner {
 };
 
 // Extension length classes: queries up to 32 in buckets of 4 (classes 0..7), longer queries by the
-// group kernel's minimal K for their widest window (classes 8..).  -1: always the plain kernel.
+// group kernel's minimal K for their widest window (classes 8..).  Only jobs whose scores fit the
+// narrow state are classed and tuned; -2: wide state, untuned; -1: always the plain kernel.
 constexpr int kExtShortClasses = 8;
 // Rescue classes: the minimal K of ksw_group_kernel (classes 0..7 for K = 2..16).
 
 uint64_t scoring_sig(const ExtScoring &sc);
 uint64_t scoring_sig(const KswScoring &sc);
-const char *mapping_name(int kind, int param, char *buf, size_t len);
+const char *mapping_name(int kind, int param, int width, char *buf, size_t len);
 
 }  // namespace bwg

@@ -16,19 +16,21 @@ __device__ __forceinline__ int dpp(int old, int src) {
     return __builtin_amdgcn_update_dpp(old, src, CTRL, 0xf, 0xf, BOUND_ZERO);
 }
 
-// Max over the 16 lanes of the row, result in every lane of the row.
+// Max over the 16 lanes of the row, result in every lane of the row.  A rotation reads no lane
+// outside the row, so the bound-control fill never applies; asking for it lets the compiler fold
+// each move into the max as a DPP operand.
 __device__ __forceinline__ int row_allmax(int x) {
-    x = max(x, dpp<kRowRor + 1, false>(x, x));
-    x = max(x, dpp<kRowRor + 2, false>(x, x));
-    x = max(x, dpp<kRowRor + 4, false>(x, x));
-    x = max(x, dpp<kRowRor + 8, false>(x, x));
+    x = max(x, dpp<kRowRor + 1, true>(0, x));
+    x = max(x, dpp<kRowRor + 2, true>(0, x));
+    x = max(x, dpp<kRowRor + 4, true>(0, x));
+    x = max(x, dpp<kRowRor + 8, true>(0, x));
     return x;
 }
 __device__ __forceinline__ int row_allmin(int x) {
-    x = min(x, dpp<kRowRor + 1, false>(x, x));
-    x = min(x, dpp<kRowRor + 2, false>(x, x));
-    x = min(x, dpp<kRowRor + 4, false>(x, x));
-    x = min(x, dpp<kRowRor + 8, false>(x, x));
+    x = min(x, dpp<kRowRor + 1, true>(0, x));
+    x = min(x, dpp<kRowRor + 2, true>(0, x));
+    x = min(x, dpp<kRowRor + 4, true>(0, x));
+    x = min(x, dpp<kRowRor + 8, true>(0, x));
     return x;
 }
 

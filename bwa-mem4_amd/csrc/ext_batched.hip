@@ -21,6 +21,7 @@
 
 #include "bwg_batched_kernels.h"
 #include "dpp.h"
+#include "target_stream.h"
 
 namespace bwg {
 namespace {
@@ -36,7 +37,41 @@ struct GroupJob {
     // row state
     int32_t i, beg, end, best, best_i, best_j, gbest_i, gscore, max_off;
     int32_t tcur;  // target code of row i
+    int32_t h1base;  // h0 - (o_del + e_del*(i+1)), kept as a running difference
 };
+
+// Column state.  The reference's eh arrays hold {H[j], E[j]} per absolute column.  Wide storage
+// keeps them as two int32; narrow storage packs H into the low and E into the high 16 bits of one
+// word, exact whenever every value fits in 16 bits: H and E are never negative and never exceed
+// h0 + qlen*max_sc, which the planner checks is below 65536 for narrow bins.  Narrow storage halves
+// the block's LDS footprint, which is what limits the group kernel's occupancy.
+template <bool NARROW> struct ColState;
+template <> struct ColState<true> {
+    using word = uint32_t;
+    static __device__ __forceinline__ word pack(int32_t h, int32_t e) { return (uint32_t)h | ((uint32_t)e << 16); }
+    static __device__ __forceinline__ int32_t h_of(word w) { return (int32_t)(w & 0xffffu); }
+    static __device__ __forceinline__ int32_t e_of(word w) { return (int32_t)(w >> 16); }
+    static __device__ __forceinline__ bool live(word w) { return w != 0; }
+};
+template <> struct ColState<false> {
+    using word = int2;
+    static __device__ __forceinline__ word pack(int32_t h, int32_t e) { return make_int2(h, e); }
+    static __device__ __forceinline__ int32_t h_of(word w) { return w.x; }
+    static __device__ __forceinline__ int32_t e_of(word w) { return w.y; }
+    static __device__ __forceinline__ bool live(word w) { return (w.x | w.y) != 0; }
+};
+
+// Substitution score of query code `qc` against the target code whose matrix row is `row`
+// (mat[t][0..4] as int8 in the low five bytes of the 64-bit pair {hi:lo}).
+__device__ __forceinline__ int32_t subst(uint2 row, uint32_t qc) {
+    return (int32_t)(int8_t)(uint8_t)__builtin_amdgcn_perm(row.y, row.x, qc);
+}
+
+// The substitution table of the block, written once before the persistent loop.
+__device__ __forceinline__ void fill_subst_table(int8_t *tbl, const ExtScoring &sc) {
+    const int t = threadIdx.x;
+    if (t < 40) tbl[t] = (t & 7) < 5 ? sc.mat[(t >> 3) * 5 + (t & 7)] : (int8_t)0;
+}
 
 // Block LDS layout (bytes): the eh pairs {H[j], E[j]} of every group, (qcap+2) columns each; then
 // every group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes);
@@ -106,6 +141,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 J.best_i = J.best_j = J.gbest_i = J.gscore = -1;
                 J.max_off = 0;
                 J.tcur = J.tlen > 0 ? J.t[0] : 0;
+                J.h1base = (int32_t)((uint32_t)J.h0 - (uint32_t)sc.o_del - (uint32_t)e_del);
                 active = J.tlen > 0;
                 if (!active && g == 0) {
                     bwg_ext_result r = {J.h0, 0, 0, 0, -1, 0};
@@ -124,13 +160,13 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         if (beg < i - J.w) beg = i - J.w;
         if (end > i + J.w + 1) end = i + J.w + 1;
         if (end > J.qlen) end = J.qlen;
-        const int32_t h1_init = beg == 0 ? max(J.h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
+        const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
         const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
 
         const int32_t base = beg + g * K;  // first window column of this lane
         const int32_t nval = min(max(end - base, 0), K);
         const int2 *ehb = (const int2 *)eh + base;
-        const int8_t *sb = planes + J.tcur * qcap + base;
+        const int8_t *sb = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap) + base;
         // pass 1: loads of all K columns are unconditional and independent, so they issue back to
         // back; columns past the window compute values that are never used
         int2 v[K];
@@ -163,7 +199,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             const int32_t h = max(max(mm[k], v[k].y), f);
             const int32_t e = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
             if (k < nval) {
-                key = max(key, (h << kKeyShift) | (base + k));
+                // lane-relative column; the lane's first column is added once after the loop
+                key = max(key, (h << kKeyShift) | k);
                 ehw[k].y = e;       // E(i+1, j)
                 ehw[k + 1].x = h;   // H(i, j): the diagonal of column j+1 next row
             }
@@ -173,7 +210,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             eh[2 * end + 1] = 0;    // E at the sentinel column
         }
         wave_lds_sync();
-        key = row_allmax(key);
+        key = row_allmax(nval > 0 ? key + base : 0);  // h << 13 | absolute column
         const int32_t row_max = key >> kKeyShift;
         const int32_t mj = key & kKeyMask;
         const int2 at_end = *(const int2 *)(eh + 2 * end);  // H(i, end-1) (h1_init if empty)
@@ -195,11 +232,9 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             J.max_off = max(J.max_off, abs(mj - i));
         } else if (sc.zdrop > 0) {
             const int32_t di = i - J.best_i, dj = mj - J.best_j;
-            if (di > dj) {
-                if (J.best - row_max - (di - dj) * e_del > sc.zdrop) done = true;
-            } else if (J.best - row_max - (dj - di) * e_ins > sc.zdrop) {
-                done = true;
-            }
+            const int32_t dd = di > dj ? di - dj : dj - di, ge = di > dj ? e_del : e_ins;
+            const int32_t gap = sc.small_ext ? __mul24(dd, ge) : dd * ge;
+            if (J.best - row_max - gap > sc.zdrop) done = true;
         }
         if (!done) {
             // band tightening (sw.rs:464-480): first live column in [beg, end), last live column
@@ -238,6 +273,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             J.end = ne + 2 < J.qlen ? ne + 2 : J.qlen;
             J.i = i + 1;
             J.tcur = tnext;
+            J.h1base = (int32_t)((uint32_t)J.h1base - (uint32_t)e_del);
             if (J.i >= J.tlen) done = true;
         }
         if (done) {
@@ -257,7 +293,9 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
 }
 
 // ---- short queries: one job per lane --------------------------------------------------------
-template <int QCAP>
+// Column state and query codes in LDS laid out [column][lane], so the 64 lanes of a wavefront
+// touch consecutive words; narrow storage as in the group kernel.
+template <int QCAP, bool NARROW>
 __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restrict__ arena,
                                                         const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
@@ -265,9 +303,13 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                                                         uint32_t count, uint32_t *__restrict__ head,
                                                         ExtScoring sc,
                                                         bwg_ext_result *__restrict__ out) {
-    // [column][lane] layouts: lanes of a wavefront touch consecutive words
-    __shared__ int2 eh[(QCAP + 1) * 64];
-    __shared__ int8_t planes[5 * QCAP * 64];  // planes[(c*QCAP + j)*64 + lane] = mat[c][query[j]]
+    using CS = ColState<NARROW>;
+    using W = typename CS::word;
+    __shared__ W eh[(QCAP + 1) * 64];
+    __shared__ uint8_t codes[QCAP * 64];
+    __shared__ __align__(8) int8_t tbl[64];
+    fill_subst_table(tbl, sc);
+    __syncthreads();
     const int lane = threadIdx.x;
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
@@ -285,18 +327,14 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
             const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
             const int32_t w = clamp_band(sc.w, qlen, sc);
             for (int j = 0; j <= qlen; ++j)
-                eh[j * 64 + lane] = make_int2(j == 0 ? h0 : max(h0 - oe_ins - (j - 1) * e_ins, 0), 0);
-            for (int j = 0; j < qlen; ++j) {
-                const int qc = q[j];
-#pragma unroll
-                for (int c = 0; c < 5; ++c) planes[(c * QCAP + j) * 64 + lane] = sc.mat[c * 5 + qc];
-            }
+                eh[j * 64 + lane] = CS::pack(j == 0 ? h0 : max(h0 - oe_ins - (j - 1) * e_ins, 0), 0);
+            for (int j = 0; j < qlen; ++j) codes[j * 64 + lane] = q[j];
             int32_t best = h0, best_i = -1, best_j = -1, gbest_i = -1, gscore = -1, max_off = 0;
             int32_t beg = 0, end = qlen;
-            int32_t tnext = tlen > 0 ? t[0] : 0;
+            LaneTargetStream ts;
+            if (tlen > 0) ts.start(t, tlen);
             for (int32_t i = 0; i < tlen; ++i) {
-                const int8_t *srow = planes + tnext * QCAP * 64 + lane;
-                if (i + 1 < tlen) tnext = t[i + 1];
+                const uint2 trow = *(const uint2 *)(tbl + 8 * ts.at(i, t, tlen));
                 int32_t f = 0, key = 0;
                 if (beg < i - w) beg = i - w;
                 if (end > i + w + 1) end = i + w + 1;
@@ -304,17 +342,17 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                 int32_t h1 = beg == 0 ? max(h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
                 int32_t j = beg;
                 for (; j < end; ++j) {
-                    const int2 v = eh[j * 64 + lane];
-                    const int32_t s = srow[j * 64];
-                    const int32_t m = v.x != 0 ? v.x + s : 0;
-                    const int32_t h = max(max(m, v.y), f);
-                    const int32_t e = max(max(v.y - e_del, m - oe_del), 0);
-                    eh[j * 64 + lane] = make_int2(h1, e);
+                    const W v = eh[j * 64 + lane];
+                    const int32_t hd = CS::h_of(v), ev = CS::e_of(v);
+                    const int32_t m = hd != 0 ? hd + subst(trow, codes[j * 64 + lane]) : 0;
+                    const int32_t h = max(max(m, ev), f);
+                    const int32_t e = max(max(ev - e_del, m - oe_del), 0);
+                    eh[j * 64 + lane] = CS::pack(h1, e);
                     h1 = h;
                     key = max(key, (h << kKeyShift) | j);
                     f = max(f - e_ins, max(m - oe_ins, 0));
                 }
-                eh[end * 64 + lane] = make_int2(h1, 0);
+                eh[end * 64 + lane] = CS::pack(h1, 0);
                 const int32_t row_max = key >> kKeyShift, mj = key & kKeyMask;
                 if (j == qlen && gscore <= h1) {
                     gbest_i = i;
@@ -335,18 +373,10 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                     }
                 }
                 int32_t a = beg;
-                while (a < end) {
-                    const int2 x = eh[a * 64 + lane];
-                    if ((x.x | x.y) != 0) break;
-                    ++a;
-                }
+                while (a < end && !CS::live(eh[a * 64 + lane])) ++a;
                 beg = a;
                 int32_t z = end;
-                while (z >= beg) {
-                    const int2 x = eh[z * 64 + lane];
-                    if ((x.x | x.y) != 0) break;
-                    --z;
-                }
+                while (z >= beg && !CS::live(eh[z * 64 + lane])) --z;
                 end = z + 2 < qlen ? z + 2 : qlen;
             }
             bwg_ext_result r;
@@ -387,13 +417,13 @@ int group_occupancy(int qcap) {
     return v;
 }
 
-template <int QCAP>
+template <int QCAP, bool NARROW>
 int thread_occupancy() {
     static std::atomic<int> cache{0};
     int v = cache.load(std::memory_order_relaxed);
     if (v > 0) return v;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_thread_kernel<QCAP>, 64, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_thread_kernel<QCAP, NARROW>, 64, 0) != hipSuccess)
         blocks = 1;
     v = blocks > 0 ? blocks : 1;
     cache.store(v, std::memory_order_relaxed);
@@ -424,26 +454,26 @@ hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const u
     }
 }
 
-int ext_thread_blocks_per_cu(int qcap) {
+int ext_thread_blocks_per_cu(int qcap, bool narrow) {
     switch (qcap) {
-    case 16: return thread_occupancy<16>();
-    case 32: return thread_occupancy<32>();
+    case 16: return narrow ? thread_occupancy<16, true>() : thread_occupancy<16, false>();
+    case 32: return narrow ? thread_occupancy<32, true>() : thread_occupancy<32, false>();
     default: return 1;
     }
 }
 
-hipError_t launch_ext_thread(hipStream_t st, int qcap, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
-                             const uint32_t *order, uint32_t count, uint32_t *head,
+hipError_t launch_ext_thread(hipStream_t st, int qcap, bool narrow, const uint8_t *arena, const uint8_t *ref,
+                             const ExtJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
                              const ExtScoring &sc, bwg_ext_result *out, int grid) {
-    switch (qcap) {
-    case 16:
-        hipLaunchKernelGGL(ext_thread_kernel<16>, dim3(grid), dim3(64), 0, st, arena, ref, jobs, order, count, head, sc, out);
-        break;
-    case 32:
-        hipLaunchKernelGGL(ext_thread_kernel<32>, dim3(grid), dim3(64), 0, st, arena, ref, jobs, order, count, head, sc, out);
-        break;
-    default: return hipErrorInvalidValue;
-    }
+#define BWG_T(Q, N)                                                                                        \
+    hipLaunchKernelGGL((ext_thread_kernel<Q, N>), dim3(grid), dim3(64), 0, st, arena, ref, jobs, order, count, \
+                       head, sc, out)
+    if (qcap == 16 && narrow) BWG_T(16, true);
+    else if (qcap == 16) BWG_T(16, false);
+    else if (qcap == 32 && narrow) BWG_T(32, true);
+    else if (qcap == 32) BWG_T(32, false);
+    else return hipErrorInvalidValue;
+#undef BWG_T
     return hipGetLastError();
 }
 

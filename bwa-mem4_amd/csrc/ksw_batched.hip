@@ -36,6 +36,14 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
                                                         uint32_t count, uint32_t *__restrict__ head,
                                                         KswScoring sc, bwg_ksw_result *__restrict__ out,
                                                         int32_t *__restrict__ rowmax) {
+    // column q of the matrix for target codes 0..4 at tblT[8*q]: one 4-byte read gives the
+    // profile word of a query base, the fifth byte its score against N
+    __shared__ __align__(8) uint8_t tblT[40];
+    if (threadIdx.x < 40) {
+        const int qc = threadIdx.x >> 3, tc = threadIdx.x & 7;
+        tblT[threadIdx.x] = tc < 5 ? (uint8_t)sc.mat[tc * 5 + qc] : (uint8_t)0;
+    }
+    __syncthreads();
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
@@ -93,9 +101,8 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
                         int32_t n4 = 0;
                         if (j < qlen) {
                             const int qc = REV ? q[qlen - 1 - j] : q[j];
-                            p = (uint32_t)(uint8_t)sc.mat[qc] | ((uint32_t)(uint8_t)sc.mat[5 + qc] << 8) |
-                                ((uint32_t)(uint8_t)sc.mat[10 + qc] << 16) | ((uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
-                            n4 = (uint8_t)sc.mat[20 + qc];
+                            p = *(const uint32_t *)(tblT + 8 * qc);
+                            n4 = tblT[8 * qc + 4];
                         }
                         prof[k] = p;  // padding columns score 0 against every target base
                         if ((k & 3) == 0) p4w[k >> 2] = 0;

@@ -96,9 +96,10 @@ int bwg_ksw_align2_batch(bwg_ctx *ctx, const bwg_ksw_params *p, size_t n, const 
  *
  * Autotuning (BWG_OPT_AUTOTUNE, default 1): the first batch with at least 4096 jobs of a length
  * class times the class's candidate mappings on a sample of those jobs (one lane per job or 16
- * lanes per job; the group kernel's columns per lane) and the context keeps the fastest for that
- * scoring.  Setting the option clears the table.  With autotuning off, the mapping follows
- * BWG_OPT_EXT_THREAD_MAX_Q and the K_STEP options, which force each candidate for tests. */
+ * lanes per job; the group kernel's columns per lane; 16- or 32-bit stored DP state) and the
+ * context keeps the fastest for that scoring.  Setting the option clears the table.  With
+ * autotuning off, the mapping follows BWG_OPT_EXT_THREAD_MAX_Q, the K_STEP options and
+ * BWG_OPT_EXT_SCORE_WIDTH, which force each candidate for tests. */
 enum { BWG_FORM_PLAIN = 0, BWG_FORM_BATCHED = 1 };
 enum {
     BWG_OPT_FORM = 1,
@@ -108,6 +109,7 @@ enum {
     BWG_OPT_AUTOTUNE = 5,         /* batched: time candidate mappings per length class (default 1) */
     BWG_OPT_EXT_K_STEP = 6,       /* untuned extension group bins use one K above the minimal (default 0) */
     BWG_OPT_KSW_K_STEP = 7,       /* untuned rescue bins use one K above the minimal (default 0) */
+    BWG_OPT_EXT_SCORE_WIDTH = 8,  /* extension DP state bits: 16 where exact (default), or 32 always */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */

@@ -29,16 +29,21 @@ def test_forced_extension_mappings_agree(gpu_ctx):
     gpu_ctx.set_form(bwg.FORM_BATCHED)
     gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
     try:
-        # thread kernel up to q=32 / up to q=16 / never; group kernels at the minimal K or one above
-        for tmax, kstep in [(32, 0), (16, 0), (0, 0), (32, 1), (0, 1)]:
-            _say("forced mapping thread_max_q=%d k_step=%d" % (tmax, kstep))
+        # thread kernel up to q=32 / up to q=16 / never; group kernels at the minimal K or one
+        # above; 16- and 32-bit column state
+        for tmax, kstep, width in [(32, 0, 16), (16, 0, 16), (0, 0, 16), (32, 1, 16), (0, 1, 16),
+                                   (32, 0, 32), (0, 1, 32)]:
+            _say("forced mapping thread_max_q=%d k_step=%d width=%d" % (tmax, kstep, width))
             gpu_ctx.set_option(bwg.OPT_EXT_THREAD_MAX_Q, tmax)
             gpu_ctx.set_option(bwg.OPT_EXT_K_STEP, kstep)
+            gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, width)
+            gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
             got = gpu_ctx.extend_arrays(batch, params)
-            assert _first_bad(got, want) is None, (tmax, kstep, _first_bad(got, want))
+            assert _first_bad(got, want) is None, (tmax, kstep, width, _first_bad(got, want))
     finally:
         gpu_ctx.set_option(bwg.OPT_EXT_THREAD_MAX_Q, 32)
         gpu_ctx.set_option(bwg.OPT_EXT_K_STEP, 0)
+        gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, 16)
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
