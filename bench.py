@@ -49,14 +49,23 @@ def time_steps(ctx, staged, steps, warmup):
 
 
 def cpu_baseline(ext_batch, mat, n_threads):
-    """The oracle on the host cores over the whole extension batch (bounded: a few seconds on 16
-    threads).  Also returns the workload's evaluated-cell count, the GCUPS denominator."""
+    """The oracle on the host cores over the whole extension batch, with the same algorithm the
+    batched GPU form runs: the ungapped-diagonal fast path, then ksw_extend2 for the declined jobs
+    (one pthread per core, static partition).  Returns the results, the time, the workload's
+    evaluated-cell count (the GCUPS denominator) and the cells of the declined jobs (the cells the
+    DP kernels evaluate)."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
     import oracle_binding as ob
     t0 = time.perf_counter()
-    out, cells = ob.extend_batch(ext_batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=n_threads, want_cells=True)
+    out, hit = ob.ungapped_batch(ext_batch, mat, 6, 1, 6, 1, 100)
+    idx = np.nonzero(~hit)[0]
+    rest = bwg.JobBatch(ext_batch.arena, ext_batch.q_off[idx], ext_batch.q_len[idx], ext_batch.t_off[idx],
+                        ext_batch.t_len[idx], ext_batch.h0[idx])
+    out[idx] = ob.extend_batch(rest, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=n_threads)
     dt = time.perf_counter() - t0
-    return out, cells, dt
+    full, job_cells = ob.extend_batch_job_cells(ext_batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=n_threads)
+    assert np.array_equal(full, out)
+    return out, int(job_cells.sum()), dt, int(job_cells[idx].sum()), int(hit.sum())
 
 
 def traffic_from_profile(jobs):
@@ -133,9 +142,22 @@ def main():
             st.run()  # first run builds the form's plan and buffers
             per, kern, main, nl = time_steps(ctx, st, args.steps if form == bwg.FORM_BATCHED else max(1, args.steps // 3), args.warmup)
             outs["ext_" + fname] = st.results()
+            hits = st.ungapped_hits()
             st.close()
-            res["ext_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
-            log("extension %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
+            res["ext_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl, ungapped_hits=hits)
+            log("extension %s: %.3f ms/step (kernels %.3f ms), %d jobs resolved by the ungapped fast path"
+                % (fname, per * 1e3, kern * 1e3, hits))
+            if form == bwg.FORM_BATCHED:
+                # the same batch with every job through the banded DP, for comparison
+                ctx.set_option(bwg.OPT_UNGAPPED, 0)
+                st = ctx.stage(ext, ext_p)
+                st.run()
+                per0, kern0, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+                assert np.array_equal(st.results(), outs["ext_batched"]), "fast path changes results"
+                st.close()
+                ctx.set_option(bwg.OPT_UNGAPPED, 1)
+                res["ext_batched_dp_only"] = dict(step_s=per0, kernel_s=kern0)
+                log("extension batched, fast path off: %.3f ms/step (kernels %.3f ms)" % (per0 * 1e3, kern0 * 1e3))
         except KeyError:
             pass
         except bwg.BwgError as e:
@@ -209,9 +231,10 @@ def main():
     cpu = None
     cells = None
     small_cells = None
+    dp_cells = cpu_hits = None
     if not args.no_cpu:
         log("cpu oracle over the extension batch on %d threads" % args.cpu_threads)
-        cpu_out, cells, cpu_dt = cpu_baseline(ext, mat, args.cpu_threads)
+        cpu_out, cells, cpu_dt, dp_cells, cpu_hits = cpu_baseline(ext, mat, args.cpu_threads)
         if not args.no_check and head in outs:
             ok = np.array_equal(cpu_out, outs[head])
             log("bit-exact vs oracle: %s" % ok)
@@ -221,10 +244,11 @@ def main():
                 sys.exit(1)
         cpu = {"value": args.jobs / cpu_dt, "unit": "jobs/s", "cores": args.cpu_threads,
                "kind": "port", "gcups": cells / cpu_dt / 1e9,
-               "sample": "all %d long-tail extension jobs, scalar C oracle, static partition" % args.jobs}
+               "sample": "all %d long-tail extension jobs: the ungapped-diagonal fast path, then the "
+                         "scalar C oracle's ksw_extend2 for the declined jobs, static partition" % args.jobs}
         small_key = "small_batched" if "small_batched" in outs else "small_plain"
         if small_key in outs:
-            s_out, small_cells, _ = cpu_baseline(small, mat, args.cpu_threads)
+            s_out, small_cells, _, _, _ = cpu_baseline(small, mat, args.cpu_threads)
             ok = np.array_equal(s_out, outs[small_key])
             log("small workload bit-exact vs oracle: %s" % ok)
             if not ok:
@@ -245,7 +269,12 @@ def main():
         line["gcups"] = cells / h["step_s"] / 1e9
         line["gcups_kernel"] = cells / h["kernel_s"] / 1e9
         line["cells"] = cells
-        ops = cells * K_EXT / h["kernel_s"] / 1e12
+        # the DP kernels evaluate the cells of the jobs the ungapped fast path declines
+        if h.get("ungapped_hits"):
+            assert h["ungapped_hits"] == cpu_hits, "device and oracle fast paths disagree"
+            line["ungapped_hits"] = h["ungapped_hits"]
+            line["dp_cells"] = dp_cells
+        ops = (dp_cells if h.get("ungapped_hits") else cells) * K_EXT / h["kernel_s"] / 1e12
         # algorithmic HBM bytes: query and target bytes, descriptor, order entry and result per job
         alg_bytes = float(np.sum(ext.q_len.astype(np.int64) + ext.t_len)) + args.jobs * (32 + 4 + 24)
         line["roofline"] = {"bound": "valu", "achieved": ops, "peak": VALU_PEAK_TOPS,
@@ -254,6 +283,10 @@ def main():
                             "hbm": {"achieved": alg_bytes / h["kernel_s"] / 1e9, "peak": HBM_PEAK_GBPS,
                                     "unit": "GB/s", "frac": alg_bytes / h["kernel_s"] / 1e9 / HBM_PEAK_GBPS,
                                     "alg_bytes": alg_bytes}}
+    if "ext_batched_dp_only" in res:
+        line["batched_dp_only_jobs_per_s"] = args.jobs / res["ext_batched_dp_only"]["step_s"]
+        if cells:
+            line["batched_dp_only_roofline_frac"] = cells * K_EXT / res["ext_batched_dp_only"]["kernel_s"] / 1e12 / VALU_PEAK_TOPS
     if "ext_plain" in res:
         line["plain_jobs_per_s"] = args.jobs / res["ext_plain"]["step_s"]
         line["speedup_vs_plain"] = jobs_s / line["plain_jobs_per_s"]

@@ -32,7 +32,7 @@ KSW_DTYPE = np.dtype([(n, "<i4") for n in KswAlignResult._fields])
 BWG_OK, BWG_EIO, BWG_E2BIG, BWG_EAGAIN, BWG_ENOMEM, BWG_ENODEV, BWG_EINVAL = 0, -5, -7, -11, -12, -19, -22
 FORM_PLAIN, FORM_BATCHED = 0, 1
 OPT_FORM, OPT_CONCURRENT_BINS, OPT_GRAPHS, OPT_EXT_THREAD_MAX_Q = 1, 2, 3, 4
-OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP, OPT_EXT_SCORE_WIDTH = 5, 6, 7, 8
+OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP, OPT_EXT_SCORE_WIDTH, OPT_UNGAPPED = 5, 6, 7, 8, 9
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 
 # Every symbol include/bwg.h declares.
@@ -42,6 +42,7 @@ EXPORTED_SYMBOLS = (
     "bwg_batch_stage_ksw", "bwg_batch_run", "bwg_batch_results", "bwg_batch_destroy",
     "bwg_server_create", "bwg_server_set_reference", "bwg_server_submit", "bwg_server_poll",
     "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy", "bwg_hbm_copy", "bwg_tune_report",
+    "bwg_batch_ungapped_hits",
 )
 
 
@@ -103,6 +104,8 @@ def _lib():
         L.bwg_batch_destroy.argtypes = [_P, _P]
         L.bwg_hbm_copy.restype = C.c_int
         L.bwg_hbm_copy.argtypes = [_P, C.c_size_t, C.c_int, _P]
+        L.bwg_batch_ungapped_hits.restype = C.c_int
+        L.bwg_batch_ungapped_hits.argtypes = [_P, _P, _P]
         L.bwg_tune_report.restype = C.c_int
         L.bwg_tune_report.argtypes = [_P, C.c_char_p, C.c_size_t]
         _L = L
@@ -272,6 +275,12 @@ class StagedBatch:
         out = np.zeros(self.n, dtype=self.dtype)
         _check(_lib().bwg_batch_results(self.ctx.h, self.h, out.ctypes.data), "bwg_batch_results")
         return out
+
+    def ungapped_hits(self):
+        """Jobs of the last batched run resolved by the ungapped-diagonal fast path."""
+        v = C.c_uint64(0)
+        _check(_lib().bwg_batch_ungapped_hits(self.ctx.h, self.h, C.byref(v)), "bwg_batch_ungapped_hits")
+        return int(v.value)
 
     def close(self):
         if self.h:

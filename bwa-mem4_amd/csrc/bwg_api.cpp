@@ -132,6 +132,9 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
     case BWG_OPT_KSW_K_STEP:
         c->tuning.ksw_k_step = v != 0;
         return BWG_OK;
+    case BWG_OPT_UNGAPPED:
+        c->tuning.ungapped = v != 0;
+        return BWG_OK;
     case BWG_OPT_EXT_SCORE_WIDTH:
         if (v != 16 && v != 32) return BWG_EINVAL;
         c->tuning.ext_force_wide = v == 32;
@@ -163,6 +166,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_EXT_K_STEP: return c->tuning.ext_k_step;
     case BWG_OPT_KSW_K_STEP: return c->tuning.ksw_k_step;
     case BWG_OPT_EXT_SCORE_WIDTH: return c->tuning.ext_force_wide ? 32 : 16;
+    case BWG_OPT_UNGAPPED: return c->tuning.ungapped;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;
@@ -238,6 +242,13 @@ int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uin
     rc = stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, c->ss.main, out);
     if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
     return rc;
+}
+
+int bwg_batch_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
+    if (!c || !b || !hits) return BWG_EINVAL;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return batched_ungapped_hits(c, b, hits);
 }
 
 int bwg_batch_run(bwg_ctx *c, bwg_batch *b) {

@@ -48,12 +48,23 @@ struct Plan {
     void *d_plain_jobs = nullptr;
     void *d_plain_scratch = nullptr;
     uint32_t n_fast = 0;
+    // extension ungapped pre-pass: compacted per-bin queues; d_heads holds the queue heads, then the
+    // compacted counts, then the bins' slot offsets and skip flags
+    bool ungapped = false;
+    int32_t ungapped_xmax = -1;
+    uint32_t *d_corder = nullptr;
+    int n_bins = 0;
+    std::vector<uint32_t> meta;
     // host copies kept alive until the uploads complete
     std::vector<uint32_t> order;
     std::vector<KswJob> ksw_jobs;
     std::vector<ExtJob> ext_plain_jobs;
     std::vector<KswJob> ksw_plain_jobs;
 };
+
+// First word of the slot offsets in the extension plan's metadata (after the queue heads and the
+// spaced-out compacted-queue counters).
+inline size_t seg_word(const Plan *p) { return (size_t)p->n_bins * (1 + kCountStride); }
 
 // Autotuning: a class is tuned once the context sees a batch with at least kTuneMinJobs of its
 // jobs; each candidate runs kTuneReps times on up to kTuneSample of them and keeps its best time.
@@ -240,11 +251,11 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
             const int grid = ext_grid(c, cd.kind, cd.param, narrow, qcap, S);
             if (e == hipSuccess) {
                 if (cd.kind == kExtGroup)
-                    e = launch_ext_group(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S, d,
-                                         b->esc, qcap, (bwg_ext_result *)b->d_out, grid);
+                    e = launch_ext_group(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S,
+                                         nullptr, d, b->esc, qcap, (bwg_ext_result *)b->d_out, grid);
                 else
                     e = launch_ext_thread(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1,
-                                          S, d, b->esc, (bwg_ext_result *)b->d_out, grid);
+                                          S, nullptr, d, b->esc, (bwg_ext_result *)b->d_out, grid);
             }
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -352,12 +363,45 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             if (!p->d_plain_jobs || !p->d_plain_scratch) return BWG_ENOMEM;
         }
     }
-    p->items = bins;
     p->n_heads = std::max<int>(1, (int)bins.size());
+    // The reference's ungapped-diagonal fast path (crates/bwa-neon/src/batched.rs:716-798) applies
+    // to uniform DNA matrices: run it first over the group and thread bins, which then drain the
+    // compacted queues of the jobs it declines.
+    const int32_t a = sc.mat[0], bmis = -(int32_t)sc.mat[1];
+    const int32_t xmax = ungapped_xmax(sc);
+    p->ungapped = tu.ungapped && is_uniform_dna(sc) && a > 0 && bmis > 0 && xmax >= 0 && !bins.empty();
+    if (p->ungapped) {
+        p->ungapped_xmax = xmax;
+        p->n_bins = (int)bins.size();
+        // [heads nb][counters, kCountStride apart][slot offsets nb][skip flags nb]
+        p->meta.assign((size_t)p->n_bins * (kCountStride + 3), 0u);
+        bool any = false;
+        for (int r = 0; r < p->n_bins; ++r) {
+            p->meta[seg_word(p) + r] = bins[r].offset;
+            const bool dp = bins[r].kind == kExtGroup || bins[r].kind == kExtThread;
+            p->meta[seg_word(p) + p->n_bins + r] = dp ? 0u : 1u;
+            any |= dp;
+        }
+        p->ungapped = any && p->n_bins <= 255;
+    }
+    if (p->ungapped) {
+        for (auto &it : bins) it.stage = 1;
+        Item u;
+        u.kind = kExtUngapped;
+        u.stage = 0;
+        u.count = (uint32_t)n;
+        u.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cus * 8, ((int64_t)n + 15) / 16));
+        bins.insert(bins.begin(), u);
+        p->d_corder = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
+        if (!p->d_corder) return BWG_ENOMEM;
+    }
+    p->items = bins;
     p->d_order = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
-    p->d_heads = (uint32_t *)c->pool.alloc((size_t)p->n_heads * 4);
+    p->d_heads = (uint32_t *)c->pool.alloc(std::max<size_t>((size_t)p->n_heads, p->meta.size()) * 4);
     if (!p->d_order || !p->d_heads) return BWG_ENOMEM;
     hipError_t e = hipMemcpyAsync(p->d_order, p->order.data(), n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && p->ungapped)
+        e = hipMemcpyAsync(p->d_heads, p->meta.data(), p->meta.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !p->ext_plain_jobs.empty())
         e = hipMemcpyAsync(p->d_plain_jobs, p->ext_plain_jobs.data(), p->ext_plain_jobs.size() * sizeof(ExtJob),
                            hipMemcpyHostToDevice, st);
@@ -552,11 +596,22 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
 hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
     switch (it.kind) {
     case kExtGroup:
-        return launch_ext_group(st, it.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order + it.offset,
-                                it.count, p->d_heads + it.head, b->esc, it.qcap, (bwg_ext_result *)b->d_out, it.grid);
-    case kExtThread:
-        return launch_ext_thread(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order + it.offset,
-                                 it.count, p->d_heads + it.head, b->esc, (bwg_ext_result *)b->d_out, it.grid);
+    case kExtThread: {
+        // after the ungapped pre-pass the bin drains its compacted queue
+        const uint32_t *order = p->ungapped ? p->d_corder + it.offset : p->d_order + it.offset;
+        const uint32_t *dcount = p->ungapped ? p->d_heads + p->n_bins + kCountStride * it.head : nullptr;
+        if (it.kind == kExtGroup)
+            return launch_ext_group(st, it.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order, it.count,
+                                    dcount, p->d_heads + it.head, b->esc, it.qcap, (bwg_ext_result *)b->d_out,
+                                    it.grid);
+        return launch_ext_thread(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order,
+                                 it.count, dcount, p->d_heads + it.head, b->esc, (bwg_ext_result *)b->d_out, it.grid);
+    }
+    case kExtUngapped:
+        return launch_ext_ungapped(st, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order, it.count,
+                                   p->d_heads + seg_word(p), p->d_heads + seg_word(p) + p->n_bins, p->n_bins,
+                                   p->d_corder, p->d_heads + p->n_bins, b->esc, p->ungapped_xmax,
+                                   (bwg_ext_result *)b->d_out, it.grid);
     case kExtPlain:
         return launch_ext_plain_scatter(st, b->d_arena, b->d_ref, (const ExtJob *)p->d_plain_jobs,
                                         p->d_order + it.offset, it.count, b->esc, (int2 *)p->d_plain_scratch,
@@ -580,7 +635,8 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
 // The launch sequence of a planned batch on a stream set: queue reset, then each stage's items,
 // concurrently over the streams when enabled.
 hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launches) {
-    hipError_t e = launch_zero_u32(ss.main, p->d_heads, (uint32_t)p->n_heads);
+    // queue heads (and, with the ungapped pre-pass, the compacted queue lengths) start at zero
+    hipError_t e = launch_zero_u32(ss.main, p->d_heads, (uint32_t)(p->ungapped ? seg_word(p) : p->n_heads));
     int64_t nl = 1;
     size_t i = 0;
     while (i < p->items.size() && e == hipSuccess) {
@@ -649,6 +705,26 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     return bwg_map_hip(e);
 }
 
+int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
+    *hits = 0;
+    if (!b->plan || b->kind != 0) return BWG_OK;
+    Plan *p = (Plan *)b->plan;
+    if (!p->ungapped) return BWG_OK;
+    std::vector<uint32_t> cnt((size_t)kCountStride * p->n_bins);
+    hipError_t e = hipMemcpyAsync(cnt.data(), p->d_heads + p->n_bins, cnt.size() * 4, hipMemcpyDeviceToHost,
+                                  c->ss.main);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    if (e != hipSuccess) return bwg_map_hip(e);
+    uint64_t eligible = 0, declined = 0;
+    for (const Item &it : p->items)
+        if (it.kind == kExtGroup || it.kind == kExtThread) {
+            eligible += it.count;
+            declined += cnt[(size_t)kCountStride * it.head];
+        }
+    *hits = eligible - declined;
+    return BWG_OK;
+}
+
 void release_batched(bwg_ctx *c, bwg_batch *b) {
     if (b->graph) {
         (void)hipGraphExecDestroy(b->graph);
@@ -662,6 +738,7 @@ void release_batched(bwg_ctx *c, bwg_batch *b) {
     c->pool.free(p->d_rowmax);
     c->pool.free(p->d_plain_jobs);
     c->pool.free(p->d_plain_scratch);
+    c->pool.free(p->d_corder);
     delete p;
     b->plan = nullptr;
 }

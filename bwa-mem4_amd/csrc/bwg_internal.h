@@ -82,6 +82,25 @@ inline ExtScoring ext_scoring(const bwg_ext_params &p) {
     return sc;
 }
 
+// The ungapped fast path's preconditions (crates/bwa-neon/src/batched.rs:493-521, 756-764): a
+// matrix of the uniform DNA shape, and the largest mismatch count for which the ungapped diagonal
+// strictly beats the cheapest gap, X * (a + b) < o_min + e_min (-1: never).
+inline bool is_uniform_dna(const ExtScoring &sc) {
+    const int8_t a = sc.mat[0], mm = sc.mat[1], npen = sc.mat[4];
+    for (int i = 0; i < 5; ++i)
+        for (int j = 0; j < 5; ++j) {
+            const int8_t want = (i == 4 || j == 4) ? npen : (i == j ? a : mm);
+            if (sc.mat[i * 5 + j] != want) return false;
+        }
+    return true;
+}
+inline int32_t ungapped_xmax(const ExtScoring &sc) {
+    const int32_t a = sc.mat[0], b = -(int32_t)sc.mat[1];
+    const int32_t o_min = sc.o_del < sc.o_ins ? sc.o_del : sc.o_ins;
+    const int32_t e_min = sc.e_del < sc.e_ins ? sc.e_del : sc.e_ins;
+    return a + b > 0 ? (o_min + e_min - 1) / (a + b) : -1;
+}
+
 inline KswScoring ksw_scoring(const bwg_ksw_params &p) {
     KswScoring sc{};
     for (int k = 0; k < 25; ++k) sc.mat[k] = p.mat[k];

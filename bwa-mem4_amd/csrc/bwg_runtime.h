@@ -56,6 +56,7 @@ struct Tuning {
     bool ext_k_step = false;       // untuned extension bins: one K above the minimal one
     bool ksw_k_step = false;       // untuned rescue bins: one K above the minimal one
     bool ext_force_wide = false;   // extension: 32-bit column state even where 16 bits are exact
+    bool ungapped = true;          // extension: the reference's ungapped-diagonal fast path first
     bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
     bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph
 };
@@ -128,6 +129,8 @@ size_t batch_device_bytes(int kind, size_t n, size_t arena_len, const uint32_t *
 // batched form (bwg_batched.cpp)
 int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches);
 void release_batched(bwg_ctx *c, bwg_batch *b);
+// Jobs of the last batched run that the ungapped fast path resolved (0 when it did not run).
+int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits);
 
 }  // namespace bwg
 

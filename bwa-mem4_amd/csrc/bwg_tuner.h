@@ -14,7 +14,7 @@
 namespace bwg {
 
 // Kernel mappings a bin can be run with.
-enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain };
+enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain, kExtUngapped };
 
 struct TuneCandidate {
     int kind = 0;

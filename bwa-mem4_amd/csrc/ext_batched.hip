@@ -91,10 +91,11 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                                                         const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ order,
-                                                        uint32_t count, uint32_t *__restrict__ head,
-                                                        ExtScoring sc, int qcap,
+                                                        uint32_t count_static, const uint32_t *__restrict__ dcount,
+                                                        uint32_t *__restrict__ head, ExtScoring sc, int qcap,
                                                         bwg_ext_result *__restrict__ out) {
     extern __shared__ int32_t lds[];
+    const uint32_t count = dcount ? *dcount : count_static;  // a compacted queue's length is on the device
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;             // lane within the group
     const int gid = threadIdx.x >> 4;    // group within the block
@@ -300,9 +301,10 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                                                         const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ order,
-                                                        uint32_t count, uint32_t *__restrict__ head,
-                                                        ExtScoring sc,
+                                                        uint32_t count_static, const uint32_t *__restrict__ dcount,
+                                                        uint32_t *__restrict__ head, ExtScoring sc,
                                                         bwg_ext_result *__restrict__ out) {
+    const uint32_t count = dcount ? *dcount : count_static;
     using CS = ColState<NARROW>;
     using W = typename CS::word;
     __shared__ W eh[(QCAP + 1) * 64];
@@ -391,13 +393,168 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
     }
 }
 
+// ---- ungapped-diagonal fast path -------------------------------------------------------------
+// The reference's batched extension first tries a closed-form walk along the main diagonal
+// (crates/bwa-neon/src/batched.rs:602-697, ungapped_hit) and runs the banded DP only for the jobs it
+// declines.  One job per 16-lane group; block b takes the contiguous order-array slots
+// [b*chunk, (b+1)*chunk) in rounds of 16.  Lane g reads columns g, g+16, ... of the query and target
+// (16 consecutive bytes per step, the first 128 columns loaded up front), the group counts
+// mismatches and N bases with ballots and keeps the mismatch columns (< 128) as a 128-bit mask.  An
+// accepted job's result is written directly; the declined ones are collected in slot order and, at
+// the end of the chunk, appended to their bins' compacted queues with one atomic per bin (the
+// counters sit kCountStride words apart, on separate cache lines).  Slot ranges
+// [seg_off[b], seg_off[b+1]) belong to bin b; bins with skip[b] set (the plain bin) are left alone.
+constexpr int kUngappedChunkMax = 4096;
+
+__global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__restrict__ arena,
+                                                           const uint8_t *__restrict__ ref,
+                                                           const ExtJob *__restrict__ jobs,
+                                                           const uint32_t *__restrict__ order, uint32_t n_slots,
+                                                           uint32_t chunk, const uint32_t *__restrict__ seg_off,
+                                                           const uint32_t *__restrict__ skip, int n_bins,
+                                                           uint32_t *__restrict__ corder,
+                                                           uint32_t *__restrict__ ccount, int32_t a, int32_t b,
+                                                           int32_t xmax, int32_t zdrop,
+                                                           bwg_ext_result *__restrict__ out) {
+    __shared__ uint32_t s_list[kUngappedChunkMax];
+    __shared__ uint8_t s_lbin[kUngappedChunkMax];
+    __shared__ uint32_t s_decl[16], s_idx[16], s_bin[16];
+    __shared__ uint32_t s_n;
+    const int lane = threadIdx.x & 63;
+    const int g = lane & 15;
+    const int gid = threadIdx.x >> 4;
+    const uint32_t lo = blockIdx.x * chunk;
+    const uint32_t hi = min(n_slots, lo + chunk);
+    if (threadIdx.x == 0) s_n = 0;
+    for (uint32_t base = lo; base < hi; base += 16u) {
+        int bin = 0;  // the round's first slot's bin (block-uniform), then this group's
+        while (bin + 1 < n_bins && base >= seg_off[bin + 1]) ++bin;
+        const uint32_t slot = base + gid;
+        bool declined = false;
+        uint32_t idx = 0;
+        if (slot < hi) {
+            while (bin + 1 < n_bins && slot >= seg_off[bin + 1]) ++bin;
+        }
+        if (slot < hi && !skip[bin]) {
+            idx = order[slot];
+            const ExtJob jb = jobs[idx];
+            const uint8_t *q = arena + jb.q_off;
+            const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
+            const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
+            bool bail = !(qlen > 0 && h0 > 0 && tlen >= qlen);
+            int32_t nmis = 0;
+            uint32_t bits[4] = {0u, 0u, 0u, 0u};
+            uint32_t qv[8], tv[8];
+#pragma unroll
+            for (int st = 0; st < 8; ++st) {  // independent loads of the first 128 columns
+                const int32_t c = 16 * st + g;
+                const bool in = !bail && c < qlen;
+                qv[st] = in ? q[c] : 0u;
+                tv[st] = in ? t[c] : 0u;
+            }
+#pragma unroll
+            for (int st = 0; st < 8; ++st) {
+                const int32_t c = 16 * st + g;
+                const bool in = !bail && c < qlen;
+                const uint64_t bn = __ballot(in && (qv[st] >= 4 || tv[st] >= 4));
+                const uint64_t bm = __ballot(in && qv[st] != tv[st]);
+                const uint32_t gn = (uint32_t)(bn >> (lane & ~15)) & 0xffffu;
+                const uint32_t gm = (uint32_t)(bm >> (lane & ~15)) & 0xffffu;
+                nmis += __popc(gm);
+                if (gn || nmis > xmax) bail = true;
+                else bits[st >> 1] |= gm << (16 * (st & 1));
+            }
+            // past column 128 only a mismatch-free walk can be accepted
+            for (int32_t c0 = 128; c0 < qlen && !bail; c0 += 16) {
+                const int32_t c = c0 + g;
+                const bool in = c < qlen;
+                const uint32_t qb = in ? q[c] : 0u, tb = in ? t[c] : 0u;
+                const uint64_t bx = __ballot(in && (qb >= 4 || tb >= 4 || qb != tb));
+                if ((uint32_t)(bx >> (lane & ~15)) & 0xffffu) bail = true;
+            }
+            bwg_ext_result r;
+            if (!bail && nmis == 0) {
+                // a mismatch-free walk only climbs: local and to-end alignments both take the query
+                r.score = h0 + qlen * a;
+                r.qle = r.tle = r.gtle = qlen;
+                r.gscore = r.score;
+                r.max_off = 0;
+            } else if (!bail) {
+                // the walk in runs between mismatches: a run of matches only climbs, so its last
+                // column is its best and z-drop can only fire at a mismatch; a dead diagonal declines.
+                // With mismatches the walk covers at most 128 columns.
+                bail = qlen > 128;
+                int32_t running = h0, best = h0, best_end = 0, prev = 0;
+                for (int w = 0; w < 4 && !bail; ++w) {
+                    uint32_t m = bits[w];
+                    while (m && !bail) {
+                        const int32_t p = 32 * w + __ffs(m) - 1;
+                        m &= m - 1;
+                        running += (p - prev) * a;
+                        if (p > prev && running > best) {
+                            best = running;
+                            best_end = p;
+                        }
+                        running -= b;
+                        if (running <= 0) bail = true;
+                        else if (zdrop > 0 && best - running > zdrop) bail = true;
+                        prev = p + 1;
+                    }
+                }
+                if (!bail) {
+                    running += (qlen - prev) * a;
+                    if (qlen > prev && running > best) {
+                        best = running;
+                        best_end = qlen;
+                    }
+                    r.score = best;
+                    r.qle = r.tle = best_end;
+                    r.gtle = qlen;
+                    r.gscore = running;
+                    r.max_off = 0;
+                }
+            }
+            if (!bail && g == 0) out[idx] = r;
+            declined = bail;
+        }
+        if (g == 0) {
+            s_decl[gid] = declined ? 1u : 0u;
+            s_idx[gid] = idx;
+            s_bin[gid] = (uint32_t)bin;
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {  // append the round's declined jobs in slot order
+            uint32_t n = s_n;
+            for (int k = 0; k < 16; ++k)
+                if (s_decl[k]) {
+                    s_list[n] = s_idx[k];
+                    s_lbin[n] = (uint8_t)s_bin[k];
+                    ++n;
+                }
+            s_n = n;
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {  // one reservation per bin run of the chunk, then the chunk's entries
+        const uint32_t n = s_n;
+        for (uint32_t i0 = 0; i0 < n;) {
+            uint32_t i1 = i0;
+            const uint32_t bn = s_lbin[i0];
+            while (i1 < n && s_lbin[i1] == bn) ++i1;
+            const uint32_t pos = seg_off[bn] + atomicAdd(&ccount[kCountStride * bn], i1 - i0);
+            for (uint32_t k = i0; k < i1; ++k) corder[pos + (k - i0)] = s_list[k];
+            i0 = i1;
+        }
+    }
+}
+
 template <int K>
 hipError_t launch_group(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs, const uint32_t *order,
-                        uint32_t count, uint32_t *head, const ExtScoring &sc, int qcap,
+                        uint32_t count, const uint32_t *dcount, uint32_t *head, const ExtScoring &sc, int qcap,
                         bwg_ext_result *out, int grid) {
     const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
     hipLaunchKernelGGL(ext_group_kernel<K>, dim3(grid), dim3(256), lds, st, arena, ref, jobs, order, count,
-                       head, sc, qcap, out);
+                       dcount, head, sc, qcap, out);
     return hipGetLastError();
 }
 
@@ -444,14 +601,31 @@ int ext_group_blocks_per_cu(int K, int qcap) {
 }
 
 hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
-                            const uint32_t *order, uint32_t count, uint32_t *head,
+                            const uint32_t *order, uint32_t count, const uint32_t *dcount, uint32_t *head,
                             const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid) {
     switch (K) {
-#define BWG_K(k) case k: return launch_group<k>(st, arena, ref, jobs, order, count, head, sc, qcap, out, grid);
+#define BWG_K(k) case k: return launch_group<k>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
+                               const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
+                               const uint32_t *skip, int n_bins, uint32_t *corder, uint32_t *ccount,
+                               const ExtScoring &sc, int32_t xmax, bwg_ext_result *out, int min_blocks) {
+    if (n_slots == 0) return hipSuccess;
+    if (n_bins > 255) return hipErrorInvalidValue;
+    // contiguous chunks of at most kUngappedChunkMax slots, a multiple of 16, at least min_blocks
+    uint32_t blocks = (uint32_t)std::max(1, min_blocks);
+    uint32_t chunk = (n_slots + blocks - 1) / blocks;
+    chunk = std::min<uint32_t>((chunk + 15) / 16 * 16, kUngappedChunkMax);
+    blocks = (n_slots + chunk - 1) / chunk;
+    hipLaunchKernelGGL(ext_ungapped_kernel, dim3(blocks), dim3(256), 0, st, arena, ref, jobs, order, n_slots, chunk,
+                       seg_off, skip, n_bins, corder, ccount, (int32_t)sc.mat[0], -(int32_t)sc.mat[1], xmax,
+                       sc.zdrop, out);
+    return hipGetLastError();
 }
 
 int ext_thread_blocks_per_cu(int qcap, bool narrow) {
@@ -463,11 +637,11 @@ int ext_thread_blocks_per_cu(int qcap, bool narrow) {
 }
 
 hipError_t launch_ext_thread(hipStream_t st, int qcap, bool narrow, const uint8_t *arena, const uint8_t *ref,
-                             const ExtJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
-                             const ExtScoring &sc, bwg_ext_result *out, int grid) {
+                             const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
+                             uint32_t *head, const ExtScoring &sc, bwg_ext_result *out, int grid) {
 #define BWG_T(Q, N)                                                                                        \
     hipLaunchKernelGGL((ext_thread_kernel<Q, N>), dim3(grid), dim3(64), 0, st, arena, ref, jobs, order, count, \
-                       head, sc, out)
+                       dcount, head, sc, out)
     if (qcap == 16 && narrow) BWG_T(16, true);
     else if (qcap == 16) BWG_T(16, false);
     else if (qcap == 32 && narrow) BWG_T(32, true);

@@ -110,6 +110,7 @@ enum {
     BWG_OPT_EXT_K_STEP = 6,       /* untuned extension group bins use one K above the minimal (default 0) */
     BWG_OPT_KSW_K_STEP = 7,       /* untuned rescue bins use one K above the minimal (default 0) */
     BWG_OPT_EXT_SCORE_WIDTH = 8,  /* extension DP state bits: 16 where exact (default), or 32 always */
+    BWG_OPT_UNGAPPED = 9,         /* batched extension: ungapped-diagonal fast path first (default 1) */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
@@ -145,6 +146,10 @@ int bwg_batch_run(bwg_ctx *ctx, bwg_batch *b);
 /* Copy results (bwg_ext_result or bwg_ksw_result, n of them) to host memory. */
 int bwg_batch_results(bwg_ctx *ctx, bwg_batch *b, void *out);
 void bwg_batch_destroy(bwg_ctx *ctx, bwg_batch *b);
+/* Extension jobs of the batch's last batched run that the ungapped-diagonal fast path resolved
+ * without the banded DP (the reference's batched extension does the same, crates/bwa-neon/src/
+ * batched.rs:716-798); 0 for rescue batches, the plain form, or with BWG_OPT_UNGAPPED off. */
+int bwg_batch_ungapped_hits(bwg_ctx *ctx, bwg_batch *b, uint64_t *hits);
 
 /* ---- serving ----------------------------------------------------------------------------------
  * Continuous batching of alignment work from concurrent producers.  No counterpart in the

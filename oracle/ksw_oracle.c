@@ -338,12 +338,14 @@ typedef struct {
     int32_t o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, minsc, max_sc, batched;
     void *out;
     int64_t cells;
+    int32_t *job_cells; /* optional: cells evaluated per job */
 } orc_slice;
 
 static void *orc_slice_run(void *arg) {
     orc_slice *s = (orc_slice *)arg;
     for (size_t k = s->lo; k < s->hi; ++k) {
         const uint8_t *q = s->arena + s->q_off[k], *t = s->arena + s->t_off[k];
+        const int64_t c0 = s->cells;
         if (s->kind == 0) {
             ((orc_ext_result *)s->out)[k] = orc_ksw_extend2(
                 (int32_t)s->q_len[k], q, (int32_t)s->t_len[k], t, s->m, s->mat, s->o_del,
@@ -358,6 +360,7 @@ static void *orc_slice_run(void *arg) {
                 ql, q, (int32_t)s->t_len[k], t, s->m, s->mat, s->o_del, s->e_del, s->o_ins,
                 s->e_ins, s->minsc, s->max_sc, orc_lanes_for(ql, s->max_sc), &s->cells);
         }
+        if (s->job_cells) s->job_cells[k] = (int32_t)(s->cells - c0);
     }
     return NULL;
 }
@@ -410,6 +413,34 @@ void orc_extend_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, con
     p.zdrop = zdrop;
     p.out = out;
     orc_run_slices(&p, n, nthreads, cells_out);
+}
+
+void orc_extend_batch_cells(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                            const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, int32_t m,
+                            const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                            int32_t w, int32_t end_bonus, int32_t zdrop, orc_ext_result *out,
+                            int32_t *job_cells, int nthreads) {
+    orc_slice p;
+    memset(&p, 0, sizeof p);
+    p.kind = 0;
+    p.arena = arena;
+    p.q_off = q_off;
+    p.q_len = q_len;
+    p.t_off = t_off;
+    p.t_len = t_len;
+    p.h0 = h0;
+    p.m = m;
+    p.mat = mat;
+    p.o_del = o_del;
+    p.e_del = e_del;
+    p.o_ins = o_ins;
+    p.e_ins = e_ins;
+    p.w = w;
+    p.end_bonus = end_bonus;
+    p.zdrop = zdrop;
+    p.out = out;
+    p.job_cells = job_cells;
+    orc_run_slices(&p, n, nthreads, NULL);
 }
 
 void orc_align2_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,

@@ -15,6 +15,7 @@
  *   orc_padded_qlen        crates/bwa-neon/src/matesw.rs:676-686 (ksw_padded_qlen),
  *                          lane rule crates/bwa-mem/src/pe.rs:420-424
  *   orc_batched_align2_job crates/bwa-neon/src/matesw.rs:299-497 (batched_ksw_align2, per job)
+ *   orc_ungapped_*         crates/bwa-neon/src/batched.rs:493-521, 602-697, 756-764 (ksw_ungapped.c)
  *
  * A second, independent restatement in the lockstep (SoA, many jobs per pass) form lives in
  * ksw_lockstep.c, and brute-force checkers in ksw_brute.c.
@@ -81,6 +82,12 @@ void orc_extend_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, con
                       const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins,
                       int32_t e_ins, int32_t w, int32_t end_bonus, int32_t zdrop,
                       orc_ext_result *out, int64_t *cells_out, int nthreads);
+/* orc_extend_batch with the evaluated-cell count of every job in job_cells[n]. */
+void orc_extend_batch_cells(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                            const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, int32_t m,
+                            const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                            int32_t w, int32_t end_bonus, int32_t zdrop, orc_ext_result *out,
+                            int32_t *job_cells, int nthreads);
 void orc_align2_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
                       const uint64_t *t_off, const uint32_t *t_len, int32_t m, const int8_t *mat,
                       int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t minsc,
@@ -116,6 +123,16 @@ int32_t orc_brute_gotoh_local(int32_t qlen, const uint8_t *query, int32_t tlen,
 void orc_brute_enum_local(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target,
                           int32_t m, const int8_t *mat, int32_t o_del, int32_t e_del,
                           int32_t o_ins, int32_t e_ins, int32_t *score, int32_t *te, int32_t *qe);
+
+/* Ungapped-diagonal fast path (ksw_ungapped.c). */
+int orc_is_uniform_dna(const int8_t *mat, int32_t m);
+int32_t orc_ungapped_xmax(const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins);
+int orc_ungapped_hit(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target, int32_t h0,
+                     int32_t a, int32_t b, int32_t xmax, int32_t zdrop, orc_ext_result *out);
+void orc_ungapped_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, int32_t m,
+                        const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                        int32_t zdrop, orc_ext_result *out, uint8_t *hit);
 
 #ifdef __cplusplus
 }

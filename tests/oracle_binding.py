@@ -89,6 +89,14 @@ def lib():
         L.orc_brute_gotoh_local.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I]
         L.orc_brute_enum_local.restype = None
         L.orc_brute_enum_local.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _P, _P, _P]
+        L.orc_extend_batch_cells.restype = None
+        L.orc_extend_batch_cells.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _I, _P,
+                                             _I, _I, _I, _I, _I, _I, _I, _P, _P, C.c_int]
+        L.orc_ungapped_batch.restype = None
+        L.orc_ungapped_batch.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I,
+                                         _P, _P]
+        L.orc_ungapped_xmax.restype = _I
+        L.orc_ungapped_xmax.argtypes = [_P, _I, _I, _I, _I]
         _lib = L
     return _lib
 
@@ -183,6 +191,36 @@ def extend_batch(batch, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, nt
                            5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop,
                            out.ctypes.data, C.addressof(cells), nthreads)
     return (out, cells.value) if want_cells else out
+
+
+def extend_batch_job_cells(batch, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, nthreads=8):
+    """extend_batch plus the number of DP cells ksw_extend2 evaluates for each job."""
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=EXT_DTYPE)
+    cells = np.zeros(n, dtype=np.int32)
+    ma, mp = _mat(mat)
+    lib().orc_extend_batch_cells(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                                 batch.t_off.ctypes.data, batch.t_len.ctypes.data, batch.h0.ctypes.data,
+                                 5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop,
+                                 out.ctypes.data, cells.ctypes.data, nthreads)
+    return out, cells
+
+
+def ungapped_batch(batch, mat, o_del, e_del, o_ins, e_ins, zdrop):
+    """The reference's ungapped fast path on every job: (results, accepted flags)."""
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=EXT_DTYPE)
+    hit = np.zeros(n, dtype=np.uint8)
+    ma, mp = _mat(mat)
+    lib().orc_ungapped_batch(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                             batch.t_off.ctypes.data, batch.t_len.ctypes.data, batch.h0.ctypes.data,
+                             5, mp, o_del, e_del, o_ins, e_ins, zdrop, out.ctypes.data, hit.ctypes.data)
+    return out, hit.astype(bool)
+
+
+def ungapped_xmax(mat, o_del, e_del, o_ins, e_ins):
+    ma, mp = _mat(mat)
+    return lib().orc_ungapped_xmax(mp, o_del, e_del, o_ins, e_ins)
 
 
 def align2_batch(batch, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, batched=True, nthreads=8,

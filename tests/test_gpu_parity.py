@@ -225,3 +225,29 @@ def test_serial_and_concurrent_bins_agree(gpu_ctx):
     gpu_ctx.set_option(bwg.OPT_CONCURRENT_BINS, 1)
     assert np.array_equal(outs[0], outs[1])
     assert np.array_equal(outs[0], ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100))
+
+
+def test_ungapped_fast_path_on_and_off(gpu_ctx):
+    """The ungapped-diagonal pre-pass (crates/bwa-neon/src/batched.rs:602-697) resolves most clean
+    extensions without the DP; results are the oracle's either way, and its hit count equals the
+    oracle restatement's."""
+    _say("ungapped fast path on/off")
+    mat = bwg.default_mat()
+    batch = bwg.generate(bwg.WORKLOAD_EXT_SMALL, 30000, seed=17)
+    params = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+    want = ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100)
+    _, hit = ob.ungapped_batch(batch, mat, 6, 1, 6, 1, 100)
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    try:
+        for on in (1, 0):
+            gpu_ctx.set_option(bwg.OPT_UNGAPPED, on)
+            st = gpu_ctx.stage(batch, params)
+            st.run()
+            got = st.results()
+            hits = st.ungapped_hits()
+            st.close()
+            bad = np.nonzero(got != want)[0]
+            assert bad.size == 0, (on, int(bad[0]))
+            assert hits == (int(hit.sum()) if on else 0)
+    finally:
+        gpu_ctx.set_option(bwg.OPT_UNGAPPED, 1)

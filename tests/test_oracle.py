@@ -8,6 +8,7 @@ reference's own acceptance-gate sweeps.
 import numpy as np
 import pytest
 
+import bwg
 import lcg_gates as G
 import oracle_binding as ob
 
@@ -274,3 +275,56 @@ def test_rescue_invariants():
         if qb >= 0:
             assert 0 <= qb <= qe < len(qs[k]) and 0 <= tb <= te < len(ts[k])
         assert (s2 == -1) == (te2 == -1)
+
+
+# ---- ungapped-diagonal fast path (crates/bwa-neon/src/batched.rs:602-697) -----------------------
+
+def test_ungapped_threshold_matches_reference_cases():
+    """x_threshold (batched.rs:756-764): 1 at bwa's defaults; the -B 3 and -B 2 -E 3 cases the
+    reference documents (a strict win over the cheapest gap)."""
+    assert ob.ungapped_xmax(ob.scmat(1, 4), 6, 1, 6, 1) == 1
+    assert ob.ungapped_xmax(ob.scmat(1, 3), 6, 1, 6, 1) == 1   # 2*(1+3) = 8 > 6+1: X = 1, not 2
+    assert ob.ungapped_xmax(ob.scmat(1, 2), 6, 3, 6, 3) == 2   # 3*(1+2) == 6+3 is a tie: X = 2
+    assert ob.ungapped_xmax([0] * 25, 6, 1, 6, 1) == -1        # degenerate matrix disables it
+
+
+def _assert_ungapped_exact(batch, mat, o_del, e_del, o_ins, e_ins, w, eb, zd):
+    fast, hit = ob.ungapped_batch(batch, mat, o_del, e_del, o_ins, e_ins, zd)
+    full = ob.extend_batch(batch, mat, o_del, e_del, o_ins, e_ins, w, eb, zd)
+    bad = np.nonzero(hit & (fast != full))[0]
+    assert bad.size == 0, "job %d: fast %s dp %s" % (bad[0], fast[bad[0]], full[bad[0]])
+    return hit
+
+
+def test_ungapped_accepts_only_dp_identical_results():
+    """Every job the fast path accepts has exactly ksw_extend2's result (the reference's claim),
+    on the synthetic workloads and on the gate sweeps with their varied scorings."""
+    mat = bwg.default_mat()
+    for kind in (bwg.WORKLOAD_EXT_SMALL, bwg.WORKLOAD_EXT_TAIL):
+        b = bwg.generate(kind, 20000, seed=3)
+        hit = _assert_ungapped_exact(b, mat, 6, 1, 6, 1, 100, 5, 100)
+        assert hit.mean() > 0.3  # clean reads: most extensions are ungapped
+    n_hit = 0
+    for gen in (G.ext_ungapped, G.ext_read_sized):
+        for case in gen():
+            b = bwg.JobBatch.from_pairs(case["queries"], case["targets"], case["h0s"])
+            hit = _assert_ungapped_exact(b, case["mat"], case["o_del"], case["e_del"], case["o_ins"],
+                                         case["e_ins"], case["w"], case["end_bonus"], case["zdrop"])
+            n_hit += int(hit.sum())
+    assert n_hit > 100
+
+
+def test_ungapped_closed_form_and_declines():
+    mat = ob.scmat(1, 4)
+    q = [0, 1, 2, 3] * 10
+    b = bwg.JobBatch.from_pairs([q, q, q, q[:5] + [4] + q[6:], q], [q + [1] * 7, q[:-1], q, q, q[:20] + [1] + q[21:]],
+                                [7, 7, 0 + 1, 7, 7])
+    fast, hit = ob.ungapped_batch(b, mat, 6, 1, 6, 1, 100)
+    assert hit[0] and tuple(fast[0]) == (7 + 40, 40, 40, 40, 47, 0)  # mismatch-free closed form
+    assert not hit[1]   # target shorter than the query
+    assert hit[2] and fast[2]["score"] == 41
+    assert not hit[3]   # an N on the diagonal
+    # one mismatch at column 20: best is the end of the walk, 7 + 39 - 4 = 42 at qle 40
+    assert hit[4] and tuple(fast[4]) == (42, 40, 40, 40, 42, 0)
+    full = ob.extend_batch(b, mat, 6, 1, 6, 1, 100, 0, 100)
+    assert np.array_equal(fast[hit], full[hit])
