@@ -33,6 +33,7 @@ BWG_OK, BWG_EIO, BWG_E2BIG, BWG_EAGAIN, BWG_ENOMEM, BWG_ENODEV, BWG_EINVAL = 0, 
 FORM_PLAIN, FORM_BATCHED = 0, 1
 OPT_FORM, OPT_CONCURRENT_BINS, OPT_GRAPHS, OPT_EXT_THREAD_MAX_Q = 1, 2, 3, 4
 OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP, OPT_EXT_SCORE_WIDTH, OPT_UNGAPPED = 5, 6, 7, 8, 9
+OPT_KSW_SCORE_WIDTH = 10
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 
 # Every symbol include/bwg.h declares.

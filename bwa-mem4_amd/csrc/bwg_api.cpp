@@ -140,6 +140,11 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
         c->tuning.ext_force_wide = v == 32;
         c->tuner.clear();
         return BWG_OK;
+    case BWG_OPT_KSW_SCORE_WIDTH:
+        if (v != 16 && v != 32) return BWG_EINVAL;
+        c->tuning.ksw_force_wide = v == 32;
+        c->tuner.clear();
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -167,6 +172,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_KSW_K_STEP: return c->tuning.ksw_k_step;
     case BWG_OPT_EXT_SCORE_WIDTH: return c->tuning.ext_force_wide ? 32 : 16;
     case BWG_OPT_UNGAPPED: return c->tuning.ungapped;
+    case BWG_OPT_KSW_SCORE_WIDTH: return c->tuning.ksw_force_wide ? 32 : 16;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;

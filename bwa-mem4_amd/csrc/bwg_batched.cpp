@@ -30,7 +30,7 @@ namespace {
 struct Item {
     int kind = 0;
     int param = 0;     // K, or the thread kernel's column capacity
-    bool narrow = false;  // extension: 16-bit column state
+    bool narrow = false;  // 16-bit DP state (extension thread kernel, packed rescue kernel)
     int qcap = 0;      // extension group: LDS sizing
     int stage = 0;     // items of one stage run concurrently; stages run in order
     uint32_t offset = 0, count = 0, head = 0;
@@ -175,8 +175,8 @@ int ext_grid(const bwg_ctx *c, int kind, int param, bool narrow, int qcap, uint3
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)count + 63) / 64));
 }
 
-int ksw_grid(const bwg_ctx *c, int K, bool rev, uint32_t count) {
-    const int per_cu = ksw_group_blocks_per_cu(K, rev);
+int ksw_grid(const bwg_ctx *c, int K, bool rev, uint32_t count, bool packed = false) {
+    const int per_cu = packed ? ksw_pk_blocks_per_cu(K, rev) : ksw_group_blocks_per_cu(K, rev);
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ksw_group_oversub,
                                                        ((int64_t)count + 15) / 16));
 }
@@ -433,11 +433,18 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
     }
     TuneEntry ent;
     ent.sample = S;
-    const int K = 2 * (cls + 1);
+    const int K = 2 * (cls % 8 + 1);
     for (int k2 = K; k2 <= std::min(16, K + 2); k2 += 2) {
         TuneCandidate t;
         t.kind = kKswFwd;
         t.param = k2;
+        ent.cands.push_back(t);
+    }
+    if (cls < 8 && !c->tuning.ksw_force_wide) {  // 16-bit-exact jobs: the packed kernel too
+        TuneCandidate t;
+        t.kind = kKswFwd;
+        t.param = K;
+        t.width = 16;
         ent.cands.push_back(t);
     }
     uint32_t *d = (uint32_t *)c->pool.alloc(((size_t)S + 1) * 4);
@@ -453,11 +460,17 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
     for (int rep = 0; rep < kTuneReps && e == hipSuccess; ++rep) {
         for (size_t k = 0; k < ent.cands.size() && e == hipSuccess; ++k) {
             const int kk = ent.cands[k].param;
+            const bool packed = ent.cands[k].width == 16;
             e = launch_zero_u32(st, d, 1);
             if (e == hipSuccess) e = hipEventRecord(e0, st);
-            if (e == hipSuccess)
-                e = launch_ksw_group(st, kk, false, b->d_arena, b->d_ref, dj, d + 1, S, d, b->ksc,
-                                     (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, kk, false, S));
+            if (e == hipSuccess) {
+                if (packed)
+                    e = launch_ksw_pk(st, kk, false, b->d_arena, b->d_ref, dj, d + 1, S, d, b->ksc,
+                                      (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, kk, false, S, true));
+                else
+                    e = launch_ksw_group(st, kk, false, b->d_arena, b->d_ref, dj, d + 1, S, d, b->ksc,
+                                         (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, kk, false, S));
+            }
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
             float ms = 0.f;
@@ -485,39 +498,51 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const KswScoring &sc = b->ksc;
     const Tuning &tu = c->tuning;
     std::vector<Item> bins;
-    auto find = [&](int kind, int param) {
+    auto find = [&](int kind, int param, bool packed) {
         for (size_t k = 0; k < bins.size(); ++k)
-            if (bins[k].kind == kind && bins[k].param == param) return (int)k;
+            if (bins[k].kind == kind && bins[k].param == param && bins[k].narrow == packed) return (int)k;
         Item it;
         it.kind = kind;
         it.param = param;
+        it.narrow = packed;
         it.stage = kind == kKswFwd ? 0 : 1;  // fast bins first in the order array
         bins.push_back(it);
         return (int)bins.size() - 1;
     };
-    // class = the minimal K (index K/2-1); tune the classes with enough jobs
-    std::vector<int8_t> kmin(n);
-    std::vector<std::vector<uint32_t>> members(8);
+    // class = the minimal K (index K/2-1), +8 for jobs whose scores may not fit 16 bits; tune the
+    // classes with enough jobs
+    const bool pk_scoring = ksw_pk_scoring_ok(sc);
+    const int32_t mm = std::max<int32_t>(1, mat_max(sc.mat));
+    std::vector<int8_t> cls_of(n);
+    std::vector<std::vector<uint32_t>> members(16);
     for (size_t k = 0; k < n; ++k) {
-        const int K = sc.o_ins >= 0 ? ksw_k_for(widest_pad((int)b->hksw[k].qlen, sc.max_sc)) : -1;
-        kmin[k] = (int8_t)K;
-        if (K > 0) members[K / 2 - 1].push_back((uint32_t)k);
+        const int qlen = (int)b->hksw[k].qlen;
+        const int K = sc.o_ins >= 0 ? ksw_k_for(widest_pad(qlen, sc.max_sc)) : -1;
+        const bool pk = pk_scoring && (int64_t)qlen * mm < kKswPkMaxScore;
+        cls_of[k] = (int8_t)(K < 0 ? -1 : K / 2 - 1 + (pk ? 0 : 8));
+        if (K > 0) members[cls_of[k]].push_back((uint32_t)k);
     }
     const uint64_t sig = scoring_sig(sc);
-    std::vector<int> chosen(8, 0);
-    for (int cl = 0; cl < 8; ++cl) {
-        chosen[cl] = tu.ksw_k_step ? std::min(16, 2 * (cl + 1) + 2) : 2 * (cl + 1);
+    std::vector<int> chosen(16, 0);
+    std::vector<char> chosen_pk(16, 0);
+    for (int cl = 0; cl < 16; ++cl) {
+        const int K = 2 * (cl % 8 + 1);
+        chosen[cl] = tu.ksw_k_step ? std::min(16, K + 2) : K;
+        chosen_pk[cl] = cl < 8 && !tu.ksw_force_wide;
         if (!tu.autotune) continue;
         if (members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
             const int rc = tune_ksw_class(c, b, sig, cl, members[cl], st);
             if (rc) return rc;
         }
         TuneEntry ent;
-        if (c->tuner.lookup(sig, cl, &ent)) chosen[cl] = ent.cands[ent.choice].param;
+        if (c->tuner.lookup(sig, cl, &ent)) {
+            chosen[cl] = ent.cands[ent.choice].param;
+            chosen_pk[cl] = ent.cands[ent.choice].width == 16;
+        }
     }
     auto key_of = [&](size_t k) {
-        const int K = kmin[k];
-        return K < 0 ? find(kKswPlain, 0) : find(kKswFwd, chosen[K / 2 - 1]);
+        const int cl = cls_of[k];
+        return cl < 0 ? find(kKswPlain, 0, false) : find(kKswFwd, chosen[cl], chosen_pk[cl] != 0);
     };
     auto work_of = [&](size_t k) { return (uint64_t)b->hksw[k].tlen; };
     p->order = bin_jobs(n, bins, key_of, work_of);
@@ -543,13 +568,13 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             Item f = it;
             f.stage = 0;
             f.head = (uint32_t)heads++;
-            f.grid = ksw_grid(c, it.param, false, it.count);
+            f.grid = ksw_grid(c, it.param, false, it.count, it.narrow);
             items.push_back(f);
             Item r = it;
             r.kind = kKswRev;
             r.stage = 2;
             r.head = (uint32_t)heads++;
-            r.grid = ksw_grid(c, it.param, true, it.count);
+            r.grid = ksw_grid(c, it.param, true, it.count, it.narrow);
             items.push_back(r);
         } else {
             Item pl = it;
@@ -618,6 +643,10 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
                                         (bwg_ext_result *)b->d_out);
     case kKswFwd:
     case kKswRev:
+        if (it.narrow)
+            return launch_ksw_pk(st, it.param, it.kind == kKswRev, b->d_arena, b->d_ref, (const KswJob *)p->d_jobs,
+                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->ksc,
+                                 (bwg_ksw_result *)b->d_out, p->d_rowmax, it.grid);
         return launch_ksw_group(st, it.param, it.kind == kKswRev, b->d_arena, b->d_ref, (const KswJob *)p->d_jobs,
                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->ksc,
                                 (bwg_ksw_result *)b->d_out, p->d_rowmax, it.grid);

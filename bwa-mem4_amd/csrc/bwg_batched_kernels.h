@@ -36,6 +36,19 @@ int ksw_group_blocks_per_cu(int K, bool rev);
 hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                             const uint32_t *order, uint32_t count, uint32_t *head, const KswScoring &sc,
                             bwg_ksw_result *out, int32_t *rowmax, int grid);
+// Packed 16-bit form of the rescue pass (K columns per lane as K/2 pairs); exact for jobs with
+// qlen * (largest matrix score) < kKswPkMaxScore under a scoring that ksw_pk_scoring_ok accepts:
+// non-negative gap opens and open + extend below 2^14, so H*64 + 63 and every unfloored E and F
+// stay inside int16.
+constexpr int64_t kKswPkMaxScore = 500;
+inline bool ksw_pk_scoring_ok(const KswScoring &sc) {
+    return sc.o_del >= 0 && sc.o_ins >= 0 && sc.e_del >= 0 && sc.e_ins >= 0 && sc.o_del + sc.e_del < 16384 &&
+           sc.o_ins + sc.e_ins < 16384;
+}
+int ksw_pk_blocks_per_cu(int K, bool rev);
+hipError_t launch_ksw_pk(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref,
+                         const KswJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
+                         const KswScoring &sc, bwg_ksw_result *out, int32_t *rowmax, int grid);
 hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t *list, uint32_t count,
                               const KswScoring &sc, const int32_t *rowmax, bwg_ksw_result *out);
 

@@ -56,6 +56,7 @@ struct Tuning {
     bool ext_k_step = false;       // untuned extension bins: one K above the minimal one
     bool ksw_k_step = false;       // untuned rescue bins: one K above the minimal one
     bool ext_force_wide = false;   // extension: 32-bit column state even where 16 bits are exact
+    bool ksw_force_wide = false;   // rescue: 32-bit DP state even where 16 bits are exact
     bool ungapped = true;          // extension: the reference's ungapped-diagonal fast path first
     bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
     bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph

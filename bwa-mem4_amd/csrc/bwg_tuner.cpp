@@ -33,7 +33,7 @@ const char *mapping_name(int kind, int param, int width, char *buf, size_t len) 
     switch (kind) {
     case kExtThread: std::snprintf(buf, len, "ext_thread<%d,i%d>", param, width); break;
     case kExtGroup: std::snprintf(buf, len, "ext_group<%d,i%d>", param, width); break;
-    case kKswFwd: std::snprintf(buf, len, "ksw_group<%d>", param); break;
+    case kKswFwd: std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16>" : "ksw_group<%d,i32>", param); break;
     default: std::snprintf(buf, len, "plain"); break;
     }
     return buf;
@@ -67,7 +67,8 @@ std::string Tuner::report() const {
         const bool resc = !e.cands.empty() && e.cands[0].kind == kKswFwd;
         int n;
         if (resc)
-            n = std::snprintf(line, sizeof line, "rescue pad<=%d", 32 * (cls + 1));
+            n = std::snprintf(line, sizeof line, "rescue pad<=%d%s", 32 * (cls % 8 + 1),
+                              cls < 8 ? " (16-bit exact)" : "");
         else if (cls < kExtShortClasses)
             n = std::snprintf(line, sizeof line, "extend qlen %d-%d", 4 * cls + 1, 4 * cls + 4);
         else

@@ -197,6 +197,202 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
     }
 }
 
+// ---- packed 16-bit rescue pass ----------------------------------------------------------------
+// The same DP as ksw_group_kernel with two query columns per 32-bit lane register: lane g owns the
+// K = 2*KH padded columns g*K .. g*K+K-1 as KH pairs (g*K + p, g*K + KH + p), so every VALU op of
+// the cell update (v_pk_add/sub/max_i16) advances both halves of the lane's block.  Exact when every
+// score fits 16 bits, which the planner guarantees by routing here only jobs with qlen times the
+// largest matrix score below 500 and gap penalties (open + extend) below 2^14: H stays below 500,
+// so H*64 + 63 fits, and E and F are kept without their floor at 0 (every reader applies that floor
+// through a max with a non-negative value), so they stay above -2*(o + e).
+//
+// The row maximum with the first-column tie rule is a 16-bit key h*64 + (63 - c) per half, c the
+// column within the lane's block, -32768 added for columns past the padded query so they never win;
+// a lane's best key becomes the usual 32-bit key h << 13 | 8191-j.  F enters the block's upper half
+// as max(F_in - KH*e_ins, c_lo), c_lo the lower half's exit from F = 0.  The query profile (five
+// target codes x KH column pairs) sits in LDS, lane-major, one 4-byte read per pair and row.
+typedef short pk16 __attribute__((ext_vector_type(2)));
+
+__device__ __forceinline__ pk16 pk_splat(int32_t v) { return (pk16){(short)v, (short)v}; }
+__device__ __forceinline__ pk16 pk_max(pk16 a, pk16 b) { return __builtin_elementwise_max(a, b); }
+__device__ __forceinline__ uint32_t pk_bits(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
+__device__ __forceinline__ pk16 pk_from(uint32_t w) { return __builtin_bit_cast(pk16, w); }
+
+template <int KH, bool REV>
+__global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__ arena,
+                                                     const uint8_t *__restrict__ ref,
+                                                     const KswJob *__restrict__ jobs,
+                                                     const uint32_t *__restrict__ order,
+                                                     uint32_t count, uint32_t *__restrict__ head,
+                                                     KswScoring sc, bwg_ksw_result *__restrict__ out,
+                                                     int32_t *__restrict__ rowmax) {
+    constexpr int K = 2 * KH;
+    __shared__ __align__(8) uint8_t tblT[40];
+    __shared__ uint32_t prof[5 * KH * 256];  // [target code][pair][thread]
+    if (threadIdx.x < 40) {
+        const int qc = threadIdx.x >> 3, tc = threadIdx.x & 7;
+        tblT[threadIdx.x] = tc < 5 ? (uint8_t)sc.mat[tc * 5 + qc] : (uint8_t)0;
+    }
+    __syncthreads();
+    const int lane = threadIdx.x & 63;
+    const int g = lane & 15;
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
+    const int32_t Ke = K * e_ins, KHe = KH * e_ins;
+    const pk16 e_ins2 = pk_splat(e_ins), e_del2 = pk_splat(e_del), oe_ins2 = pk_splat(oe_ins),
+               oe_del2 = pk_splat(oe_del), zero2 = pk_splat(0);
+
+    bool active = false, exhausted = false;
+    TargetView t = {nullptr, 1};
+    int32_t tlen = 0, endsc = 0, i = 0, gmax = 0, te = -1, qe = -1, tnext = 0;
+    uint32_t idx = 0;
+    int32_t *rm = nullptr;
+    pk16 hrow[KH], erow[KH], kk[KH];
+
+    for (;;) {
+        if (!active && !exhausted) {
+            uint32_t slot = 0;
+            if (g == 0) slot = atomicAdd(head, 1u);
+            slot = __shfl(slot, lane & ~15, 64);
+            if (slot >= count) {
+                exhausted = true;
+            } else {
+                idx = order[slot];
+                const KswJob jb = jobs[idx];
+                const uint8_t *q = arena + jb.q_off;
+                t = target_view(arena, ref, jb.t_off, jb.tlen);
+                int32_t qlen = (int32_t)jb.qlen;
+                tlen = (int32_t)jb.tlen;
+                endsc = 0x7fffffff;
+                bool run = qlen > 0 && tlen > 0;
+                if (REV) {
+                    const bwg_ksw_result f = out[idx];
+                    run = f.score >= sc.minsc && f.qe >= 0;
+                    qlen = min(f.qe + 1, qlen);
+                    tlen = f.te + 1;
+                    endsc = f.score;
+                    te = f.te;  // forward coordinates, kept for the start computation
+                    qe = f.qe;
+                } else {
+                    rm = rowmax + jb.aux0;
+                    if (!run && g == 0) {
+                        bwg_ksw_result r;
+                        r.score = 0;
+                        r.qb = r.qe = r.tb = r.te = r.score2 = r.te2 = -1;
+                        out[idx] = r;
+                    }
+                }
+                if (run) {
+                    const int32_t qpad = ksw_padded(qlen, sc.max_sc);
+#pragma unroll
+                    for (int p = 0; p < KH; ++p) {
+                        const int jl = g * K + p, jh = g * K + KH + p;
+                        // padding columns score 0 against every target base
+                        const uint32_t ql = jl < qlen ? (REV ? q[qlen - 1 - jl] : q[jl]) : 8u;
+                        const uint32_t qh = jh < qlen ? (REV ? q[qlen - 1 - jh] : q[jh]) : 8u;
+#pragma unroll
+                        for (int c = 0; c < 5; ++c) {
+                            const int32_t sl = ql < 8u ? (int32_t)(int8_t)tblT[8 * ql + c] : 0;
+                            const int32_t sh = qh < 8u ? (int32_t)(int8_t)tblT[8 * qh + c] : 0;
+                            prof[(c * KH + p) * 256 + threadIdx.x] = pk_bits((pk16){(short)sl, (short)sh});
+                        }
+                        // columns past the padded query are not part of the DP: keyed out
+                        kk[p] = (pk16){(short)(jl < qpad ? 63 - p : -32768), (short)(jh < qpad ? 63 - KH - p : -32768)};
+                        hrow[p] = zero2;
+                        erow[p] = zero2;
+                    }
+                    i = 0;
+                    gmax = 0;
+                    if (!REV) {
+                        te = -1;
+                        qe = -1;
+                    }
+                    tnext = REV ? t[tlen - 1] : t[0];
+                    active = true;
+                }
+            }
+        }
+        if (__ballot(active || !exhausted) == 0) break;
+        if (!active) continue;
+
+        // ---- one DP row ----------------------------------------------------------------------
+        const int32_t tc = tnext;
+        if (i + 1 < tlen) tnext = REV ? t[tlen - 2 - i] : t[i + 1];
+        const uint32_t *pr = prof + (size_t)tc * KH * 256 + threadIdx.x;
+        pk16 s2[KH];
+#pragma unroll
+        for (int p = 0; p < KH; ++p) s2[p] = pk_from(pr[p * 256]);
+        // H(i-1, column-1): lower half of pair 0 from the previous lane's last column (0 left of
+        // column 0), upper half of pair 0 from this lane's last lower column, then pair p-1
+        const int32_t prev_last = row_shift_in((int32_t)(pk_bits(hrow[KH - 1]) >> 16), 0);
+        pk16 hd = pk_from((pk_bits(hrow[KH - 1]) << 16) | ((uint32_t)prev_last & 0xffffu));
+        pk16 dd[KH], dm[KH];
+        pk16 grun = zero2;
+#pragma unroll
+        for (int p = 0; p < KH; ++p) {
+            const pk16 d = pk_max(pk_max(hd + s2[p], erow[p]), zero2);
+            hd = hrow[p];
+            dd[p] = d;
+            dm[p] = d - oe_ins2;
+            grun = pk_max(grun - e_ins2, dm[p]);
+        }
+        const int32_t c_lo = max((int32_t)grun.x, 0), c_hi = max((int32_t)grun.y, 0);
+        const int32_t U = max(c_hi, c_lo - KHe) + g * Ke;
+        const int32_t incl = row_scan_max_nonneg(U);
+        const int32_t excl = row_shift_in(incl, -Ke);
+        const int32_t f_in = excl - (g - 1) * Ke;
+        pk16 f = (pk16){(short)f_in, (short)max(f_in - KHe, c_lo)};
+        pk16 key2 = zero2;
+#pragma unroll
+        for (int p = 0; p < KH; ++p) {
+            const pk16 h = pk_max(dd[p], f);
+            key2 = pk_max(key2, h * pk_splat(64) + kk[p]);
+            erow[p] = pk_max(erow[p] - e_del2, h - oe_del2);
+            f = pk_max(f - e_ins2, dm[p]);
+            hrow[p] = h;
+        }
+        const int32_t kbest = max((int32_t)key2.x, (int32_t)key2.y);
+        const int32_t lkey = kbest > 0 ? ((kbest >> 6) << kKeyShift) | (kKeyMask - (g * K + 63 - (kbest & 63))) : 0;
+        const int32_t key = row_allmax(lkey);
+        const int32_t imax = key >> kKeyShift;
+        if (!REV && g == 0) rm[64 * i] = imax;
+        bool done = false;
+        if (imax > gmax) {
+            gmax = imax;
+            if (REV) {
+                if (gmax >= endsc) {
+                    // first row reaching the forward score: the alignment starts rte/rqe back
+                    if (g == 0 && gmax == endsc) {
+                        const int32_t rqe = kKeyMask - (key & kKeyMask);
+                        out[idx].tb = te - i;
+                        out[idx].qb = qe - rqe;
+                    }
+                    done = true;
+                }
+            } else {
+                te = i;
+                qe = kKeyMask - (key & kKeyMask);
+            }
+        }
+        ++i;
+        if (i >= tlen) done = true;
+        if (done) {
+            if (!REV && g == 0) {
+                bwg_ksw_result r;
+                r.score = gmax;
+                r.qb = -1;
+                r.qe = te >= 0 ? qe : -1;
+                r.tb = -1;
+                r.te = te;
+                r.score2 = -1;
+                r.te2 = -1;
+                out[idx] = r;
+            }
+            active = false;
+        }
+    }
+}
+
 // score2/te2 from the recorded row maxima (crates/bwa-extend/src/sw.rs:806-851): replay
 // push_row's merge rule and keep the best closed entry outside the exclusion window around te.
 __global__ __launch_bounds__(256) void ksw_tracker_kernel(const KswJob *__restrict__ jobs,
@@ -257,7 +453,48 @@ int occ() {
     return v;
 }
 
+template <int KH, bool REV>
+int occ_pk() {
+    static std::atomic<int> cache{0};
+    int v = cache.load(std::memory_order_relaxed);
+    if (v > 0) return v;
+    int blocks = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ksw_pk_kernel<KH, REV>, 256, 0) != hipSuccess)
+        blocks = 1;
+    v = blocks > 0 ? blocks : 1;
+    cache.store(v, std::memory_order_relaxed);
+    return v;
+}
+
 }  // namespace
+
+int ksw_pk_blocks_per_cu(int K, bool rev) {
+    switch (K) {
+#define BWG_K(k) case k: return rev ? occ_pk<k / 2, true>() : occ_pk<k / 2, false>();
+        BWG_KSW_GROUP_KS
+#undef BWG_K
+    default: return 1;
+    }
+}
+
+hipError_t launch_ksw_pk(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref,
+                         const KswJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
+                         const KswScoring &sc, bwg_ksw_result *out, int32_t *rowmax, int grid) {
+    switch (K) {
+#define BWG_K(k)                                                                                          \
+    case k:                                                                                               \
+        if (rev)                                                                                          \
+            hipLaunchKernelGGL((ksw_pk_kernel<k / 2, true>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
+                               order, count, head, sc, out, rowmax);                                      \
+        else                                                                                              \
+            hipLaunchKernelGGL((ksw_pk_kernel<k / 2, false>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
+                               order, count, head, sc, out, rowmax);                                      \
+        return hipGetLastError();
+        BWG_KSW_GROUP_KS
+#undef BWG_K
+    default: return hipErrorInvalidValue;
+    }
+}
 
 int ksw_group_blocks_per_cu(int K, bool rev) {
     switch (K) {

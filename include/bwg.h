@@ -99,7 +99,7 @@ int bwg_ksw_align2_batch(bwg_ctx *ctx, const bwg_ksw_params *p, size_t n, const 
  * lanes per job; the group kernel's columns per lane; 16- or 32-bit stored DP state) and the
  * context keeps the fastest for that scoring.  Setting the option clears the table.  With
  * autotuning off, the mapping follows BWG_OPT_EXT_THREAD_MAX_Q, the K_STEP options and
- * BWG_OPT_EXT_SCORE_WIDTH, which force each candidate for tests. */
+ * BWG_OPT_EXT_SCORE_WIDTH / BWG_OPT_KSW_SCORE_WIDTH, which force each candidate for tests. */
 enum { BWG_FORM_PLAIN = 0, BWG_FORM_BATCHED = 1 };
 enum {
     BWG_OPT_FORM = 1,
@@ -111,6 +111,7 @@ enum {
     BWG_OPT_KSW_K_STEP = 7,       /* untuned rescue bins use one K above the minimal (default 0) */
     BWG_OPT_EXT_SCORE_WIDTH = 8,  /* extension DP state bits: 16 where exact (default), or 32 always */
     BWG_OPT_UNGAPPED = 9,         /* batched extension: ungapped-diagonal fast path first (default 1) */
+    BWG_OPT_KSW_SCORE_WIDTH = 10, /* rescue DP state bits: 16 where exact (default), or 32 always */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */

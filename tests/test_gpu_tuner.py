@@ -55,13 +55,71 @@ def test_forced_rescue_mappings_agree(gpu_ctx):
     gpu_ctx.set_form(bwg.FORM_BATCHED)
     gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
     try:
-        for kstep in (0, 1):
-            _say("forced rescue k_step=%d" % kstep)
+        # minimal K or one above; packed 16-bit pairs where exact, or 32-bit state always
+        for kstep, width in [(0, 16), (1, 16), (0, 32), (1, 32)]:
+            _say("forced rescue k_step=%d width=%d" % (kstep, width))
             gpu_ctx.set_option(bwg.OPT_KSW_K_STEP, kstep)
+            gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, width)
+            gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
             got = gpu_ctx.align2_arrays(batch, params)
-            assert _first_bad(got, want) is None, (kstep, _first_bad(got, want))
+            assert _first_bad(got, want) is None, (kstep, width, _first_bad(got, want))
     finally:
         gpu_ctx.set_option(bwg.OPT_KSW_K_STEP, 0)
+        gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, 16)
+        gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
+
+
+def _rescue_near_bound(rng, n, lo, hi):
+    """Queries of lo..hi bases placed (mutated) inside random windows, so the best local score
+    reaches qlen * match for the exact copies: the packed kernel's 16-bit bound is met from both
+    sides."""
+    qs, ts = [], []
+    for k in range(n):
+        L = int(rng.integers(lo, hi + 1))
+        q = rng.integers(0, 4, L).astype(np.uint8)
+        body = q.copy()
+        if k % 3:
+            for _ in range(int(rng.integers(1, 6))):
+                pos = int(rng.integers(0, len(body)))
+                op = int(rng.integers(0, 3))
+                if op == 0:
+                    body[pos] = (body[pos] + 1) % 4
+                elif op == 1:
+                    body = np.delete(body, pos)
+                else:
+                    body = np.insert(body, pos, rng.integers(0, 4, int(rng.integers(1, 4))).astype(np.uint8))
+        if k % 17 == 0:
+            q[int(rng.integers(0, L))] = 4  # an N
+        pre = rng.integers(0, 4, int(rng.integers(0, 80))).astype(np.uint8)
+        post = rng.integers(0, 4, int(rng.integers(0, 80))).astype(np.uint8)
+        qs.append(q)
+        ts.append(np.concatenate([pre, body, post]))
+    return bwg.JobBatch.from_pairs(qs, ts)
+
+
+@pytest.mark.parametrize("a,b,o,e,lo,hi", [(1, 4, 6, 1, 100, 256), (2, 3, 5, 2, 200, 256),
+                                           (3, 4, 6, 1, 130, 200), (4, 2, 0, 16383, 100, 150),
+                                           (1, 4, 16000, 300, 100, 256), (2, 3, 16383, 1, 200, 256)])
+def test_rescue_score_width_boundary(gpu_ctx, a, b, o, e, lo, hi):
+    """Jobs on both sides of the 16-bit bound (qlen * match < 500) in one batch, with ordinary and
+    extreme gap penalties (the last set is one past the packed kernel's gap limit): both widths
+    agree with the oracle."""
+    _say("rescue width boundary a=%d b=%d o=%d e=%d" % (a, b, o, e))
+    rng = np.random.default_rng(1000 + a * 7 + o)
+    batch = _rescue_near_bound(rng, 1500, lo, hi)
+    mat = bwg.default_mat(a, b)
+    params = bwg.ksw_params(5, mat, o, e, o, e, 20, a)
+    want = ob.align2_batch(batch, mat, o, e, o, e, 20, a)
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
+    try:
+        for width in (16, 32):
+            gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, width)
+            gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
+            got = gpu_ctx.align2_arrays(batch, params)
+            assert _first_bad(got, want) is None, (width, _first_bad(got, want))
+    finally:
+        gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, 16)
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
