@@ -94,7 +94,7 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
-    ap.add_argument("--only", default="ext,small,resc,e2e", help="comma list of workloads to time")
+    ap.add_argument("--only", default="ext,small,resc,side,e2e", help="comma list of workloads to time")
     ap.add_argument("--serve-rates", default="5e4,2e5,2e6,1e7,3e7,6e7",
                     help="offered loads (jobs/s) of the serving sweep; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
@@ -189,6 +189,26 @@ def main():
             pass
         except bwg.BwgError as e:
             log("form %s unavailable: %s" % (fname, e))
+    side = {}
+    if "side" in only:
+        # one side of the seed extensions with bwa's band doubling, all rounds on the device: the
+        # long-tail batch at w = 100 (as above) and at w = 5, where a few percent of the jobs fail
+        # the acceptance test and run again at w = 10; left (-1) and right (h0) jobs alternate
+        ctx.set_form(bwg.FORM_BATCHED)
+        prev = np.where(np.arange(args.jobs) % 2 == 0, -1, ext.h0).astype(np.int32)
+        for w in (100, 5):
+            p = bwg.ext_params(5, mat, 6, 1, 6, 1, w, 5, 100)
+            log("side batch w=%d: staging" % w)
+            st = ctx.stage_side(ext, p, prev, 2)
+            st.run()
+            per, kern, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+            bands = st.side_bands()
+            if w == 100 and "ext_batched" in outs:
+                assert np.array_equal(st.results(), outs["ext_batched"]), "side batch differs from the plain batch"
+            st.close()
+            side[w] = dict(step_s=per, kernel_s=kern, retried=int((bands != w).sum()))
+            log("side batch w=%d: %.3f ms/step (kernels %.3f ms), %d jobs retried at w=%d"
+                % (w, per * 1e3, kern * 1e3, side[w]["retried"], 2 * w))
     e2e = None
     if "e2e" in only and "ext" in only:
         # PCIe-inclusive: the synchronous entry point from host buffers (stage, run, copy back)
@@ -292,6 +312,9 @@ def main():
         line["speedup_vs_plain"] = jobs_s / line["plain_jobs_per_s"]
     if e2e:
         line["e2e_jobs_per_s"] = e2e
+    for w, r in side.items():
+        line["side_w%d_jobs_per_s" % w] = args.jobs / r["step_s"]
+        line["side_w%d_retried" % w] = r["retried"]
     for f in ("batched", "plain"):
         if "small_" + f in res:
             r = res["small_" + f]

@@ -43,7 +43,7 @@ EXPORTED_SYMBOLS = (
     "bwg_batch_stage_ksw", "bwg_batch_run", "bwg_batch_results", "bwg_batch_destroy",
     "bwg_server_create", "bwg_server_set_reference", "bwg_server_submit", "bwg_server_poll",
     "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy", "bwg_hbm_copy", "bwg_tune_report",
-    "bwg_batch_ungapped_hits",
+    "bwg_batch_ungapped_hits", "bwg_extend_side_batch", "bwg_batch_stage_side", "bwg_batch_side_bands",
 )
 
 
@@ -109,6 +109,14 @@ def _lib():
         L.bwg_batch_ungapped_hits.argtypes = [_P, _P, _P]
         L.bwg_tune_report.restype = C.c_int
         L.bwg_tune_report.argtypes = [_P, C.c_char_p, C.c_size_t]
+        L.bwg_extend_side_batch.restype = C.c_int
+        L.bwg_extend_side_batch.argtypes = [_P, _P, C.c_int, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P,
+                                            _P, _P]
+        L.bwg_batch_stage_side.restype = C.c_int
+        L.bwg_batch_stage_side.argtypes = [_P, _P, C.c_int, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P,
+                                           _P]
+        L.bwg_batch_side_bands.restype = C.c_int
+        L.bwg_batch_side_bands.argtypes = [_P, _P, _P]
         _L = L
     return _L
 
@@ -252,6 +260,28 @@ class Context:
                                            out.ctypes.data), "bwg_ksw_align2_batch")
         return out
 
+    def extend_side_arrays(self, batch, params, prev, max_band_try=2):
+        """One side of the seed extensions with bwa's band doubling (the reference's run_side,
+        crates/bwa-mem/src/across.rs:1357-1460): (results, accepted band per job)."""
+        out = np.zeros(len(batch), dtype=EXT_DTYPE)
+        w_out = np.zeros(len(batch), dtype=np.int32)
+        prev = np.ascontiguousarray(prev, dtype=np.int32)
+        n, ap, al, qo, ql, to, tl = batch._args()
+        _check(_lib().bwg_extend_side_batch(self.h, C.byref(params), int(max_band_try), n, ap, al, qo, ql, to, tl,
+                                            batch.h0.ctypes.data, prev.ctypes.data, out.ctypes.data,
+                                            w_out.ctypes.data), "bwg_extend_side_batch")
+        return out, w_out
+
+    def stage_side(self, batch, params, prev, max_band_try=2):
+        """Stage a band-retry side batch; run it with StagedBatch.run, bands with .side_bands()."""
+        h = C.c_void_p()
+        prev = np.ascontiguousarray(prev, dtype=np.int32)
+        n, ap, al, qo, ql, to, tl = batch._args()
+        _check(_lib().bwg_batch_stage_side(self.h, C.byref(params), int(max_band_try), n, ap, al, qo, ql, to, tl,
+                                           batch.h0.ctypes.data, prev.ctypes.data, C.byref(h)),
+               "bwg_batch_stage_side")
+        return StagedBatch(self, h, n, EXT_DTYPE)
+
     def stage(self, batch, params):
         """Stage a batch into device memory; returns a StagedBatch that can be run repeatedly."""
         h = C.c_void_p()
@@ -276,6 +306,12 @@ class StagedBatch:
         out = np.zeros(self.n, dtype=self.dtype)
         _check(_lib().bwg_batch_results(self.ctx.h, self.h, out.ctypes.data), "bwg_batch_results")
         return out
+
+    def side_bands(self):
+        """Accepted band of every job of a side batch's last run."""
+        w = np.zeros(self.n, dtype=np.int32)
+        _check(_lib().bwg_batch_side_bands(self.ctx.h, self.h, w.ctypes.data), "bwg_batch_side_bands")
+        return w
 
     def ungapped_hits(self):
         """Jobs of the last batched run resolved by the ungapped-diagonal fast path."""

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <climits>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -211,25 +212,66 @@ int bwg_hbm_copy(bwg_ctx *c, size_t bytes, int reps, double *gb_per_s) {
     return bwg_map_hip(e);
 }
 
+}  // extern "C"
+
+namespace {
+
+// Preconditions of ksw_extend2 (crates/bwa-extend/src/lib.rs:181-195, sw.rs:203-230).
+int check_ext(const bwg_ext_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
+              const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0) {
+    int rc = check_mat(p->m, p->mat);
+    if (rc) return rc;
+    if (p->e_del < 1 || p->e_ins < 1) return BWG_EINVAL;
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    if (rc) return rc;
+    if (n && !h0) return BWG_EINVAL;
+    for (size_t k = 0; k < n; ++k)
+        if (h0[k] <= 0 || q_len[k] < 1) return BWG_EINVAL;
+    return BWG_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
 int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
                         const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
                         bwg_batch **out) {
     if (!c || !p || !out) return BWG_EINVAL;
     *out = nullptr;
-    int rc = check_mat(p->m, p->mat);
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
     if (rc) return rc;
-    if (p->e_del < 1 || p->e_ins < 1) return BWG_EINVAL;
-    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
-    if (rc) return rc;
-    for (size_t k = 0; k < n; ++k)
-        if (h0[k] <= 0 || q_len[k] < 1) return BWG_EINVAL;
     ExtScoring sc = ext_scoring(*p);
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     std::lock_guard<std::mutex> g(c->run_mu);
     rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, c->ss.main, out);
     if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));  // the caller may free its buffers
     return rc;
+}
+
+int bwg_batch_stage_side(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,
+                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
+                         const uint32_t *t_len, const int32_t *h0, const int32_t *prev, bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    if (max_band_try < 1 || max_band_try > 8 || p->w < 0) return BWG_EINVAL;
+    if (p->w > (INT32_MAX >> (max_band_try - 1))) return BWG_E2BIG;
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (rc) return rc;
+    if (n && !prev) return BWG_EINVAL;
+    ExtScoring sc = ext_scoring(*p);
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    rc = stage_side(c, sc, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev, c->ss.main,
+                    out);
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
+    return rc;
+}
+
+int bwg_batch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
+    if (!c || !b || (!w_out && b->n) || !b->side_tries) return BWG_EINVAL;
+    return fetch_side_bands(c, b, w_out);
 }
 
 int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
@@ -282,6 +324,23 @@ int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_
     if (rc) return rc;
     rc = bwg_batch_run(c, b);
     if (!rc) rc = bwg_batch_results(c, b, out);
+    bwg_batch_destroy(c, b);
+    return rc;
+}
+
+int bwg_extend_side_batch(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,
+                          size_t arena_len, const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
+                          const uint32_t *t_len, const int32_t *h0, const int32_t *prev, bwg_ext_result *out,
+                          int32_t *w_out) {
+    if (n == 0) return (c && p && max_band_try >= 1) ? BWG_OK : BWG_EINVAL;
+    if (!out || !w_out) return BWG_EINVAL;
+    bwg_batch *b = nullptr;
+    int rc = bwg_batch_stage_side(c, p, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev,
+                                  &b);
+    if (rc) return rc;
+    rc = bwg_batch_run(c, b);
+    if (!rc) rc = bwg_batch_results(c, b, out);
+    if (!rc) rc = bwg_batch_side_bands(c, b, w_out);
     bwg_batch_destroy(c, b);
     return rc;
 }

@@ -39,6 +39,9 @@ struct Item {
 };
 
 struct Plan {
+    ExtScoring esc{};      // extension: the scoring this plan's kernels run with (its band w)
+    int retry_round = 0;   // extension: 0, or the band-retry round this plan runs
+    bool compact = false;  // group and thread bins drain compacted queues filled by a stage-0 item
     std::vector<Item> items;
     int n_heads = 0;
     uint32_t *d_order = nullptr;
@@ -221,8 +224,8 @@ std::vector<TuneCandidate> ext_candidates(int cls) {
 // Time each candidate mapping of one class on a sample of the batch's own jobs (already resident
 // on the device) and record the fastest.  The sample kernels write the sampled jobs' true results,
 // which the batch's own launches overwrite with the same values.
-int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::vector<uint32_t> &members,
-                   hipStream_t st) {
+int tune_ext_class(bwg_ctx *c, bwg_batch *b, const ExtScoring &esc, uint64_t sig, int cls,
+                   const std::vector<uint32_t> &members, hipStream_t st) {
     std::lock_guard<std::mutex> g(c->tuner.tune_mu);
     if (c->tuner.lookup(sig, cls, nullptr)) return BWG_OK;
     const uint32_t S = (uint32_t)std::min<size_t>(members.size(), kTuneSample);
@@ -252,10 +255,10 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
             if (e == hipSuccess) {
                 if (cd.kind == kExtGroup)
                     e = launch_ext_group(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S,
-                                         nullptr, d, b->esc, qcap, (bwg_ext_result *)b->d_out, grid);
+                                         nullptr, d, esc, qcap, (bwg_ext_result *)b->d_out, grid);
                 else
                     e = launch_ext_thread(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1,
-                                          S, nullptr, d, b->esc, (bwg_ext_result *)b->d_out, grid);
+                                          S, nullptr, d, esc, (bwg_ext_result *)b->d_out, grid);
             }
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -277,9 +280,13 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
     return BWG_OK;
 }
 
+// Plan an extension batch at the scoring p->esc.  Round 0 may tune and runs the ungapped fast path
+// first; a band-retry round (p->retry_round > 0) bins every job at its wider band, and its first
+// item is the acceptance filter that fills the bins' compacted queues with the jobs sent back.
 int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
-    const ExtScoring &sc = b->esc;
+    const ExtScoring &sc = p->esc;
+    const bool retry = p->retry_round > 0;
     const Tuning &tu = c->tuning;
     std::vector<Item> bins;
     auto find = [&](int kind, int param, bool narrow) {
@@ -306,8 +313,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0), choice_width(n_classes, 16);
     for (int cl = 0; cl < n_classes; ++cl) {
         if (!tu.autotune) break;
-        if (members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
-            const int rc = tune_ext_class(c, b, sig, cl, members[cl], st);
+        if (!retry && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+            const int rc = tune_ext_class(c, b, sc, sig, cl, members[cl], st);
             if (rc) return rc;
         }
         TuneEntry ent;
@@ -366,12 +373,13 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     p->n_heads = std::max<int>(1, (int)bins.size());
     // The reference's ungapped-diagonal fast path (crates/bwa-neon/src/batched.rs:716-798) applies
     // to uniform DNA matrices: run it first over the group and thread bins, which then drain the
-    // compacted queues of the jobs it declines.
+    // compacted queues of the jobs it declines.  A band-retry round drains the queues its
+    // acceptance filter fills instead.
     const int32_t a = sc.mat[0], bmis = -(int32_t)sc.mat[1];
     const int32_t xmax = ungapped_xmax(sc);
-    p->ungapped = tu.ungapped && is_uniform_dna(sc) && a > 0 && bmis > 0 && xmax >= 0 && !bins.empty();
-    if (p->ungapped) {
-        p->ungapped_xmax = xmax;
+    p->ungapped = !retry && tu.ungapped && is_uniform_dna(sc) && a > 0 && bmis > 0 && xmax >= 0 && !bins.empty();
+    p->compact = (p->ungapped || retry) && !bins.empty();
+    if (p->compact) {
         p->n_bins = (int)bins.size();
         // [heads nb][counters, kCountStride apart][slot offsets nb][skip flags nb]
         p->meta.assign((size_t)p->n_bins * (kCountStride + 3), 0u);
@@ -382,15 +390,21 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             p->meta[seg_word(p) + p->n_bins + r] = dp ? 0u : 1u;
             any |= dp;
         }
-        p->ungapped = any && p->n_bins <= 255;
+        if (p->ungapped) p->ungapped = any && p->n_bins <= 255;
+        p->compact = p->ungapped || retry;
     }
-    if (p->ungapped) {
+    if (p->compact) {
         for (auto &it : bins) it.stage = 1;
         Item u;
-        u.kind = kExtUngapped;
         u.stage = 0;
         u.count = (uint32_t)n;
-        u.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cus * 8, ((int64_t)n + 15) / 16));
+        if (p->ungapped) {
+            p->ungapped_xmax = xmax;
+            u.kind = kExtUngapped;
+            u.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cus * 8, ((int64_t)n + 15) / 16));
+        } else {
+            u.kind = kExtRetryFilter;
+        }
         bins.insert(bins.begin(), u);
         p->d_corder = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
         if (!p->d_corder) return BWG_ENOMEM;
@@ -400,7 +414,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     p->d_heads = (uint32_t *)c->pool.alloc(std::max<size_t>((size_t)p->n_heads, p->meta.size()) * 4);
     if (!p->d_order || !p->d_heads) return BWG_ENOMEM;
     hipError_t e = hipMemcpyAsync(p->d_order, p->order.data(), n * 4, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && p->ungapped)
+    if (e == hipSuccess && p->compact)
         e = hipMemcpyAsync(p->d_heads, p->meta.data(), p->meta.size() * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && !p->ext_plain_jobs.empty())
         e = hipMemcpyAsync(p->d_plain_jobs, p->ext_plain_jobs.data(), p->ext_plain_jobs.size() * sizeof(ExtJob),
@@ -622,25 +636,33 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
     switch (it.kind) {
     case kExtGroup:
     case kExtThread: {
-        // after the ungapped pre-pass the bin drains its compacted queue
-        const uint32_t *order = p->ungapped ? p->d_corder + it.offset : p->d_order + it.offset;
-        const uint32_t *dcount = p->ungapped ? p->d_heads + p->n_bins + kCountStride * it.head : nullptr;
+        // after the ungapped pre-pass or a retry filter the bin drains its compacted queue
+        const uint32_t *order = p->compact ? p->d_corder + it.offset : p->d_order + it.offset;
+        const uint32_t *dcount = p->compact ? p->d_heads + p->n_bins + kCountStride * it.head : nullptr;
         if (it.kind == kExtGroup)
             return launch_ext_group(st, it.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order, it.count,
-                                    dcount, p->d_heads + it.head, b->esc, it.qcap, (bwg_ext_result *)b->d_out,
+                                    dcount, p->d_heads + it.head, p->esc, it.qcap, (bwg_ext_result *)b->d_out,
                                     it.grid);
         return launch_ext_thread(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order,
-                                 it.count, dcount, p->d_heads + it.head, b->esc, (bwg_ext_result *)b->d_out, it.grid);
+                                 it.count, dcount, p->d_heads + it.head, p->esc, (bwg_ext_result *)b->d_out, it.grid);
+    }
+    case kExtRetryFilter: {
+        const int r = p->retry_round;
+        const int32_t w_prev = b->esc.w << (r - 1), w_next = b->esc.w << r;
+        return launch_ext_retry_filter(st, p->d_order, it.count, p->d_heads + seg_word(p),
+                                       p->d_heads + seg_word(p) + p->n_bins, p->n_bins,
+                                       (const bwg_ext_result *)b->d_out, b->d_prev0, b->d_prev, b->d_active, r == 1,
+                                       w_prev, w_next, b->d_wout, p->d_corder, p->d_heads + p->n_bins);
     }
     case kExtUngapped:
         return launch_ext_ungapped(st, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order, it.count,
                                    p->d_heads + seg_word(p), p->d_heads + seg_word(p) + p->n_bins, p->n_bins,
-                                   p->d_corder, p->d_heads + p->n_bins, b->esc, p->ungapped_xmax,
+                                   p->d_corder, p->d_heads + p->n_bins, p->esc, p->ungapped_xmax,
                                    (bwg_ext_result *)b->d_out, it.grid);
     case kExtPlain:
         return launch_ext_plain_scatter(st, b->d_arena, b->d_ref, (const ExtJob *)p->d_plain_jobs,
-                                        p->d_order + it.offset, it.count, b->esc, (int2 *)p->d_plain_scratch,
-                                        (bwg_ext_result *)b->d_out);
+                                        p->d_order + it.offset, it.count, p->esc, (int2 *)p->d_plain_scratch,
+                                        (bwg_ext_result *)b->d_out, p->retry_round > 0 ? b->d_active : nullptr);
     case kKswFwd:
     case kKswRev:
         if (it.narrow)
@@ -665,7 +687,7 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
 // concurrently over the streams when enabled.
 hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launches) {
     // queue heads (and, with the ungapped pre-pass, the compacted queue lengths) start at zero
-    hipError_t e = launch_zero_u32(ss.main, p->d_heads, (uint32_t)(p->ungapped ? seg_word(p) : p->n_heads));
+    hipError_t e = launch_zero_u32(ss.main, p->d_heads, (uint32_t)(p->compact ? seg_word(p) : p->n_heads));
     int64_t nl = 1;
     size_t i = 0;
     while (i < p->items.size() && e == hipSuccess) {
@@ -695,17 +717,60 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
     return e;
 }
 
+// Plans of a batch: one, or for a band-retry batch one per round, the round-r plan binning every job
+// at band w << r.
+int plan_batch(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
+    if (b->plan) return BWG_OK;
+    Plan *p = new Plan;
+    b->plan = p;
+    p->esc = b->esc;
+    int rc = b->kind == 0 ? plan_ext(c, b, p, st) : plan_ksw(c, b, p, st);
+    for (int r = 1; r < b->side_tries && !rc; ++r) {
+        Plan *q = new Plan;
+        b->retry_plans.push_back(q);
+        q->esc = b->esc;
+        q->esc.w = b->esc.w << r;
+        q->retry_round = r;
+        rc = plan_ext(c, b, q, st);
+    }
+    return rc;
+}
+
+// The whole launch sequence of a batch: its plan, then each retry round's (filter, then bins).  With
+// a single band try every job keeps the first band.
+hipError_t emit_all(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) {
+    int64_t nl = 0;
+    hipError_t e = emit(c, b, (Plan *)b->plan, ss, &nl);
+    for (void *q : b->retry_plans) {
+        int64_t k = 0;
+        if (e == hipSuccess) e = emit(c, b, (Plan *)q, ss, &k);
+        nl += k;
+    }
+    if (e == hipSuccess && b->side_tries == 1) {
+        e = launch_fill_i32(ss.main, b->d_wout, (uint32_t)b->n, b->esc.w);
+        ++nl;
+    }
+    *launches = nl;
+    return e;
+}
+
+void free_plan(bwg_ctx *c, Plan *p) {
+    c->pool.free(p->d_order);
+    c->pool.free(p->d_heads);
+    c->pool.free(p->d_jobs);
+    c->pool.free(p->d_rowmax);
+    c->pool.free(p->d_plain_jobs);
+    c->pool.free(p->d_plain_scratch);
+    c->pool.free(p->d_corder);
+    delete p;
+}
+
 }  // namespace
 
 int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) {
     *launches = 0;
-    if (!b->plan) {
-        Plan *p = new Plan;
-        b->plan = p;
-        const int rc = b->kind == 0 ? plan_ext(c, b, p, ss.main) : plan_ksw(c, b, p, ss.main);
-        if (rc) return rc;
-    }
-    Plan *p = (Plan *)b->plan;
+    int rc = plan_batch(c, b, ss.main);
+    if (rc) return rc;
     hipError_t e = hipSuccess;
     // Capture the launch sequence once the batch has run eagerly; the eager run is also the
     // reference every later replay is compared with by the tests.
@@ -713,7 +778,7 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
         hipGraph_t g = nullptr;
         e = hipStreamBeginCapture(ss.main, hipStreamCaptureModeThreadLocal);
         int64_t nl = 0;
-        const hipError_t e2 = e == hipSuccess ? emit(c, b, p, ss, &nl) : e;
+        const hipError_t e2 = e == hipSuccess ? emit_all(c, b, ss, &nl) : e;
         if (e == hipSuccess) e = hipStreamEndCapture(ss.main, &g);
         if (e == hipSuccess && e2 != hipSuccess) e = e2;
         if (e == hipSuccess) e = hipGraphInstantiate(&b->graph, g, nullptr, nullptr, 0);
@@ -724,9 +789,10 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     if (e == hipSuccess) {
         if (b->graph) {
             e = hipGraphLaunch(b->graph, ss.main);
-            *launches = (int64_t)p->items.size() + 1;
+            *launches = b->graph_launches;
         } else {
-            e = emit(c, b, p, ss, launches);
+            e = emit_all(c, b, ss, launches);
+            b->graph_launches = *launches;
         }
     }
     if (e == hipSuccess) e = hipEventRecord(b->ev_end, ss.main);
@@ -759,16 +825,10 @@ void release_batched(bwg_ctx *c, bwg_batch *b) {
         (void)hipGraphExecDestroy(b->graph);
         b->graph = nullptr;
     }
+    for (void *q : b->retry_plans) free_plan(c, (Plan *)q);
+    b->retry_plans.clear();
     if (!b->plan) return;
-    Plan *p = (Plan *)b->plan;
-    c->pool.free(p->d_order);
-    c->pool.free(p->d_heads);
-    c->pool.free(p->d_jobs);
-    c->pool.free(p->d_rowmax);
-    c->pool.free(p->d_plain_jobs);
-    c->pool.free(p->d_plain_scratch);
-    c->pool.free(p->d_corder);
-    delete p;
+    free_plan(c, (Plan *)b->plan);
     b->plan = nullptr;
 }
 

@@ -31,6 +31,19 @@ hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8
                                const uint32_t *skip, int n_bins, uint32_t *corder, uint32_t *ccount,
                                const ExtScoring &sc, int32_t xmax, bwg_ext_result *out, int min_blocks);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n);
+hipError_t launch_fill_i32(hipStream_t st, int32_t *p, uint32_t n, int32_t v);
+// Band-retry acceptance between two rounds of a one-sided extension batch (the reference's
+// run_side, crates/bwa-mem/src/across.rs:1365-1460).  Over the next round's order array (bins
+// delimited by seg_off): each job still active takes the acceptance test on its result at band
+// w_prev against its previous score (prev_in in the first filter, prev_out after); an accepted job
+// is deactivated, a rejected one keeps its score in prev_out, gets w_next in w_out and is appended to
+// its bin's compacted queue unless the bin has skip set (those bins read `active` themselves).  The
+// first filter treats every job as active and writes w_out for all of them.
+hipError_t launch_ext_retry_filter(hipStream_t st, const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
+                                   const uint32_t *skip, int n_bins, const bwg_ext_result *out,
+                                   const int32_t *prev_in, int32_t *prev_out, uint8_t *active, bool first,
+                                   int32_t w_prev, int32_t w_next, int32_t *w_out, uint32_t *corder,
+                                   uint32_t *ccount);
 
 int ksw_group_blocks_per_cu(int K, bool rev);
 hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,

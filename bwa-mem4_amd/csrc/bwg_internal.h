@@ -142,10 +142,11 @@ hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const uint8_t 
 hipError_t launch_ksw_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                             uint32_t n, const KswScoring &sc, uint8_t *scratch, bwg_ksw_result *out);
 
-// Same kernels over a compacted job list whose k-th result goes to out[out_index[k]].
+// Same kernels over a compacted job list whose k-th result goes to out[out_index[k]]; with
+// `active` set, only jobs whose flag active[out_index[k]] is non-zero run.
 hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const ExtScoring &sc,
-                                    int2 *scratch, bwg_ext_result *out);
+                                    int2 *scratch, bwg_ext_result *out, const uint8_t *active = nullptr);
 hipError_t launch_ksw_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const KswJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const KswScoring &sc,
                                     uint8_t *scratch, bwg_ksw_result *out);

@@ -161,6 +161,37 @@ int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, 
     return BWG_OK;
 }
 
+int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint8_t *arena, size_t arena_len,
+               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+               const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out) {
+    bwg_batch *b = nullptr;
+    int rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, &b);
+    if (rc) return rc;
+    b->side_tries = tries;
+    const size_t words = std::max<size_t>(n, 1) * 4;
+    b->d_prev0 = (int32_t *)c->pool.alloc(words);
+    b->d_prev = (int32_t *)c->pool.alloc(words);
+    b->d_wout = (int32_t *)c->pool.alloc(words);
+    b->d_active = (uint8_t *)c->pool.alloc(std::max<size_t>(n, 1));
+    if (!b->d_prev0 || !b->d_prev || !b->d_wout || !b->d_active) rc = BWG_ENOMEM;
+    if (!rc && n) rc = bwg_map_hip(hipMemcpyAsync(b->d_prev0, prev, n * 4, hipMemcpyHostToDevice, st));
+    if (rc) {
+        destroy_batch(c, b);
+        return rc;
+    }
+    *out = b;
+    return BWG_OK;
+}
+
+int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
+    if (b->n == 0) return BWG_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    hipError_t e = hipMemcpyAsync(w_out, b->d_wout, b->n * 4, hipMemcpyDeviceToHost, c->ss.main);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    return bwg_map_hip(e);
+}
+
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out) {
@@ -251,7 +282,8 @@ int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
 
 int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
     if (b->n == 0) return BWG_OK;
-    if (form == BWG_FORM_PLAIN) {
+    // band-retry batches keep their rounds on the device, which only the batched form does
+    if (form == BWG_FORM_PLAIN && !b->side_tries) {
         int rc = prepare_plain(c, b, ss.main);
         if (rc) return rc;
         hipError_t e = hipEventRecord(b->ev_start, ss.main);
@@ -315,6 +347,10 @@ void destroy_batch(bwg_ctx *c, bwg_batch *b) {
     c->pool.free(b->d_out);
     c->pool.free(b->d_plain_scratch);
     c->pool.free(b->d_plain_jobs);
+    c->pool.free(b->d_prev0);
+    c->pool.free(b->d_prev);
+    c->pool.free(b->d_wout);
+    c->pool.free(b->d_active);
     release_batched(c, b);
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
     if (b->ev_end) (void)hipEventDestroy(b->ev_end);

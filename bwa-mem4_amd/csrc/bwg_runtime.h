@@ -94,9 +94,16 @@ struct bwg_batch {
     // plain form state, created on first plain run
     void *d_plain_scratch = nullptr;
     void *d_plain_jobs = nullptr;
+    // band-retry extension (bwg_batch_stage_side): band tries, the callers' previous scores, and
+    // per-job device state between rounds
+    int side_tries = 0;
+    int32_t *d_prev0 = nullptr, *d_prev = nullptr, *d_wout = nullptr;
+    uint8_t *d_active = nullptr;
     // batched form state (bwg_batched.cpp)
     void *plan = nullptr;
+    std::vector<void *> retry_plans;  // band-retry rounds 1.. of a side batch
     hipGraphExec_t graph = nullptr;
+    int64_t graph_launches = 0;
     int batched_runs = 0;
     // timing of the last run
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
@@ -112,6 +119,12 @@ int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, 
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
               bwg_batch **out);
+// A one-sided extension batch with the reference's band doubling: up to `tries` rounds at bands
+// sc.w << r; prev holds each job's score before the first round.
+int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint8_t *arena, size_t arena_len,
+               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+               const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out);
+int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out);
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out);

@@ -14,7 +14,8 @@
 namespace bwg {
 
 // Kernel mappings a bin can be run with.
-enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain, kExtUngapped };
+enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain, kExtUngapped,
+                kExtRetryFilter };
 
 struct TuneCandidate {
     int kind = 0;

@@ -548,6 +548,54 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
     }
 }
 
+// ---- band-retry acceptance ------------------------------------------------------------------
+// One thread per slot of the next round's order array.  The acceptance test is the reference's
+// (crates/bwa-mem/src/across.rs:1444-1445; lib.rs:360): the score did not change from the previous
+// band, or the best cell's diagonal offset stayed under (w>>1) + (w>>2).  Rejected jobs are appended
+// to their bins' compacted queues with one atomic per bin and wavefront: slots are sorted by bin, so
+// a wavefront's rejected lanes fall in a few bins, and the leader of each reserves their slots.
+__global__ __launch_bounds__(256) void ext_retry_filter_kernel(
+    const uint32_t *__restrict__ order, uint32_t n_slots, const uint32_t *__restrict__ seg_off,
+    const uint32_t *__restrict__ skip, int n_bins, const bwg_ext_result *__restrict__ out,
+    const int32_t *__restrict__ prev_in, int32_t *__restrict__ prev_out, uint8_t *__restrict__ active, int first,
+    int32_t w_prev, int32_t w_next, int32_t *__restrict__ w_out, uint32_t *__restrict__ corder,
+    uint32_t *__restrict__ ccount) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    bool queue = false;
+    uint32_t idx = 0;
+    int bin = 0;
+    if (slot < n_slots) {
+        while (bin + 1 < n_bins && slot >= seg_off[bin + 1]) ++bin;
+        idx = order[slot];
+        if (first || active[idx]) {
+            const bwg_ext_result r = out[idx];
+            const int32_t prev = first ? prev_in[idx] : prev_out[idx];
+            const bool accept = r.score == prev || r.max_off < (w_prev >> 1) + (w_prev >> 2);
+            active[idx] = accept ? 0 : 1;
+            if (!accept) {
+                prev_out[idx] = r.score;
+                w_out[idx] = w_next;
+                queue = !skip[bin];
+            } else if (first) {
+                w_out[idx] = w_prev;
+            }
+        }
+    }
+    uint64_t pending = __ballot(queue);
+    while (pending) {
+        const int leader = __ffsll((unsigned long long)pending) - 1;
+        const int lb = __shfl(bin, leader, 64);
+        const uint64_t mine = __ballot(queue && bin == lb);
+        uint32_t pos = 0;
+        if (lane == leader) pos = atomicAdd(&ccount[kCountStride * lb], (uint32_t)__popcll(mine));
+        pos = __shfl(pos, leader, 64);
+        if (queue && bin == lb)
+            corder[seg_off[lb] + pos + (uint32_t)__popcll(mine & ((1ull << lane) - 1ull))] = idx;
+        pending &= ~mine;
+    }
+}
+
 template <int K>
 hipError_t launch_group(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs, const uint32_t *order,
                         uint32_t count, const uint32_t *dcount, uint32_t *head, const ExtScoring &sc, int qcap,
@@ -609,6 +657,18 @@ hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const u
 #undef BWG_K
     default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_ext_retry_filter(hipStream_t st, const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
+                                   const uint32_t *skip, int n_bins, const bwg_ext_result *out,
+                                   const int32_t *prev_in, int32_t *prev_out, uint8_t *active, bool first,
+                                   int32_t w_prev, int32_t w_next, int32_t *w_out, uint32_t *corder,
+                                   uint32_t *ccount) {
+    if (n_slots == 0) return hipSuccess;
+    hipLaunchKernelGGL(ext_retry_filter_kernel, dim3((n_slots + 255) / 256), dim3(256), 0, st, order, n_slots,
+                       seg_off, skip, n_bins, out, prev_in, prev_out, active, first ? 1 : 0, w_prev, w_next, w_out,
+                       corder, ccount);
+    return hipGetLastError();
 }
 
 hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,

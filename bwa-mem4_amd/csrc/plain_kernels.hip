@@ -16,12 +16,15 @@ __global__ __launch_bounds__(256) void ext_plain_kernel(const uint8_t *__restric
                                                         uint32_t n, ExtScoring sc,
                                                         int2 *__restrict__ scratch,
                                                         const uint32_t *__restrict__ out_index,
+                                                        const uint8_t *__restrict__ active,
                                                         bwg_ext_result *__restrict__ out) {
     __shared__ int8_t smat[32];
     if (threadIdx.x < 25) smat[threadIdx.x] = sc.mat[threadIdx.x];
     __syncthreads();
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
+    // a band-retry round runs only the jobs its acceptance filter sent back
+    if (active && !active[out_index ? out_index[k] : k]) return;
     const ExtJob J = jobs[k];
     const uint8_t *q = arena + J.q_off;
     const TargetView t = target_view(arena, ref, J.t_off, J.tlen);
@@ -239,6 +242,11 @@ __global__ void zero_u32_kernel(uint32_t *p, uint32_t n) {
     if (k < n) p[k] = 0;
 }
 
+__global__ void fill_i32_kernel(int32_t *p, uint32_t n, int32_t v) {
+    const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (k < n) p[k] = v;
+}
+
 }  // namespace
 
 hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
@@ -246,17 +254,23 @@ hipError_t launch_ext_plain(hipStream_t st, const uint8_t *arena, const uint8_t 
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
     hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, ref, jobs, n,
-                       sc, scratch, (const uint32_t *)nullptr, out);
+                       sc, scratch, (const uint32_t *)nullptr, (const uint8_t *)nullptr, out);
     return hipGetLastError();
 }
 
 hipError_t launch_ext_plain_scatter(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                     const uint32_t *out_index, uint32_t n, const ExtScoring &sc,
-                                    int2 *scratch, bwg_ext_result *out) {
+                                    int2 *scratch, bwg_ext_result *out, const uint8_t *active) {
     if (n == 0) return hipSuccess;
     const uint32_t bs = 256;
     hipLaunchKernelGGL(ext_plain_kernel, dim3((n + bs - 1) / bs), dim3(bs), 0, st, arena, ref, jobs, n,
-                       sc, scratch, out_index, out);
+                       sc, scratch, out_index, active, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_i32(hipStream_t st, int32_t *p, uint32_t n, int32_t v) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_i32_kernel, dim3((n + 255) / 256), dim3(256), 0, st, p, n, v);
     return hipGetLastError();
 }
 

@@ -152,6 +152,30 @@ void bwg_batch_destroy(bwg_ctx *ctx, bwg_batch *b);
  * batched.rs:716-798); 0 for rescue batches, the plain form, or with BWG_OPT_UNGAPPED off. */
 int bwg_batch_ungapped_hits(bwg_ctx *ctx, bwg_batch *b, uint64_t *hits);
 
+/* ---- one-sided extension with band doubling ------------------------------------------------------
+ * Replaces the reference's batched band-retry driver run_side (crates/bwa-mem/src/across.rs:1357-1460),
+ * whose per-read twin is extend_side (crates/bwa-mem/src/lib.rs:332-373): round r extends the jobs
+ * still pending at band p->w << r (p->end_bonus is the side's clipping penalty), and a job is
+ * accepted when its score equals its previous score, when its best cell's diagonal offset max_off is
+ * below (w >> 1) + (w >> 2), or in the last of max_band_try rounds (bwa: 2).  prev[k] is the
+ * job's score before the first round (-1 for a left extension; the region's score for a right one,
+ * as the reference's across.rs:548-556 describes).  All rounds run on the device in one launch
+ * sequence: the acceptance test between rounds compacts the rejected jobs into the next round's
+ * bins, with no host round trip.  out[k] is the accepted round's result and w_out[k] its band, the
+ * value the caller folds into the region's w.  The region bookkeeping (local versus to-end choice,
+ * across.rs:1446-1512) stays with the caller.  Side batches always run the batched engine. */
+int bwg_extend_side_batch(bwg_ctx *ctx, const bwg_ext_params *p, int max_band_try, size_t n,
+                          const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
+                          const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                          const int32_t *h0, const int32_t *prev, bwg_ext_result *out, int32_t *w_out);
+/* Staged form of bwg_extend_side_batch: run with bwg_batch_run, results with bwg_batch_results,
+ * accepted bands with bwg_batch_side_bands. */
+int bwg_batch_stage_side(bwg_ctx *ctx, const bwg_ext_params *p, int max_band_try, size_t n,
+                         const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
+                         const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                         const int32_t *h0, const int32_t *prev, bwg_batch **out);
+int bwg_batch_side_bands(bwg_ctx *ctx, bwg_batch *b, int32_t *w_out);
+
 /* ---- serving ----------------------------------------------------------------------------------
  * Continuous batching of alignment work from concurrent producers.  No counterpart in the
  * reference: its callers run extension and rescue per rayon chunk

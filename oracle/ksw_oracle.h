@@ -16,6 +16,8 @@
  *                          lane rule crates/bwa-mem/src/pe.rs:420-424
  *   orc_batched_align2_job crates/bwa-neon/src/matesw.rs:299-497 (batched_ksw_align2, per job)
  *   orc_ungapped_*         crates/bwa-neon/src/batched.rs:493-521, 602-697, 756-764 (ksw_ungapped.c)
+ *   orc_extend_side        crates/bwa-mem/src/lib.rs:332-373     (extend_side; ksw_side.c)
+ *   orc_run_side           crates/bwa-mem/src/across.rs:1357-1460 (run_side; ksw_side.c)
  *
  * A second, independent restatement in the lockstep (SoA, many jobs per pass) form lives in
  * ksw_lockstep.c, and brute-force checkers in ksw_brute.c.
@@ -133,6 +135,18 @@ void orc_ungapped_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, c
                         const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, int32_t m,
                         const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
                         int32_t zdrop, orc_ext_result *out, uint8_t *hit);
+
+/* Band doubling around ksw_extend2 (ksw_side.c): one job's tries, and the batched rounds.  w_out
+ * receives the accepted band. */
+orc_ext_result orc_extend_side(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target,
+                               int32_t m, const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins,
+                               int32_t e_ins, int32_t w, int32_t end_bonus, int32_t zdrop, int32_t h0,
+                               int32_t prev0, int32_t max_band_try, int32_t *w_out);
+void orc_run_side(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                  const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, const int32_t *prev0,
+                  int32_t m, const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                  int32_t w, int32_t end_bonus, int32_t zdrop, int32_t max_band_try, orc_ext_result *out,
+                  int32_t *w_out, int nthreads);
 
 #ifdef __cplusplus
 }

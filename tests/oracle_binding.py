@@ -97,6 +97,11 @@ def lib():
                                          _P, _P]
         L.orc_ungapped_xmax.restype = _I
         L.orc_ungapped_xmax.argtypes = [_P, _I, _I, _I, _I]
+        L.orc_extend_side.restype = ExtResult
+        L.orc_extend_side.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_run_side.restype = None
+        L.orc_run_side.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I,
+                                   _I, _P, _P, C.c_int]
         _lib = L
     return _lib
 
@@ -129,6 +134,31 @@ def extend2(q, t, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0, want
     r = lib().orc_ksw_extend2(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins, w,
                               end_bonus, zdrop, h0, C.addressof(cells))
     return (r.tup(), cells.value) if want_cells else r.tup()
+
+
+def extend_side(q, t, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, h0, prev0, max_band_try=2):
+    """One job's band tries (extend_side): (result tuple, accepted band)."""
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    wo = C.c_int32(0)
+    r = lib().orc_extend_side(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop,
+                              h0, prev0, max_band_try, C.addressof(wo))
+    return r.tup(), wo.value
+
+
+def run_side(batch, prev, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, max_band_try=2, nthreads=8):
+    """The batched band-retry rounds (run_side): (results, accepted band per job)."""
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=EXT_DTYPE)
+    w_out = np.zeros(n, dtype=np.int32)
+    prev = np.ascontiguousarray(prev, dtype=np.int32)
+    ma, mp = _mat(mat)
+    lib().orc_run_side(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                       batch.t_off.ctypes.data, batch.t_len.ctypes.data, batch.h0.ctypes.data, prev.ctypes.data,
+                       5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, max_band_try, out.ctypes.data,
+                       w_out.ctypes.data, nthreads)
+    return out, w_out
 
 
 def local_fwd(q, t, mat, o_del, e_del, o_ins, e_ins, minsc, endsc, max_sc, lanes):
