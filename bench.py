@@ -94,7 +94,9 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
-    ap.add_argument("--only", default="ext,small,resc,side,e2e", help="comma list of workloads to time")
+    ap.add_argument("--only", default="ext,small,resc,side,matesw,e2e", help="comma list of workloads to time")
+    ap.add_argument("--matesw-anchors", type=int, default=200_000,
+                    help="anchors of the mate-rescue round (4 orientation jobs each)")
     ap.add_argument("--serve-rates", default="5e4,2e5,2e6,1e7,3e7,6e7",
                     help="offered loads (jobs/s) of the serving sweep; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
@@ -209,6 +211,37 @@ def main():
             side[w] = dict(step_s=per, kernel_s=kern, retried=int((bands != w).sum()))
             log("side batch w=%d: %.3f ms/step (kernels %.3f ms), %d jobs retried at w=%d"
                 % (w, per * 1e3, kern * 1e3, side[w]["retried"], 2 * w))
+    matesw = None
+    if "matesw" in only and args.matesw_anchors:
+        # one mate-rescue round with device-resident windows: a 64 Mbp synthetic genome in 8
+        # contigs, one anchor per read pair (150 bp reads, insert N(550,150), 30% unplaced mates)
+        ctx.set_form(bwg.FORM_BATCHED)
+        log("mate rescue: generating %d anchors on a 64 Mbp genome" % args.matesw_anchors)
+        rnd = bwg.generate_rescue_round(args.matesw_anchors, genome_len=64_000_000, n_contigs=8, seed=43)
+        ctx.set_reference(rnd.fwd, rnd.contigs)
+        msw_p = bwg.ksw_params(5, mat, 6, 1, 6, 1, rnd.min_seed_len, 1)  # minsc = min_seed_len * a
+        st = ctx.stage_matesw(rnd, msw_p)
+        st.run()
+        per, kern, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+        win, out = st.windows(), st.results()
+        st.close()
+        valid = (win["flags"] & bwg.WIN_VALID) != 0
+        cells = float(np.sum((win["re"] - win["rb"])[valid].astype(np.int64) *
+                             rnd.anchors["l_ms"].repeat(4)[valid].astype(np.int64)))
+        matesw = dict(step_s=per, kernel_s=kern, jobs=int(valid.sum()), cells=cells)
+        log("mate rescue: %.3f ms/step (kernels %.3f ms), %d windows aligned, %.0f G cells/s"
+            % (per * 1e3, kern * 1e3, matesw["jobs"], cells / per / 1e9))
+        if not args.no_check:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_binding as ob
+            sub_n = min(10_000, args.matesw_anchors)
+            part = bwg.RescueRound(rnd.fwd, rnd.contigs, rnd.pes, rnd.anchors[:sub_n], rnd.ma_rb[:rnd.n_ma],
+                                   rnd.arena, rnd.min_seed_len)
+            want_w, want = ob.matesw_batch(part, mat, 6, 1, 6, 1, rnd.min_seed_len, 1, nthreads=args.cpu_threads)
+            ok = np.array_equal(want_w, win[:4 * sub_n]) and np.array_equal(want, out[:4 * sub_n])
+            log("mate rescue: first %d anchors bit-exact vs oracle: %s" % (sub_n, ok))
+            if not ok:
+                sys.exit(1)
     e2e = None
     if "e2e" in only and "ext" in only:
         # PCIe-inclusive: the synchronous entry point from host buffers (stage, run, copy back)
@@ -312,6 +345,10 @@ def main():
         line["speedup_vs_plain"] = jobs_s / line["plain_jobs_per_s"]
     if e2e:
         line["e2e_jobs_per_s"] = e2e
+    if matesw:
+        line["matesw_anchors_per_s"] = args.matesw_anchors / matesw["step_s"]
+        line["matesw_jobs_per_s"] = matesw["jobs"] / matesw["step_s"]
+        line["matesw_gcell_s"] = matesw["cells"] / matesw["step_s"] / 1e9
     for w, r in side.items():
         line["side_w%d_jobs_per_s" % w] = args.jobs / r["step_s"]
         line["side_w%d_retried" % w] = r["retried"]

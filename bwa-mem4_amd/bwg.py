@@ -44,7 +44,16 @@ EXPORTED_SYMBOLS = (
     "bwg_server_create", "bwg_server_set_reference", "bwg_server_submit", "bwg_server_poll",
     "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy", "bwg_hbm_copy", "bwg_tune_report",
     "bwg_batch_ungapped_hits", "bwg_extend_side_batch", "bwg_batch_stage_side", "bwg_batch_side_bands",
+    "bwg_set_reference", "bwg_matesw_batch", "bwg_batch_stage_matesw", "bwg_batch_windows",
 )
+
+# bwg_mate_anchor, bwg_rescue_window, bwg_pe_orient and bwg_contig as numpy records
+ANCHOR_DTYPE = np.dtype([("rb", "<i8"), ("rid", "<i4"), ("l_ms", "<u4"), ("ms_off", "<u8"),
+                         ("ma_begin", "<u8"), ("ma_end", "<u8")])
+WINDOW_DTYPE = np.dtype([("rb", "<i8"), ("re", "<i8"), ("rid", "<i4"), ("flags", "<i4")])
+PE_DTYPE = np.dtype([("low", "<i4"), ("high", "<i4"), ("failed", "<i4")])
+CONTIG_DTYPE = np.dtype([("offset", "<i8"), ("len", "<i4"), ("_pad", "<i4")])
+WIN_VALID, WIN_REV, WIN_SKIPPED = 1, 2, 4
 
 
 class BwgError(RuntimeError):
@@ -117,6 +126,16 @@ def _lib():
                                            _P]
         L.bwg_batch_side_bands.restype = C.c_int
         L.bwg_batch_side_bands.argtypes = [_P, _P, _P]
+        L.bwg_set_reference.restype = C.c_int
+        L.bwg_set_reference.argtypes = [_P, _P, C.c_int64, _P, C.c_int32]
+        L.bwg_matesw_batch.restype = C.c_int
+        L.bwg_matesw_batch.argtypes = [_P, _P, C.c_int32, _P, C.c_size_t, _P, _P, C.c_size_t, _P, C.c_size_t,
+                                       _P, _P]
+        L.bwg_batch_stage_matesw.restype = C.c_int
+        L.bwg_batch_stage_matesw.argtypes = [_P, _P, C.c_int32, _P, C.c_size_t, _P, _P, C.c_size_t, _P,
+                                             C.c_size_t, _P]
+        L.bwg_batch_windows.restype = C.c_int
+        L.bwg_batch_windows.argtypes = [_P, _P, _P]
         _L = L
     return _L
 
@@ -272,6 +291,34 @@ class Context:
                                             w_out.ctypes.data), "bwg_extend_side_batch")
         return out, w_out
 
+    def set_reference(self, fwd, contigs):
+        """Upload a reference (forward strand codes, one byte per base) and its contigs
+        (CONTIG_DTYPE records, ascending offsets) for mate rescue."""
+        fwd = np.ascontiguousarray(fwd, dtype=np.uint8)
+        contigs = np.ascontiguousarray(contigs, dtype=CONTIG_DTYPE)
+        _check(_lib().bwg_set_reference(self.h, fwd.ctypes.data, len(fwd), contigs.ctypes.data, len(contigs)),
+               "bwg_set_reference")
+
+    def matesw_arrays(self, rnd, params):
+        """One mate-rescue round (RescueRound) on the device: (windows, results), 4 per anchor."""
+        n = len(rnd.anchors)
+        win = np.zeros(4 * n, dtype=WINDOW_DTYPE)
+        out = np.zeros(4 * n, dtype=KSW_DTYPE)
+        _check(_lib().bwg_matesw_batch(self.h, C.byref(params), int(rnd.min_seed_len), rnd.pes.ctypes.data, n,
+                                       rnd.anchors.ctypes.data, rnd.ma_rb.ctypes.data, rnd.n_ma,
+                                       rnd.arena.ctypes.data, len(rnd.arena), win.ctypes.data, out.ctypes.data),
+               "bwg_matesw_batch")
+        return win, out
+
+    def stage_matesw(self, rnd, params):
+        h = C.c_void_p()
+        n = len(rnd.anchors)
+        _check(_lib().bwg_batch_stage_matesw(self.h, C.byref(params), int(rnd.min_seed_len), rnd.pes.ctypes.data,
+                                             n, rnd.anchors.ctypes.data, rnd.ma_rb.ctypes.data, rnd.n_ma,
+                                             rnd.arena.ctypes.data, len(rnd.arena), C.byref(h)),
+               "bwg_batch_stage_matesw")
+        return StagedBatch(self, h, 4 * n, KSW_DTYPE)
+
     def stage_side(self, batch, params, prev, max_band_try=2):
         """Stage a band-retry side batch; run it with StagedBatch.run, bands with .side_bands()."""
         h = C.c_void_p()
@@ -306,6 +353,12 @@ class StagedBatch:
         out = np.zeros(self.n, dtype=self.dtype)
         _check(_lib().bwg_batch_results(self.ctx.h, self.h, out.ctypes.data), "bwg_batch_results")
         return out
+
+    def windows(self):
+        """Rescue windows of a mate-rescue batch's last run (4 per anchor)."""
+        w = np.zeros(self.n, dtype=WINDOW_DTYPE)
+        _check(_lib().bwg_batch_windows(self.ctx.h, self.h, w.ctypes.data), "bwg_batch_windows")
+        return w
 
     def side_bands(self):
         """Accepted band of every job of a side batch's last run."""
@@ -424,6 +477,90 @@ def generate(kind, n, seed=42):
     finally:
         W.bwgw_free(h)
     return JobBatch(arena, q_off, q_len, t_off, t_len, h0, kinds)
+
+
+class RescueRound:
+    """One mate-rescue round's inputs: the reference (forward codes and contigs), insert windows per
+    orientation, and anchors whose mates sit in `arena`, with the mates' existing regions ma_rb."""
+
+    def __init__(self, fwd, contigs, pes, anchors, ma_rb, arena, min_seed_len):
+        self.fwd = np.ascontiguousarray(fwd, dtype=np.uint8)
+        self.contigs = np.ascontiguousarray(contigs, dtype=CONTIG_DTYPE)
+        self.pes = np.ascontiguousarray(pes, dtype=PE_DTYPE)
+        self.anchors = np.ascontiguousarray(anchors, dtype=ANCHOR_DTYPE)
+        self.ma_rb = np.ascontiguousarray(ma_rb if len(ma_rb) else np.zeros(1), dtype=np.int64)[:max(1, len(ma_rb))]
+        self.n_ma = len(ma_rb)
+        self.arena = np.ascontiguousarray(arena, dtype=np.uint8)
+        self.min_seed_len = int(min_seed_len)
+
+    @property
+    def l_pac(self):
+        return len(self.fwd)
+
+
+def _revcomp(s):
+    s = np.asarray(s, dtype=np.uint8)[::-1]
+    return np.where(s < 4, 3 - s, s).astype(np.uint8)
+
+
+def generate_rescue_round(n_pairs, genome_len=2_000_000, n_contigs=4, read_len=150, seed=42,
+                          insert_mean=550, insert_sd=150, random_frac=0.3, sub_rate=0.01):
+    """A synthetic rescue round: a random genome cut into contigs, read pairs drawn as FR fragments
+    (1% substitutions), and one anchor per pair, read 1's region on the forward strand or read 2's
+    on the reverse half, with the other read as the mate; `random_frac` of the mates are random
+    sequence.  Orientation windows follow bwa's FR-dominated statistics (RF and FF/RR failed except
+    for some rounds that enable them); a few anchors carry a mate region already at a proper
+    distance, which makes matesw_collect skip that orientation."""
+    rng = np.random.default_rng(seed)
+    fwd = rng.integers(0, 4, genome_len).astype(np.uint8)
+    cuts = np.sort(rng.choice(np.arange(1, genome_len), n_contigs - 1, replace=False))
+    starts = np.concatenate([[0], cuts])
+    lens = np.diff(np.concatenate([starts, [genome_len]]))
+    contigs = np.zeros(n_contigs, dtype=CONTIG_DTYPE)
+    contigs["offset"] = starts
+    contigs["len"] = lens
+    l2 = 2 * genome_len
+    pes = np.zeros(4, dtype=PE_DTYPE)
+    lo, hi = max(1, insert_mean - 4 * insert_sd), insert_mean + 4 * insert_sd
+    pes[:] = (lo, hi, 1)
+    pes[1] = (lo, hi, 0)                      # FR
+    if seed % 2:
+        pes[2] = (max(1, lo // 2), hi // 2, 0)  # RF too, in some rounds
+    n, l = n_pairs, read_len
+    c = rng.integers(0, n_contigs, n)
+    isize = np.clip(np.rint(rng.normal(insert_mean, insert_sd, n)), l, 4 * insert_mean).astype(np.int64)
+    c = np.where(lens[c] <= isize + 2, int(np.argmax(lens)), c)
+    x = starts[c] + (rng.random(n) * np.maximum(1, lens[c] - isize)).astype(np.int64)
+    cols = np.arange(l)
+    r1 = fwd[x[:, None] + cols]
+    g2 = fwd[(x + isize - l)[:, None] + cols][:, ::-1]
+    r2 = np.where(g2 < 4, 3 - g2, g2).astype(np.uint8)
+    for r in (r1, r2):
+        m = rng.random(r.shape) < sub_rate
+        r[m] = (r[m] + rng.integers(1, 4, int(m.sum()))) % 4
+    first = rng.random(n) < 0.5  # anchor read 1 (forward strand), else read 2 on the reverse half
+    rb = np.where(first, x, l2 - (x + isize))
+    mates = np.where(first[:, None], r2, r1)
+    unplaced = rng.random(n) < random_frac
+    mates[unplaced] = rng.integers(0, 4, (int(unplaced.sum()), l))
+    # existing mate regions: one at a proper FR distance (5%), one unrelated (5%)
+    paired = rng.random(n) < 0.05
+    other = rng.random(n) < 0.05
+    cnt = paired.astype(np.int64) + other
+    ma_end = np.cumsum(cnt)
+    ma_begin = ma_end - cnt
+    ma_rb = np.zeros(int(cnt.sum()), dtype=np.int64)
+    ma_rb[ma_begin[paired]] = np.where(first, l2 - (x + isize), x)[paired]
+    ma_rb[(ma_begin + paired)[other]] = rng.integers(0, l2, int(other.sum()))
+    anchors = np.zeros(n, dtype=ANCHOR_DTYPE)
+    anchors["rb"] = rb
+    anchors["rid"] = c
+    anchors["l_ms"] = l
+    anchors["ms_off"] = np.arange(n, dtype=np.uint64) * l
+    anchors["ma_begin"] = ma_begin
+    anchors["ma_end"] = ma_end
+    arena = np.ascontiguousarray(mates.reshape(-1), dtype=np.uint8)
+    return RescueRound(fwd, contigs, pes, anchors, np.asarray(ma_rb, dtype=np.int64), arena, 19)
 
 
 def default_mat(a=1, b=4):

@@ -100,6 +100,9 @@ void bwg_destroy(bwg_ctx *c) {
     (void)hipSetDevice(c->device);
     c->ss.destroy();
     c->pool.release_all();
+    if (c->d_ref2) (void)hipFree(c->d_ref2);
+    if (c->d_contig_off) (void)hipFree(c->d_contig_off);
+    if (c->d_contig_len) (void)hipFree(c->d_contig_len);
     delete c;
 }
 
@@ -272,6 +275,114 @@ int bwg_batch_stage_side(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, 
 int bwg_batch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
     if (!c || !b || (!w_out && b->n) || !b->side_tries) return BWG_EINVAL;
     return fetch_side_bands(c, b, w_out);
+}
+
+int bwg_set_reference(bwg_ctx *c, const uint8_t *fwd, int64_t l_pac, const bwg_contig *contigs,
+                      int32_t n_contigs) {
+    if (!c || l_pac <= 0 || !fwd || !contigs || n_contigs < 1) return BWG_EINVAL;
+    for (int64_t k = 0; k < l_pac; ++k)
+        if (fwd[k] > 4) return BWG_EINVAL;
+    for (int32_t r = 0; r < n_contigs; ++r) {
+        if (contigs[r].offset < 0 || contigs[r].len < 0 || contigs[r].offset + contigs[r].len > l_pac)
+            return BWG_EINVAL;
+        if (r && contigs[r].offset < contigs[r - 1].offset) return BWG_EINVAL;
+    }
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    if (c->ref_users.load()) return BWG_EAGAIN;  // staged rescue batches still read the old one
+    if (c->d_ref2) (void)hipFree(c->d_ref2);
+    if (c->d_contig_off) (void)hipFree(c->d_contig_off);
+    if (c->d_contig_len) (void)hipFree(c->d_contig_len);
+    c->d_ref2 = nullptr;
+    c->d_contig_off = nullptr;
+    c->d_contig_len = nullptr;
+    c->l_pac = 0;
+    c->n_contigs = 0;
+    std::vector<int64_t> off(n_contigs);
+    std::vector<int32_t> len(n_contigs);
+    for (int32_t r = 0; r < n_contigs; ++r) {
+        off[r] = contigs[r].offset;
+        len[r] = contigs[r].len;
+    }
+    hipError_t e = hipMalloc(&c->d_ref2, (size_t)(2 * l_pac) + 64);
+    if (e == hipSuccess) e = hipMalloc(&c->d_contig_off, (size_t)n_contigs * 8);
+    if (e == hipSuccess) e = hipMalloc(&c->d_contig_len, (size_t)n_contigs * 4);
+    if (e == hipSuccess) e = hipMemcpyAsync(c->d_ref2, fwd, (size_t)l_pac, hipMemcpyHostToDevice, c->ss.main);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_contig_off, off.data(), (size_t)n_contigs * 8, hipMemcpyHostToDevice, c->ss.main);
+    if (e == hipSuccess)
+        e = hipMemcpyAsync(c->d_contig_len, len.data(), (size_t)n_contigs * 4, hipMemcpyHostToDevice, c->ss.main);
+    if (e == hipSuccess) e = launch_reverse_half(c->ss.main, c->d_ref2, l_pac);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    if (e != hipSuccess) {
+        if (c->d_ref2) (void)hipFree(c->d_ref2);
+        if (c->d_contig_off) (void)hipFree(c->d_contig_off);
+        if (c->d_contig_len) (void)hipFree(c->d_contig_len);
+        c->d_ref2 = nullptr;
+        c->d_contig_off = nullptr;
+        c->d_contig_len = nullptr;
+        return bwg_map_hip(e);
+    }
+    c->l_pac = l_pac;
+    c->n_contigs = n_contigs;
+    c->launches += 1;
+    return BWG_OK;
+}
+
+int bwg_batch_stage_matesw(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, const bwg_pe_orient pes[4],
+                           size_t n, const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma,
+                           const uint8_t *arena, size_t arena_len, bwg_batch **out) {
+    if (!c || !p || !pes || !out || (n && !anchors) || (n_ma && !ma_rb)) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_mat(p->m, p->mat);
+    if (rc) return rc;
+    if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
+    if (!c->d_ref2) return BWG_EINVAL;  // bwg_set_reference first
+    if (n > (size_t)(INT32_MAX / 4)) return BWG_E2BIG;
+    MateRescueParams mp{};
+    mp.l_pac = c->l_pac;
+    mp.min_seed_len = min_seed_len;
+    mp.n_contigs = c->n_contigs;
+    for (int r = 0; r < 4; ++r) {
+        mp.low[r] = pes[r].low;
+        mp.high[r] = pes[r].high;
+        mp.failed[r] = pes[r].failed ? 1 : 0;
+    }
+    for (size_t k = 0; k < n; ++k) {
+        const bwg_mate_anchor &a = anchors[k];
+        if (a.l_ms > kMaxQlen) return BWG_E2BIG;
+        if (a.ms_off > arena_len || a.l_ms > arena_len - a.ms_off || !codes_ok(arena + a.ms_off, a.l_ms))
+            return BWG_EINVAL;
+        if (a.ma_begin > a.ma_end || a.ma_end > n_ma) return BWG_EINVAL;
+        for (int r = 0; r < 4; ++r)
+            if (!mp.failed[r] && (int64_t)mp.high[r] - mp.low[r] + a.l_ms > (int64_t)kMaxTlen) return BWG_E2BIG;
+    }
+    KswScoring sc = ksw_scoring(*p);
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    rc = stage_matesw(c, sc, mp, n, anchors, ma_rb, n_ma, arena, arena_len, c->ss.main, out);
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
+    return rc;
+}
+
+int bwg_batch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win) {
+    if (!c || !b || !b->matesw || (!win && b->n)) return BWG_EINVAL;
+    return fetch_windows(c, b, win);
+}
+
+int bwg_matesw_batch(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, const bwg_pe_orient pes[4], size_t n,
+                     const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma, const uint8_t *arena,
+                     size_t arena_len, bwg_rescue_window *win, bwg_ksw_result *out) {
+    if (n == 0) return (c && p && pes) ? BWG_OK : BWG_EINVAL;
+    if (!win || !out) return BWG_EINVAL;
+    bwg_batch *b = nullptr;
+    int rc = bwg_batch_stage_matesw(c, p, min_seed_len, pes, n, anchors, ma_rb, n_ma, arena, arena_len, &b);
+    if (rc) return rc;
+    rc = bwg_batch_run(c, b);
+    if (!rc) rc = bwg_batch_results(c, b, out);
+    if (!rc) rc = bwg_batch_windows(c, b, win);
+    bwg_batch_destroy(c, b);
+    return rc;
 }
 
 int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,

@@ -50,6 +50,7 @@ struct Plan {
     int32_t *d_rowmax = nullptr;
     void *d_plain_jobs = nullptr;
     void *d_plain_scratch = nullptr;
+    int32_t *d_plain_slot = nullptr;  // rescue windows: a job's index among the plain descriptors, or -1
     uint32_t n_fast = 0;
     // extension ungapped pre-pass: compacted per-bin queues; d_heads holds the queue heads, then the
     // compacted counts, then the bins' slot offsets and skip flags
@@ -544,7 +545,8 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         chosen[cl] = tu.ksw_k_step ? std::min(16, K + 2) : K;
         chosen_pk[cl] = cl < 8 && !tu.ksw_force_wide;
         if (!tu.autotune) continue;
-        if (members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+        // a rescue-window batch's descriptors are written on the device, so it only reads the table
+        if (!b->matesw && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
             const int rc = tune_ksw_class(c, b, sig, cl, members[cl], st);
             if (rc) return rc;
         }
@@ -615,6 +617,27 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         t.count = p->n_fast;
         items.push_back(t);
     }
+    std::vector<int32_t> plain_slot;
+    if (b->matesw) {
+        // the windows (and the mates' reverse complements) are computed on the device first
+        for (auto &it : items) it.stage += 1;
+        Item rc;
+        rc.kind = kKswMateRc;
+        rc.count = b->n_anchors;
+        items.push_back(rc);
+        Item wi;
+        wi.kind = kKswWindows;
+        wi.count = b->n_anchors;
+        items.push_back(wi);
+        if (!p->ksw_plain_jobs.empty()) {
+            plain_slot.assign(n, -1);
+            for (const auto &it : bins)
+                if (it.kind == kKswPlain)
+                    for (uint32_t r2 = 0; r2 < it.count; ++r2) plain_slot[p->order[it.offset + r2]] = (int32_t)r2;
+            p->d_plain_slot = (int32_t *)c->pool.alloc(n * 4);
+            if (!p->d_plain_slot) return BWG_ENOMEM;
+        }
+    }
     std::stable_sort(items.begin(), items.end(), [](const Item &a, const Item &b2) { return a.stage < b2.stage; });
     p->items = items;
     p->n_heads = std::max(1, heads);
@@ -629,6 +652,8 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     if (e == hipSuccess && !p->ksw_plain_jobs.empty())
         e = hipMemcpyAsync(p->d_plain_jobs, p->ksw_plain_jobs.data(), p->ksw_plain_jobs.size() * sizeof(KswJob),
                            hipMemcpyHostToDevice, st);
+    if (e == hipSuccess && !plain_slot.empty())
+        e = hipMemcpyAsync(p->d_plain_slot, plain_slot.data(), n * 4, hipMemcpyHostToDevice, st);
     return bwg_map_hip(e);
 }
 
@@ -672,6 +697,12 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
         return launch_ksw_group(st, it.param, it.kind == kKswRev, b->d_arena, b->d_ref, (const KswJob *)p->d_jobs,
                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->ksc,
                                 (bwg_ksw_result *)b->d_out, p->d_rowmax, it.grid);
+    case kKswMateRc:
+        return launch_mate_revcomp(st, b->d_arena, b->d_anchors, b->n_anchors);
+    case kKswWindows:
+        return launch_matesw_windows(st, b->d_anchors, b->n_anchors, b->d_ma_rb, b->mp, b->d_contig_off,
+                                     b->d_contig_len, (KswJob *)p->d_jobs, p->d_plain_slot,
+                                     (KswJob *)p->d_plain_jobs, b->d_win);
     case kKswTracker:
         return launch_ksw_tracker(st, (const KswJob *)p->d_jobs, p->d_order, it.count, b->ksc, p->d_rowmax,
                                   (bwg_ksw_result *)b->d_out);
@@ -762,6 +793,7 @@ void free_plan(bwg_ctx *c, Plan *p) {
     c->pool.free(p->d_plain_jobs);
     c->pool.free(p->d_plain_scratch);
     c->pool.free(p->d_corder);
+    c->pool.free(p->d_plain_slot);
     delete p;
 }
 

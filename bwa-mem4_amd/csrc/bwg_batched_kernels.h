@@ -65,4 +65,28 @@ hipError_t launch_ksw_pk(hipStream_t st, int K, bool rev, const uint8_t *arena, 
 hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t *list, uint32_t count,
                               const KswScoring &sc, const int32_t *rowmax, bwg_ksw_result *out);
 
+// Mate rescue with device-resident windows (rescue_windows.hip).  An anchor as the device reads it:
+// the bwg_mate_anchor fields plus where its mate's reverse complement is written in the arena.
+struct MateAnchor {
+    int64_t rb;
+    int32_t rid;
+    uint32_t l_ms;
+    uint64_t ms_off, rc_off, ma_begin, ma_end;
+};
+struct MateRescueParams {
+    int64_t l_pac;
+    int32_t low[4], high[4], failed[4];
+    int32_t min_seed_len, n_contigs;
+};
+// ref2[l_pac + k] = complement(ref2[l_pac - 1 - k]) for k < l_pac.
+hipError_t launch_reverse_half(hipStream_t st, uint8_t *ref2, int64_t l_pac);
+hipError_t launch_mate_revcomp(hipStream_t st, uint8_t *arena, const MateAnchor *anchors, uint32_t n);
+// Windows and descriptors of jobs 4k+r: jobs[] (indexed by job) and, for jobs in the plain bin,
+// plain_jobs[plain_slot[job]] get the mate (or its reverse complement) as query and the clamped
+// window of the two-strand reference as target (an empty target when the window is not valid).
+hipError_t launch_matesw_windows(hipStream_t st, const MateAnchor *anchors, uint32_t n, const int64_t *ma_rb,
+                                 const MateRescueParams &mp, const int64_t *contig_off, const int32_t *contig_len,
+                                 KswJob *jobs, const int32_t *plain_slot, KswJob *plain_jobs,
+                                 bwg_rescue_window *win);
+
 }  // namespace bwg

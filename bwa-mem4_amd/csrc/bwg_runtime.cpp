@@ -183,6 +183,86 @@ int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint
     return BWG_OK;
 }
 
+int stage_matesw(bwg_ctx *c, const KswScoring &sc, const MateRescueParams &mp, size_t n,
+                 const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma, const uint8_t *arena,
+                 size_t arena_len, hipStream_t st, bwg_batch **out) {
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    const size_t nj = 4 * n;
+    bwg_batch *b = new_batch(1, nj);
+    if (!b) return BWG_ENOMEM;
+    b->ksc = sc;
+    b->matesw = true;
+    b->n_anchors = (uint32_t)n;
+    b->mp = mp;
+    b->d_ref = c->d_ref2;
+    b->d_contig_off = c->d_contig_off;
+    b->d_contig_len = c->d_contig_len;
+    b->ref_owner = c;
+    c->ref_users++;
+    // the mates' reverse complements go after the caller's arena
+    std::vector<MateAnchor> ha(n);
+    size_t rc = arena_len;
+    for (size_t k = 0; k < n; ++k) {
+        MateAnchor &a = ha[k];
+        a.rb = anchors[k].rb;
+        a.rid = anchors[k].rid;
+        a.l_ms = anchors[k].l_ms;
+        a.ms_off = anchors[k].ms_off;
+        a.rc_off = rc;
+        a.ma_begin = anchors[k].ma_begin;
+        a.ma_end = anchors[k].ma_end;
+        rc += anchors[k].l_ms;
+    }
+    // planning descriptors: the mate length and the widest window an orientation can give
+    // (hi - lo + l_ms before any clamp); the device writes the actual windows before the DP runs
+    b->hksw.resize(nj);
+    for (size_t k = 0; k < n; ++k)
+        for (int r = 0; r < 4; ++r) {
+            KswJob &j = b->hksw[4 * k + r];
+            const int64_t bound = mp.failed[r] ? 0 : (int64_t)mp.high[r] - mp.low[r] + anchors[k].l_ms;
+            j.q_off = anchors[k].ms_off;
+            j.t_off = kTFromRef;
+            j.qlen = anchors[k].l_ms;
+            j.tlen = (uint32_t)std::max<int64_t>(0, bound);
+            j.aux0 = (uint32_t)(4 * k + r);
+            j.aux1 = 0;
+        }
+    b->arena_len = rc;
+    b->d_arena = (uint8_t *)c->pool.alloc(rc + 64);
+    b->d_jobs = c->pool.alloc(std::max<size_t>(nj * sizeof(KswJob), 64));
+    b->d_out = c->pool.alloc(std::max<size_t>(nj * sizeof(bwg_ksw_result), 64));
+    b->d_anchors = (MateAnchor *)c->pool.alloc(std::max<size_t>(n * sizeof(MateAnchor), 64));
+    b->d_ma_rb = (int64_t *)c->pool.alloc(std::max<size_t>(n_ma * 8, 64));
+    b->d_win = (bwg_rescue_window *)c->pool.alloc(std::max<size_t>(nj * sizeof(bwg_rescue_window), 64));
+    int rcode = BWG_OK;
+    if (!b->d_arena || !b->d_jobs || !b->d_out || !b->d_anchors || !b->d_ma_rb || !b->d_win) rcode = BWG_ENOMEM;
+    hipError_t e = hipSuccess;
+    if (!rcode && arena_len) e = hipMemcpyAsync(b->d_arena, arena, arena_len, hipMemcpyHostToDevice, st);
+    if (!rcode && e == hipSuccess && nj)
+        e = hipMemcpyAsync(b->d_jobs, b->hksw.data(), nj * sizeof(KswJob), hipMemcpyHostToDevice, st);
+    if (!rcode && e == hipSuccess && n)
+        e = hipMemcpyAsync(b->d_anchors, ha.data(), n * sizeof(MateAnchor), hipMemcpyHostToDevice, st);
+    if (!rcode && e == hipSuccess && n_ma)
+        e = hipMemcpyAsync(b->d_ma_rb, ma_rb, n_ma * 8, hipMemcpyHostToDevice, st);
+    if (!rcode) rcode = bwg_map_hip(e);
+    if (rcode) {
+        destroy_batch(c, b);
+        return rcode;
+    }
+    *out = b;
+    return BWG_OK;
+}
+
+int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win) {
+    if (b->n == 0) return BWG_OK;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    hipError_t e = hipMemcpyAsync(win, b->d_win, b->n * sizeof(bwg_rescue_window), hipMemcpyDeviceToHost,
+                                  c->ss.main);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    return bwg_map_hip(e);
+}
+
 int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
     if (b->n == 0) return BWG_OK;
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
@@ -282,8 +362,9 @@ int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
 
 int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
     if (b->n == 0) return BWG_OK;
-    // band-retry batches keep their rounds on the device, which only the batched form does
-    if (form == BWG_FORM_PLAIN && !b->side_tries) {
+    // band-retry batches keep their rounds on the device, and rescue batches compute their windows
+    // there; only the batched form does either
+    if (form == BWG_FORM_PLAIN && !b->side_tries && !b->matesw) {
         int rc = prepare_plain(c, b, ss.main);
         if (rc) return rc;
         hipError_t e = hipEventRecord(b->ev_start, ss.main);
@@ -351,6 +432,10 @@ void destroy_batch(bwg_ctx *c, bwg_batch *b) {
     c->pool.free(b->d_prev);
     c->pool.free(b->d_wout);
     c->pool.free(b->d_active);
+    c->pool.free(b->d_anchors);
+    c->pool.free(b->d_ma_rb);
+    c->pool.free(b->d_win);
+    if (b->ref_owner) b->ref_owner->ref_users--;
     release_batched(c, b);
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);
     if (b->ev_end) (void)hipEventDestroy(b->ev_end);

@@ -8,6 +8,7 @@
 #include <mutex>
 #include <vector>
 
+#include "bwg_batched_kernels.h"
 #include "bwg_internal.h"
 #include "bwg_tuner.h"
 
@@ -75,6 +76,13 @@ struct bwg_ctx {
     std::atomic<int64_t> launches{0}, jobs{0};
     int64_t last_kernel_ns = 0, last_main_ns = 0, last_main_launches = 0;
     bwg::DevicePool pool;
+    // two-strand reference for mate rescue (bwg_set_reference): 2*l_pac bytes and the contigs
+    uint8_t *d_ref2 = nullptr;
+    int64_t l_pac = 0;
+    int64_t *d_contig_off = nullptr;
+    int32_t *d_contig_len = nullptr;
+    int32_t n_contigs = 0;
+    std::atomic<int> ref_users{0};  // staged rescue batches reading the reference
 };
 
 struct bwg_batch {
@@ -99,6 +107,16 @@ struct bwg_batch {
     int side_tries = 0;
     int32_t *d_prev0 = nullptr, *d_prev = nullptr, *d_wout = nullptr;
     uint8_t *d_active = nullptr;
+    // mate rescue with device-resident windows (bwg_batch_stage_matesw): 4 jobs per anchor
+    bool matesw = false;
+    uint32_t n_anchors = 0;
+    bwg::MateRescueParams mp{};
+    bwg::MateAnchor *d_anchors = nullptr;
+    int64_t *d_ma_rb = nullptr;
+    bwg_rescue_window *d_win = nullptr;
+    const int64_t *d_contig_off = nullptr;
+    const int32_t *d_contig_len = nullptr;
+    bwg_ctx *ref_owner = nullptr;  // the context whose reference this batch holds a use of
     // batched form state (bwg_batched.cpp)
     void *plan = nullptr;
     std::vector<void *> retry_plans;  // band-retry rounds 1.. of a side batch
@@ -125,6 +143,12 @@ int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint
                const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out);
 int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out);
+// A mate-rescue round: n anchors, 4n jobs whose windows the device computes against the context's
+// two-strand reference.
+int stage_matesw(bwg_ctx *c, const KswScoring &sc, const MateRescueParams &mp, size_t n,
+                 const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma, const uint8_t *arena,
+                 size_t arena_len, hipStream_t st, bwg_batch **out);
+int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win);
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out);

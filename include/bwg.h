@@ -176,6 +176,56 @@ int bwg_batch_stage_side(bwg_ctx *ctx, const bwg_ext_params *p, int max_band_try
                          const int32_t *h0, const int32_t *prev, bwg_batch **out);
 int bwg_batch_side_bands(bwg_ctx *ctx, bwg_batch *b, int32_t *w_out);
 
+/* ---- mate rescue with device-resident windows -----------------------------------------------
+ * Replaces one round of the reference's batched mate rescue (crates/bwa-mem/src/pe.rs:1040-1113):
+ * matesw_collect (pe.rs:596-679) with its reference fetch bns_fetch_seq (pe.rs:211-269), the flat
+ * batched_ksw_align2 call (matesw.rs:299-309), and nothing of matesw_apply (pe.rs:716), which stays
+ * with the caller.  The reference (bwg_set_reference) lives on the device as both strands, one byte
+ * per base (2 * l_pac bytes); windows and their sequences never leave the device.
+ *
+ * Per anchor (one region `a` of one read, with the mate's sequence) and orientation r = 0..3 the
+ * device computes what matesw_collect does: the skip test (pes[r].failed, or a mate region already
+ * at a proper distance in direction r, mem_infer_dir pe.rs:192-209), the insert window from
+ * pes[r].low/high and the mate length, the clamp to [0, 2*l_pac), the contig clamp around the
+ * window's midpoint (mirrored on the reverse strand), and the rid and min_seed_len test; then it
+ * aligns the mate (reverse-complemented when the orientation says so) against the window with
+ * batched_ksw_align2 semantics. */
+typedef struct { int64_t offset; int32_t len; } bwg_contig;          /* forward-strand annotation */
+typedef struct { int32_t low, high, failed; } bwg_pe_orient;        /* PeStat, pe.rs:143-158 */
+typedef struct {
+    int64_t rb;          /* the anchor region's start in the doubled 2*l_pac space */
+    int32_t rid;         /* its contig */
+    uint32_t l_ms;       /* mate length */
+    uint64_t ms_off;     /* mate sequence (codes 0..4, as read) in the arena */
+    uint64_t ma_begin;   /* the mate's existing regions: ma_rb[ma_begin .. ma_end) */
+    uint64_t ma_end;
+} bwg_mate_anchor;
+enum { BWG_WIN_VALID = 1, BWG_WIN_REV = 2, BWG_WIN_SKIPPED = 4 };
+typedef struct {
+    int64_t rb, re;      /* the clamped window in 2*l_pac space (valid windows) */
+    int32_t rid;         /* contig of the window's midpoint, -1 outside every contig */
+    int32_t flags;       /* BWG_WIN_*: a job ran / the mate was reverse-complemented / r skipped */
+} bwg_rescue_window;
+/* Upload the forward strand (codes 0..4, l_pac bytes) and its contigs (ascending offsets, the
+ * reference's bns anns); the device builds the reverse-complement half.  Replaces any previous
+ * reference of the context. */
+int bwg_set_reference(bwg_ctx *ctx, const uint8_t *fwd, int64_t l_pac, const bwg_contig *contigs,
+                      int32_t n_contigs);
+/* One rescue round for n anchors: 4n windows and 4n results, index 4*k + r.  Jobs whose
+ * window is not valid get the default result {0,-1,-1,-1,-1,-1,-1}.  p->minsc and p->max_sc are the
+ * caller's min_seed_len * a and a (pe.rs:1096-1098); min_seed_len is the window-length floor. */
+int bwg_matesw_batch(bwg_ctx *ctx, const bwg_ksw_params *p, int32_t min_seed_len,
+                     const bwg_pe_orient pes[4], size_t n, const bwg_mate_anchor *anchors,
+                     const int64_t *ma_rb, size_t n_ma, const uint8_t *arena, size_t arena_len,
+                     bwg_rescue_window *win, bwg_ksw_result *out);
+/* Staged form: run with bwg_batch_run, results (4n) with bwg_batch_results, windows with
+ * bwg_batch_windows. */
+int bwg_batch_stage_matesw(bwg_ctx *ctx, const bwg_ksw_params *p, int32_t min_seed_len,
+                           const bwg_pe_orient pes[4], size_t n, const bwg_mate_anchor *anchors,
+                           const int64_t *ma_rb, size_t n_ma, const uint8_t *arena, size_t arena_len,
+                           bwg_batch **out);
+int bwg_batch_windows(bwg_ctx *ctx, bwg_batch *b, bwg_rescue_window *win);
+
 /* ---- serving ----------------------------------------------------------------------------------
  * Continuous batching of alignment work from concurrent producers.  No counterpart in the
  * reference: its callers run extension and rescue per rayon chunk

@@ -18,6 +18,9 @@
  *   orc_ungapped_*         crates/bwa-neon/src/batched.rs:493-521, 602-697, 756-764 (ksw_ungapped.c)
  *   orc_extend_side        crates/bwa-mem/src/lib.rs:332-373     (extend_side; ksw_side.c)
  *   orc_run_side           crates/bwa-mem/src/across.rs:1357-1460 (run_side; ksw_side.c)
+ *   orc_matesw_*           crates/bwa-mem/src/pe.rs:192-269, 596-679, 1040-1113 and
+ *                          crates/bwa-index/src/bntseq.rs:237-243, 337-367 (mate rescue windows;
+ *                          ksw_matesw.c)
  *
  * A second, independent restatement in the lockstep (SoA, many jobs per pass) form lives in
  * ksw_lockstep.c, and brute-force checkers in ksw_brute.c.
@@ -147,6 +150,33 @@ void orc_run_side(size_t n, const uint8_t *arena, const uint64_t *q_off, const u
                   int32_t m, const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
                   int32_t w, int32_t end_bonus, int32_t zdrop, int32_t max_band_try, orc_ext_result *out,
                   int32_t *w_out, int nthreads);
+
+/* Mate rescue windows (ksw_matesw.c).  Layouts match bwg_mate_anchor and bwg_rescue_window. */
+typedef struct {
+    int64_t rb;
+    int32_t rid;
+    uint32_t l_ms;
+    uint64_t ms_off, ma_begin, ma_end;
+} orc_mate_anchor;
+typedef struct {
+    int64_t rb, re;
+    int32_t rid, flags;
+} orc_rescue_window;
+void orc_infer_dir(int64_t l_pac, int64_t b1, int64_t b2, int32_t *dir, int64_t *dist);
+int64_t orc_depos(int64_t l_pac, int64_t pos, int *is_rev);
+int32_t orc_pos2rid(int64_t l_pac, const int64_t *off, int32_t n, int64_t pos_f);
+void orc_fetch_window(int64_t l_pac, const int64_t *off, const int32_t *len, int32_t n, int64_t rb, int64_t mid,
+                      int64_t re, int64_t *rb_out, int64_t *re_out, int32_t *rid_out);
+uint8_t orc_ref_base(const uint8_t *fwd, int64_t l_pac, int64_t p);
+void orc_matesw_collect(int64_t l_pac, const int64_t *off, const int32_t *len, int32_t n_contigs,
+                        const int32_t *low, const int32_t *high, const int32_t *failed, int32_t min_seed_len,
+                        int64_t a_rb, int32_t a_rid, int64_t l_ms, const int64_t *ma_rb, size_t n_ma,
+                        orc_rescue_window *win);
+void orc_matesw_batch(int64_t l_pac, const uint8_t *fwd, const int64_t *off, const int32_t *len, int32_t n_contigs,
+                      const int32_t *low, const int32_t *high, const int32_t *failed, int32_t min_seed_len,
+                      size_t n, const orc_mate_anchor *anchors, const int64_t *ma_rb, const uint8_t *arena,
+                      const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t minsc,
+                      int32_t max_sc, orc_rescue_window *win, orc_ksw_result *out, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -99,6 +99,17 @@ def lib():
         L.orc_ungapped_xmax.argtypes = [_P, _I, _I, _I, _I]
         L.orc_extend_side.restype = ExtResult
         L.orc_extend_side.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_infer_dir.restype = None
+        L.orc_infer_dir.argtypes = [C.c_int64, C.c_int64, C.c_int64, _P, _P]
+        L.orc_pos2rid.restype = _I
+        L.orc_pos2rid.argtypes = [C.c_int64, _P, _I, C.c_int64]
+        L.orc_fetch_window.restype = None
+        L.orc_fetch_window.argtypes = [C.c_int64, _P, _P, _I, C.c_int64, C.c_int64, C.c_int64, _P, _P, _P]
+        L.orc_ref_base.restype = C.c_uint8
+        L.orc_ref_base.argtypes = [_P, C.c_int64, C.c_int64]
+        L.orc_matesw_batch.restype = None
+        L.orc_matesw_batch.argtypes = [C.c_int64, _P, _P, _P, _I, _P, _P, _P, _I, C.c_size_t, _P, _P, _P, _P,
+                                       _I, _I, _I, _I, _I, _I, _P, _P, C.c_int]
         L.orc_run_side.restype = None
         L.orc_run_side.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I,
                                    _I, _P, _P, C.c_int]
@@ -159,6 +170,56 @@ def run_side(batch, prev, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, 
                        5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, max_band_try, out.ctypes.data,
                        w_out.ctypes.data, nthreads)
     return out, w_out
+
+
+WINDOW_DTYPE = np.dtype([("rb", "<i8"), ("re", "<i8"), ("rid", "<i4"), ("flags", "<i4")])
+
+
+def infer_dir(l_pac, b1, b2):
+    d, dist = C.c_int32(0), C.c_int64(0)
+    lib().orc_infer_dir(l_pac, b1, b2, C.addressof(d), C.addressof(dist))
+    return d.value, dist.value
+
+
+def _contig_arrays(contigs):
+    off = np.ascontiguousarray(contigs["offset"], dtype=np.int64)
+    ln = np.ascontiguousarray(contigs["len"], dtype=np.int32)
+    return off, ln
+
+
+def pos2rid(l_pac, contigs, pos_f):
+    off, _ = _contig_arrays(contigs)
+    return lib().orc_pos2rid(l_pac, off.ctypes.data, len(off), pos_f)
+
+
+def fetch_window(l_pac, contigs, rb, mid, re):
+    off, ln = _contig_arrays(contigs)
+    a, b, r = C.c_int64(0), C.c_int64(0), C.c_int32(0)
+    lib().orc_fetch_window(l_pac, off.ctypes.data, ln.ctypes.data, len(off), rb, mid, re, C.addressof(a),
+                           C.addressof(b), C.addressof(r))
+    return a.value, b.value, r.value
+
+
+def ref_bases(fwd, rb, re):
+    fwd = np.ascontiguousarray(fwd, dtype=np.uint8)
+    return np.array([lib().orc_ref_base(fwd.ctypes.data, len(fwd), p) for p in range(rb, re)], dtype=np.uint8)
+
+
+def matesw_batch(rnd, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, nthreads=8):
+    """One mate-rescue round (RescueRound): (windows, results), 4 per anchor."""
+    n = len(rnd.anchors)
+    win = np.zeros(4 * n, dtype=WINDOW_DTYPE)
+    out = np.zeros(4 * n, dtype=KSW_DTYPE)
+    off, ln = _contig_arrays(rnd.contigs)
+    low = np.ascontiguousarray(rnd.pes["low"], dtype=np.int32)
+    high = np.ascontiguousarray(rnd.pes["high"], dtype=np.int32)
+    failed = np.ascontiguousarray(rnd.pes["failed"], dtype=np.int32)
+    ma, mp = _mat(mat)
+    lib().orc_matesw_batch(rnd.l_pac, rnd.fwd.ctypes.data, off.ctypes.data, ln.ctypes.data, len(off),
+                           low.ctypes.data, high.ctypes.data, failed.ctypes.data, rnd.min_seed_len, n,
+                           rnd.anchors.ctypes.data, rnd.ma_rb.ctypes.data, rnd.arena.ctypes.data, mp, o_del, e_del,
+                           o_ins, e_ins, minsc, max_sc, win.ctypes.data, out.ctypes.data, nthreads)
+    return win, out
 
 
 def local_fwd(q, t, mat, o_del, e_del, o_ins, e_ins, minsc, endsc, max_sc, lanes):
