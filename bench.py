@@ -94,7 +94,10 @@ def main():
     ap.add_argument("--no-check", action="store_true")
     ap.add_argument("--cpu-threads", type=int, default=16)
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
-    ap.add_argument("--only", default="ext,small,resc,side,matesw,e2e", help="comma list of workloads to time")
+    ap.add_argument("--only", default="ext,small,resc,side,matesw,global,e2e",
+                    help="comma list of workloads to time")
+    ap.add_argument("--global-jobs", type=int, default=200_000,
+                    help="region-to-CIGAR jobs of the banded global alignment workload")
     ap.add_argument("--matesw-anchors", type=int, default=200_000,
                     help="anchors of the mate-rescue round (4 orientation jobs each)")
     ap.add_argument("--serve-rates", default="5e4,2e5,2e6,1e7,3e7,6e7",
@@ -242,6 +245,33 @@ def main():
             log("mate rescue: first %d anchors bit-exact vs oracle: %s" % (sub_n, ok))
             if not ok:
                 sys.exit(1)
+    glb = None
+    if "global" in only and args.global_jobs:
+        # region-to-CIGAR: reads of 100/150/250 bp against their reference span (1% substitutions,
+        # 30% of the regions with indels), each with the band the reference's CIGAR step gives
+        ctx.set_form(bwg.FORM_BATCHED)
+        log("global alignment: generating %d region jobs" % args.global_jobs)
+        gb, gw = bwg.generate_global_jobs(args.global_jobs, seed=44)
+        gp = bwg.global_params(5, mat, 6, 1, 6, 1)
+        st = ctx.stage_global(gb, gp, gw)
+        st.run()
+        per, kern, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+        gscore = st.results()
+        gn, gc = st.cigars()
+        st.close()
+        qg, tg = gb.q_len.astype(np.int64), gb.t_len.astype(np.int64)
+        gcells = float(np.sum(np.minimum(qg, 2 * gw.astype(np.int64) + 1) * tg))  # upper bound: full band rows
+        glb = dict(step_s=per, kernel_s=kern, cells=gcells)
+        log("global alignment: %.3f ms/step (kernels %.3f ms), %.3g jobs/s" % (per * 1e3, kern * 1e3,
+                                                                                 args.global_jobs / per))
+        if not args.no_check:
+            sys.path.insert(0, os.path.join(ROOT, "tests"))
+            import oracle_binding as ob
+            ws, wn, wc = ob.global_batch(gb, gw, mat, 6, 1, 6, 1, nthreads=args.cpu_threads)
+            ok = np.array_equal(ws, gscore) and np.array_equal(wn.astype(np.uint32), gn) and np.array_equal(wc, gc)
+            log("global alignment: bit-exact vs oracle: %s" % ok)
+            if not ok:
+                sys.exit(1)
     e2e = None
     if "e2e" in only and "ext" in only:
         # PCIe-inclusive: the synchronous entry point from host buffers (stage, run, copy back)
@@ -349,6 +379,9 @@ def main():
         line["matesw_anchors_per_s"] = args.matesw_anchors / matesw["step_s"]
         line["matesw_jobs_per_s"] = matesw["jobs"] / matesw["step_s"]
         line["matesw_gcell_s"] = matesw["cells"] / matesw["step_s"] / 1e9
+    if glb:
+        line["global_jobs_per_s"] = args.global_jobs / glb["step_s"]
+        line["global_band_gcell_s"] = glb["cells"] / glb["step_s"] / 1e9
     for w, r in side.items():
         line["side_w%d_jobs_per_s" % w] = args.jobs / r["step_s"]
         line["side_w%d_retried" % w] = r["retried"]

@@ -45,6 +45,7 @@ EXPORTED_SYMBOLS = (
     "bwg_server_wait", "bwg_server_stats", "bwg_server_destroy", "bwg_hbm_copy", "bwg_tune_report",
     "bwg_batch_ungapped_hits", "bwg_extend_side_batch", "bwg_batch_stage_side", "bwg_batch_side_bands",
     "bwg_set_reference", "bwg_matesw_batch", "bwg_batch_stage_matesw", "bwg_batch_windows",
+    "bwg_global_batch", "bwg_batch_stage_global", "bwg_batch_cigar_lengths", "bwg_batch_cigars",
 )
 
 # bwg_mate_anchor, bwg_rescue_window, bwg_pe_orient and bwg_contig as numpy records
@@ -72,6 +73,11 @@ class KswParams(C.Structure):
     _fields_ = [("m", C.c_int32), ("mat", C.c_int8 * 25), ("o_del", C.c_int32),
                 ("e_del", C.c_int32), ("o_ins", C.c_int32), ("e_ins", C.c_int32),
                 ("minsc", C.c_int32), ("max_sc", C.c_int32)]
+
+
+class GlobalParams(C.Structure):
+    _fields_ = [("m", C.c_int32), ("mat", C.c_int8 * 25), ("o_del", C.c_int32),
+                ("e_del", C.c_int32), ("o_ins", C.c_int32), ("e_ins", C.c_int32)]
 
 
 _L = None
@@ -136,6 +142,15 @@ def _lib():
                                              C.c_size_t, _P]
         L.bwg_batch_windows.restype = C.c_int
         L.bwg_batch_windows.argtypes = [_P, _P, _P]
+        L.bwg_global_batch.restype = C.c_int
+        L.bwg_global_batch.argtypes = [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _P,
+                                       C.c_size_t]
+        L.bwg_batch_stage_global.restype = C.c_int
+        L.bwg_batch_stage_global.argtypes = [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P]
+        L.bwg_batch_cigar_lengths.restype = C.c_int
+        L.bwg_batch_cigar_lengths.argtypes = [_P, _P, _P]
+        L.bwg_batch_cigars.restype = C.c_int
+        L.bwg_batch_cigars.argtypes = [_P, _P, _P, C.c_size_t]
         _L = L
     return _L
 
@@ -159,6 +174,10 @@ def _mat25(mat):
 
 def ext_params(m, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop):
     return ExtParams(m, _mat25(mat), o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop)
+
+
+def global_params(m, mat, o_del, e_del, o_ins, e_ins):
+    return GlobalParams(m, _mat25(mat), o_del, e_del, o_ins, e_ins)
 
 
 def ksw_params(m, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc):
@@ -291,6 +310,31 @@ class Context:
                                             w_out.ctypes.data), "bwg_extend_side_batch")
         return out, w_out
 
+    def global_arrays(self, batch, params, w):
+        """ksw_global2 on every job with its own band w[k]: (scores, runs per job, the runs of all
+        jobs concatenated as len << 4 | op words)."""
+        n = len(batch)
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        score = np.zeros(n, dtype=np.int32)
+        ncig = np.zeros(n, dtype=np.uint32)
+        cap = int(np.sum(batch.q_len.astype(np.int64) + batch.t_len + 2)) if n else 0
+        cig = np.zeros(max(1, cap), dtype=np.uint32)
+        nn, ap, al, qo, ql, to, tl = batch._args()
+        _check(_lib().bwg_global_batch(self.h, C.byref(params), nn, ap, al, qo, ql, to, tl, w.ctypes.data,
+                                       score.ctypes.data, ncig.ctypes.data, cig.ctypes.data, cap),
+               "bwg_global_batch")
+        return score, ncig, cig[:int(ncig.sum())]
+
+    def stage_global(self, batch, params, w):
+        h = C.c_void_p()
+        w = np.ascontiguousarray(w, dtype=np.int32)
+        n, ap, al, qo, ql, to, tl = batch._args()
+        _check(_lib().bwg_batch_stage_global(self.h, C.byref(params), n, ap, al, qo, ql, to, tl, w.ctypes.data,
+                                             C.byref(h)), "bwg_batch_stage_global")
+        sb = StagedBatch(self, h, n, np.int32)
+        sb.cigar_cap = int(np.sum(batch.q_len.astype(np.int64) + batch.t_len + 2)) if n else 0
+        return sb
+
     def set_reference(self, fwd, contigs):
         """Upload a reference (forward strand codes, one byte per base) and its contigs
         (CONTIG_DTYPE records, ascending offsets) for mate rescue."""
@@ -353,6 +397,14 @@ class StagedBatch:
         out = np.zeros(self.n, dtype=self.dtype)
         _check(_lib().bwg_batch_results(self.ctx.h, self.h, out.ctypes.data), "bwg_batch_results")
         return out
+
+    def cigars(self):
+        """Runs per job and the concatenated runs of a global-alignment batch's last run."""
+        ncig = np.zeros(self.n, dtype=np.uint32)
+        _check(_lib().bwg_batch_cigar_lengths(self.ctx.h, self.h, ncig.ctypes.data), "bwg_batch_cigar_lengths")
+        cig = np.zeros(max(1, int(ncig.sum())), dtype=np.uint32)
+        _check(_lib().bwg_batch_cigars(self.ctx.h, self.h, cig.ctypes.data, len(cig)), "bwg_batch_cigars")
+        return ncig, cig[:int(ncig.sum())]
 
     def windows(self):
         """Rescue windows of a mate-rescue batch's last run (4 per anchor)."""
@@ -561,6 +613,49 @@ def generate_rescue_round(n_pairs, genome_len=2_000_000, n_contigs=4, read_len=1
     anchors["ma_end"] = ma_end
     arena = np.ascontiguousarray(mates.reshape(-1), dtype=np.uint8)
     return RescueRound(fwd, contigs, pes, anchors, np.asarray(ma_rb, dtype=np.int64), arena, 19)
+
+
+def global_band(qlen, tlen, a=1, o_del=6, e_del=1, o_ins=6, e_ins=1, w_region=100):
+    """The band the reference's region-to-CIGAR step gives ksw_global2 (crates/bwa-mem/src/
+    cigar.rs:417-444): the affordable gap for half the read, plus the length difference, capped
+    by the region's band and floored at the length difference + 3."""
+    half = ((np.asarray(qlen, np.int64) + 1) >> 1) * a
+    max_ins = ((half - o_ins) / e_ins + 1.0).astype(np.int64)
+    max_del = ((half - o_del) / e_del + 1.0).astype(np.int64)
+    max_gap = np.maximum(np.maximum(max_ins, max_del), 1)
+    diff = np.abs(np.asarray(tlen, np.int64) - np.asarray(qlen, np.int64))
+    w = (max_gap + diff + 1) >> 1
+    w = np.minimum(w, w_region)
+    return np.maximum(w, diff + 3).astype(np.int32)
+
+
+def generate_global_jobs(n, seed=42, read_lens=(100, 150, 250), sub_rate=0.01, indel_frac=0.3, max_indel=12):
+    """Synthetic region-to-CIGAR jobs: a read segment as query and its reference span as target
+    (1% substitutions; `indel_frac` of the jobs with one or two indels of 1..max_indel bases),
+    each with the band global_band() gives.  Returns (JobBatch, w)."""
+    rng = np.random.default_rng(seed)
+    lens = rng.choice(np.asarray(read_lens), n)
+    qs, ts = [], []
+    for k in range(n):
+        L = int(lens[k])
+        q = rng.integers(0, 4, L).astype(np.uint8)
+        t = q.copy()
+        m = rng.random(L) < sub_rate
+        t[m] = (t[m] + rng.integers(1, 4, int(m.sum()))) % 4
+        if rng.random() < indel_frac:
+            for _ in range(int(rng.integers(1, 3))):
+                p = int(rng.integers(1, len(t)))
+                g = int(rng.integers(1, max_indel + 1))
+                if rng.random() < 0.5:
+                    t = np.concatenate([t[:p], rng.integers(0, 4, g).astype(np.uint8), t[p:]])
+                else:
+                    t = np.concatenate([t[:p], t[min(len(t), p + g):]])
+        if len(t) == 0:
+            t = np.zeros(1, np.uint8)
+        qs.append(q)
+        ts.append(t)
+    batch = JobBatch.from_pairs(qs, ts)
+    return batch, global_band(batch.q_len, batch.t_len)
 
 
 def default_mat(a=1, b=4):

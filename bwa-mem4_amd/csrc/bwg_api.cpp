@@ -10,6 +10,7 @@
 #include <algorithm>
 #include <atomic>
 #include <climits>
+#include <cstdlib>
 #include <cstdio>
 #include <cstring>
 #include <mutex>
@@ -275,6 +276,64 @@ int bwg_batch_stage_side(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, 
 int bwg_batch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
     if (!c || !b || (!w_out && b->n) || !b->side_tries) return BWG_EINVAL;
     return fetch_side_bands(c, b, w_out);
+}
+
+int bwg_batch_stage_global(bwg_ctx *c, const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len,
+                           const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                           const int32_t *w, bwg_batch **out) {
+    if (!c || !p || !out || (n && !w)) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_mat(p->m, p->mat);
+    if (rc) return rc;
+    if (p->e_del < 1 || p->e_ins < 1 || p->o_del < 0 || p->o_ins < 0) return BWG_EINVAL;
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    if (rc) return rc;
+    for (size_t k = 0; k < n; ++k) {
+        if (q_len[k] > 512) return BWG_E2BIG;
+        // the band must reach the last cell (cigar.rs:441-443 guarantees it)
+        if ((int64_t)w[k] < std::abs((int64_t)q_len[k] - (int64_t)t_len[k])) return BWG_EINVAL;
+        // gap costs of a whole side stay far from MINUS_INF
+        const int64_t span = (int64_t)std::max(q_len[k], t_len[k]) + 1;
+        if (p->o_del + (int64_t)p->e_del * span >= (1 << 28) || p->o_ins + (int64_t)p->e_ins * span >= (1 << 28))
+            return BWG_E2BIG;
+    }
+    GlobalScoring sc{};
+    for (int k = 0; k < 25; ++k) sc.mat[k] = p->mat[k];
+    sc.o_del = p->o_del;
+    sc.e_del = p->e_del;
+    sc.o_ins = p->o_ins;
+    sc.e_ins = p->e_ins;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    rc = stage_global(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, w, c->ss.main, out);
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
+    return rc;
+}
+
+int bwg_batch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_cigar) {
+    if (!c || !b || b->kind != 2 || (!n_cigar && b->n)) return BWG_EINVAL;
+    return fetch_cigar_lengths(c, b, n_cigar);
+}
+
+int bwg_batch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *cigar, size_t cap) {
+    if (!c || !b || b->kind != 2 || (!cigar && cap)) return BWG_EINVAL;
+    return fetch_cigars(c, b, cigar, cap);
+}
+
+int bwg_global_batch(bwg_ctx *c, const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len,
+                     const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                     const int32_t *w, int32_t *score, uint32_t *n_cigar, uint32_t *cigar, size_t cigar_cap) {
+    if (n == 0) return (c && p) ? BWG_OK : BWG_EINVAL;
+    if (!score || !n_cigar) return BWG_EINVAL;
+    bwg_batch *b = nullptr;
+    int rc = bwg_batch_stage_global(c, p, n, arena, arena_len, q_off, q_len, t_off, t_len, w, &b);
+    if (rc) return rc;
+    rc = bwg_batch_run(c, b);
+    if (!rc) rc = bwg_batch_results(c, b, score);
+    if (!rc) rc = bwg_batch_cigar_lengths(c, b, n_cigar);
+    if (!rc) rc = bwg_batch_cigars(c, b, cigar, cigar_cap);
+    bwg_batch_destroy(c, b);
+    return rc;
 }
 
 int bwg_set_reference(bwg_ctx *c, const uint8_t *fwd, int64_t l_pac, const bwg_contig *contigs,

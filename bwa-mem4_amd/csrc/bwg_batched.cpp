@@ -657,6 +657,68 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     return bwg_map_hip(e);
 }
 
+// Global alignment with traceback: bins by the group kernel's K for the band's widest row
+// (min(qlen, 2w+1) columns), each job's traceback codes and CIGAR scratch laid out by the plan.
+int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
+    const size_t n = b->n;
+    std::vector<Item> bins;
+    std::vector<GlobalJob> jobs = b->hglb;
+    auto key_of = [&](size_t k) {
+        GlobalJob &j = jobs[k];
+        const int64_t ncol = std::min<int64_t>(j.qlen, 2 * (int64_t)std::max(j.w, 0) + 1);
+        const int K = odd_k_for((int)std::max<int64_t>(ncol, 1));
+        j.k = (uint32_t)K;
+        for (size_t r = 0; r < bins.size(); ++r)
+            if (bins[r].param == K) {
+                bins[r].qcap = std::max(bins[r].qcap, qcap_class((int)j.qlen));
+                return (int)r;
+            }
+        Item it;
+        it.kind = kGlbGroup;
+        it.param = K;
+        it.qcap = qcap_class((int)j.qlen);
+        bins.push_back(it);
+        return (int)bins.size() - 1;
+    };
+    auto work_of = [&](size_t k) { return (uint64_t)jobs[k].tlen * (std::min<uint64_t>(jobs[k].qlen, 64) + 1); };
+    p->order = bin_jobs(n, bins, key_of, work_of);
+    uint64_t zw = 0, ow = 0;
+    for (size_t k = 0; k < n; ++k) {
+        GlobalJob &j = jobs[k];
+        j.z_off = zw;
+        j.ops_off = ow;
+        zw += (uint64_t)j.tlen * 16 * ((j.k + 7) / 8);
+        ow += (uint64_t)j.qlen + j.tlen + 2;
+    }
+    std::vector<Item> items;
+    for (size_t r = 0; r < bins.size(); ++r) {
+        Item it = bins[r];
+        it.head = (uint32_t)r;
+        it.stage = 0;
+        const int per_cu = glb_group_blocks_per_cu(it.param, it.qcap);
+        it.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)it.count + 15) / 16));
+        items.push_back(it);
+    }
+    Item tr;
+    tr.kind = kGlbTrace;
+    tr.stage = 1;
+    tr.count = (uint32_t)n;
+    items.push_back(tr);
+    p->items = items;
+    p->n_heads = std::max<int>(1, (int)bins.size());
+    p->d_order = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
+    p->d_heads = (uint32_t *)c->pool.alloc((size_t)p->n_heads * 4);
+    p->d_jobs = c->pool.alloc(std::max<size_t>(1, n) * sizeof(GlobalJob));
+    b->d_z = (uint32_t *)c->pool.alloc(std::max<uint64_t>(1, zw) * 4);
+    b->d_ops = (uint32_t *)c->pool.alloc(std::max<uint64_t>(1, ow) * 4);
+    b->d_nops = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
+    if (!p->d_order || !p->d_heads || !p->d_jobs || !b->d_z || !b->d_ops || !b->d_nops) return BWG_ENOMEM;
+    hipError_t e = hipMemcpyAsync(p->d_order, p->order.data(), n * 4, hipMemcpyHostToDevice, st);
+    if (e == hipSuccess) e = hipMemcpyAsync(p->d_jobs, jobs.data(), n * sizeof(GlobalJob), hipMemcpyHostToDevice, st);
+    b->hglb = jobs;  // with the bins' K and the layout offsets, for the CIGAR gather
+    return bwg_map_hip(e);
+}
+
 hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
     switch (it.kind) {
     case kExtGroup:
@@ -697,6 +759,12 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
         return launch_ksw_group(st, it.param, it.kind == kKswRev, b->d_arena, b->d_ref, (const KswJob *)p->d_jobs,
                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->ksc,
                                 (bwg_ksw_result *)b->d_out, p->d_rowmax, it.grid);
+    case kGlbGroup:
+        return launch_glb_group(st, it.param, b->d_arena, b->d_ref, (const GlobalJob *)p->d_jobs,
+                                p->d_order + it.offset, it.count, p->d_heads + it.head, b->gsc, it.qcap, b->d_z,
+                                (int32_t *)b->d_out, it.grid);
+    case kGlbTrace:
+        return launch_glb_traceback(st, (const GlobalJob *)p->d_jobs, it.count, b->d_z, b->d_ops, b->d_nops);
     case kKswMateRc:
         return launch_mate_revcomp(st, b->d_arena, b->d_anchors, b->n_anchors);
     case kKswWindows:
@@ -755,7 +823,7 @@ int plan_batch(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
     Plan *p = new Plan;
     b->plan = p;
     p->esc = b->esc;
-    int rc = b->kind == 0 ? plan_ext(c, b, p, st) : plan_ksw(c, b, p, st);
+    int rc = b->kind == 0 ? plan_ext(c, b, p, st) : b->kind == 1 ? plan_ksw(c, b, p, st) : plan_glb(c, b, p, st);
     for (int r = 1; r < b->side_tries && !rc; ++r) {
         Plan *q = new Plan;
         b->retry_plans.push_back(q);
@@ -831,6 +899,8 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     b->batched_runs++;
     return bwg_map_hip(e);
 }
+
+const void *glb_plan_jobs(const bwg_batch *b) { return b->plan ? ((const Plan *)b->plan)->d_jobs : nullptr; }
 
 int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
     *hits = 0;

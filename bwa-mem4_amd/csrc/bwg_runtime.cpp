@@ -253,6 +253,77 @@ int stage_matesw(bwg_ctx *c, const KswScoring &sc, const MateRescueParams &mp, s
     return BWG_OK;
 }
 
+int stage_global(bwg_ctx *c, const GlobalScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
+                 const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                 const int32_t *w, hipStream_t st, bwg_batch **out) {
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    bwg_batch *b = new_batch(2, n);
+    if (!b) return BWG_ENOMEM;
+    b->gsc = sc;
+    b->hglb.resize(n);
+    for (size_t k = 0; k < n; ++k) {
+        GlobalJob &j = b->hglb[k];
+        j.q_off = q_off[k];
+        j.t_off = t_off[k];
+        j.qlen = q_len[k];
+        j.tlen = t_len[k];
+        // a band wider than both sequences changes nothing: keep w where band arithmetic cannot
+        // overflow
+        j.w = (int32_t)std::min<int64_t>(std::max(w[k], 0), (int64_t)q_len[k] + t_len[k] + 1);
+        j.k = 0;
+        j.z_off = j.ops_off = 0;
+    }
+    int rc = stage_buffers(c, b, arena, arena_len, nullptr, 0, n * 4, st);
+    if (rc) {
+        destroy_batch(c, b);
+        return rc;
+    }
+    *out = b;
+    return BWG_OK;
+}
+
+int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops) {
+    if (b->n == 0) return BWG_OK;
+    if (!b->d_nops) return BWG_EINVAL;  // not run yet
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    hipError_t e = hipMemcpyAsync(n_ops, b->d_nops, b->n * 4, hipMemcpyDeviceToHost, c->ss.main);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    if (e != hipSuccess) return bwg_map_hip(e);
+    for (size_t k = 0; k < b->n; ++k)
+        if (n_ops[k] == 0xffffffffu) return BWG_EINVAL;  // no alignment path inside the band
+    return BWG_OK;
+}
+
+int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap) {
+    if (b->n == 0) return BWG_OK;
+    std::vector<uint32_t> cnt(b->n);
+    int rc = fetch_cigar_lengths(c, b, cnt.data());
+    if (rc) return rc;
+    std::vector<uint64_t> off(b->n);
+    uint64_t total = 0;
+    for (size_t k = 0; k < b->n; ++k) {
+        off[k] = total;
+        total += cnt[k];
+    }
+    if (total > cap) return BWG_E2BIG;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    uint64_t *d_off = (uint64_t *)c->pool.alloc(b->n * 8);
+    uint32_t *d_dst = (uint32_t *)c->pool.alloc(std::max<uint64_t>(1, total) * 4);
+    hipError_t e = (d_off && d_dst) ? hipSuccess : hipErrorOutOfMemory;
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), b->n * 8, hipMemcpyHostToDevice, c->ss.main);
+    if (e == hipSuccess)
+        e = launch_glb_cigar_gather(c->ss.main, (const GlobalJob *)glb_plan_jobs(b), (uint32_t)b->n, b->d_ops,
+                                    b->d_nops, d_off, d_dst);
+    if (e == hipSuccess && total)
+        e = hipMemcpyAsync(ops, d_dst, total * 4, hipMemcpyDeviceToHost, c->ss.main);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    c->pool.free(d_off);
+    c->pool.free(d_dst);
+    return bwg_map_hip(e);
+}
+
 int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win) {
     if (b->n == 0) return BWG_OK;
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
@@ -362,9 +433,9 @@ int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
 
 int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
     if (b->n == 0) return BWG_OK;
-    // band-retry batches keep their rounds on the device, and rescue batches compute their windows
-    // there; only the batched form does either
-    if (form == BWG_FORM_PLAIN && !b->side_tries && !b->matesw) {
+    // band-retry batches keep their rounds on the device, rescue-window batches compute their
+    // windows there, and global alignment has only the batched kernels
+    if (form == BWG_FORM_PLAIN && !b->side_tries && !b->matesw && b->kind != 2) {
         int rc = prepare_plain(c, b, ss.main);
         if (rc) return rc;
         hipError_t e = hipEventRecord(b->ev_start, ss.main);
@@ -408,7 +479,8 @@ int run_batch(bwg_ctx *c, bwg_batch *b) {
 int enqueue_results(bwg_ctx *c, bwg_batch *b, void *out, hipStream_t st) {
     (void)c;
     if (b->n == 0) return BWG_OK;
-    const size_t bytes = b->n * (b->kind == 0 ? sizeof(bwg_ext_result) : sizeof(bwg_ksw_result));
+    const size_t bytes =
+        b->n * (b->kind == 0 ? sizeof(bwg_ext_result) : b->kind == 1 ? sizeof(bwg_ksw_result) : sizeof(int32_t));
     return bwg_map_hip(hipMemcpyAsync(out, b->d_out, bytes, hipMemcpyDeviceToHost, st));
 }
 
@@ -435,6 +507,9 @@ void destroy_batch(bwg_ctx *c, bwg_batch *b) {
     c->pool.free(b->d_anchors);
     c->pool.free(b->d_ma_rb);
     c->pool.free(b->d_win);
+    c->pool.free(b->d_z);
+    c->pool.free(b->d_ops);
+    c->pool.free(b->d_nops);
     if (b->ref_owner) b->ref_owner->ref_users--;
     release_batched(c, b);
     if (b->ev_start) (void)hipEventDestroy(b->ev_start);

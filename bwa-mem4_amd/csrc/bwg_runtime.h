@@ -86,13 +86,18 @@ struct bwg_ctx {
 };
 
 struct bwg_batch {
-    int kind = 0;  // 0 extension, 1 rescue
+    int kind = 0;  // 0 extension, 1 rescue, 2 global alignment with traceback
     size_t n = 0;
     bwg::ExtScoring esc{};
     bwg::KswScoring ksc{};
     // host copies of the job descriptors, used to plan the launches
     std::vector<bwg::ExtJob> hext;
     std::vector<bwg::KswJob> hksw;
+    std::vector<bwg::GlobalJob> hglb;
+    bwg::GlobalScoring gsc{};
+    uint32_t *d_z = nullptr;      // global: traceback codes
+    uint32_t *d_ops = nullptr;    // global: CIGAR runs back to front, per-job scratch
+    uint32_t *d_nops = nullptr;   // global: runs per job
     // device buffers
     uint8_t *d_arena = nullptr;  // query bytes, and target bytes unless targets come from a reference
     size_t arena_len = 0;
@@ -149,6 +154,12 @@ int stage_matesw(bwg_ctx *c, const KswScoring &sc, const MateRescueParams &mp, s
                  const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma, const uint8_t *arena,
                  size_t arena_len, hipStream_t st, bwg_batch **out);
 int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win);
+// Banded global alignment with traceback: n jobs with their own bands.
+int stage_global(bwg_ctx *c, const GlobalScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
+                 const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                 const int32_t *w, hipStream_t st, bwg_batch **out);
+int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops);
+int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap);
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out);
@@ -166,6 +177,8 @@ size_t batch_device_bytes(int kind, size_t n, size_t arena_len, const uint32_t *
 
 // batched form (bwg_batched.cpp)
 int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches);
+// The planned descriptors of a global-alignment batch (device), null before its first run.
+const void *glb_plan_jobs(const bwg_batch *b);
 void release_batched(bwg_ctx *c, bwg_batch *b);
 // Jobs of the last batched run that the ungapped fast path resolved (0 when it did not run).
 int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits);

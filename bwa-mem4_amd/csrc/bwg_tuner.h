@@ -15,7 +15,7 @@ namespace bwg {
 
 // Kernel mappings a bin can be run with.
 enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain, kExtUngapped,
-                kExtRetryFilter, kKswMateRc, kKswWindows };
+                kExtRetryFilter, kKswMateRc, kKswWindows, kGlbGroup, kGlbTrace };
 
 struct TuneCandidate {
     int kind = 0;

@@ -176,6 +176,30 @@ int bwg_batch_stage_side(bwg_ctx *ctx, const bwg_ext_params *p, int max_band_try
                          const int32_t *h0, const int32_t *prev, bwg_batch **out);
 int bwg_batch_side_bands(bwg_ctx *ctx, bwg_batch *b, int32_t *w_out);
 
+/* ---- banded global alignment with traceback -------------------------------------------------
+ * Replaces ksw_global2 (crates/bwa-extend/src/sw.rs:561-740), called per alignment region at
+ * crates/bwa-mem/src/cigar.rs:446 to turn a settled region into its CIGAR: end-to-end alignment of
+ * all of the query against all of the target inside the band |j - i| <= w, gaps opened from
+ * aligned pairs, unreachable cells at MINUS_INF, ties M over E over F, traceback from the band's
+ * last cell.  Each job has its own band w[k] (the caller's clamp, cigar.rs:417-444), which must
+ * cover the length difference (w[k] >= |qlen - tlen|, as that clamp guarantees); qlen <= 512.
+ * score[k] is the reference's score; the CIGAR words are len << 4 | op (0 = M, 1 = I, 2 = D), the
+ * runs of job k following those of job k-1 in `cigar`, n_cigar[k] of them. */
+typedef struct { int32_t m; int8_t mat[25]; int32_t o_del, e_del, o_ins, e_ins; } bwg_global_params;
+/* BWG_E2BIG when cigar_cap is smaller than the total number of runs (n_cigar is filled anyway). */
+int bwg_global_batch(bwg_ctx *ctx, const bwg_global_params *p, size_t n, const uint8_t *arena,
+                     size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                     const uint64_t *t_off, const uint32_t *t_len, const int32_t *w, int32_t *score,
+                     uint32_t *n_cigar, uint32_t *cigar, size_t cigar_cap);
+/* Staged form: run with bwg_batch_run, scores (n int32) with bwg_batch_results, then the run
+ * counts and the runs. */
+int bwg_batch_stage_global(bwg_ctx *ctx, const bwg_global_params *p, size_t n, const uint8_t *arena,
+                           size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                           const uint64_t *t_off, const uint32_t *t_len, const int32_t *w,
+                           bwg_batch **out);
+int bwg_batch_cigar_lengths(bwg_ctx *ctx, bwg_batch *b, uint32_t *n_cigar);
+int bwg_batch_cigars(bwg_ctx *ctx, bwg_batch *b, uint32_t *cigar, size_t cigar_cap);
+
 /* ---- mate rescue with device-resident windows -----------------------------------------------
  * Replaces one round of the reference's batched mate rescue (crates/bwa-mem/src/pe.rs:1040-1113):
  * matesw_collect (pe.rs:596-679) with its reference fetch bns_fetch_seq (pe.rs:211-269), the flat

@@ -18,6 +18,7 @@
  *   orc_ungapped_*         crates/bwa-neon/src/batched.rs:493-521, 602-697, 756-764 (ksw_ungapped.c)
  *   orc_extend_side        crates/bwa-mem/src/lib.rs:332-373     (extend_side; ksw_side.c)
  *   orc_run_side           crates/bwa-mem/src/across.rs:1357-1460 (run_side; ksw_side.c)
+ *   orc_ksw_global2        crates/bwa-extend/src/sw.rs:515-740 (ksw_global2, push_cigar; ksw_global.c)
  *   orc_matesw_*           crates/bwa-mem/src/pe.rs:192-269, 596-679, 1040-1113 and
  *                          crates/bwa-index/src/bntseq.rs:237-243, 337-367 (mate rescue windows;
  *                          ksw_matesw.c)
@@ -177,6 +178,17 @@ void orc_matesw_batch(int64_t l_pac, const uint8_t *fwd, const int64_t *off, con
                       size_t n, const orc_mate_anchor *anchors, const int64_t *ma_rb, const uint8_t *arena,
                       const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t minsc,
                       int32_t max_sc, orc_rescue_window *win, orc_ksw_result *out, int nthreads);
+
+/* Banded global alignment with traceback (ksw_global.c).  cigar needs qlen + tlen + 1 words. */
+int32_t orc_ksw_global2(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target, int32_t m,
+                        const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t w,
+                        uint32_t *cigar, int32_t *n_cigar);
+void orc_global_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                      const uint64_t *t_off, const uint32_t *t_len, const int32_t *w, const int8_t *mat,
+                      int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t *score,
+                      const uint64_t *cig_off, uint32_t *cigar, int32_t *n_cigar, int nthreads);
+int32_t orc_brute_global(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target, const int8_t *mat,
+                         int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins);
 
 #ifdef __cplusplus
 }

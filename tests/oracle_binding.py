@@ -110,6 +110,13 @@ def lib():
         L.orc_matesw_batch.restype = None
         L.orc_matesw_batch.argtypes = [C.c_int64, _P, _P, _P, _I, _P, _P, _P, _I, C.c_size_t, _P, _P, _P, _P,
                                        _I, _I, _I, _I, _I, _I, _P, _P, C.c_int]
+        L.orc_ksw_global2.restype = _I
+        L.orc_ksw_global2.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _P, _P]
+        L.orc_global_batch.restype = None
+        L.orc_global_batch.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _I, _I, _I, _I, _P, _P, _P, _P,
+                                       C.c_int]
+        L.orc_brute_global.restype = _I
+        L.orc_brute_global.argtypes = [_I, _P, _I, _P, _P, _I, _I, _I, _I]
         L.orc_run_side.restype = None
         L.orc_run_side.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I,
                                    _I, _P, _P, C.c_int]
@@ -220,6 +227,44 @@ def matesw_batch(rnd, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, nthreads=8
                            rnd.anchors.ctypes.data, rnd.ma_rb.ctypes.data, rnd.arena.ctypes.data, mp, o_del, e_del,
                            o_ins, e_ins, minsc, max_sc, win.ctypes.data, out.ctypes.data, nthreads)
     return win, out
+
+
+def global2(q, t, mat, o_del, e_del, o_ins, e_ins, w):
+    """ksw_global2: (score, CIGAR words len << 4 | op, op 0 = M, 1 = I, 2 = D)."""
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    cig = np.zeros(len(qa) + len(ta) + 2, dtype=np.uint32)
+    n = C.c_int32(0)
+    sc = lib().orc_ksw_global2(len(qa), qp, len(ta), tp, 5, mp, o_del, e_del, o_ins, e_ins, w, cig.ctypes.data,
+                               C.addressof(n))
+    return sc, [int(x) for x in cig[:n.value]]
+
+
+def global_batch(batch, w, mat, o_del, e_del, o_ins, e_ins, nthreads=8):
+    """ksw_global2 over a batch with per-job bands: (scores, n_cigar, cigar words concatenated)."""
+    n = len(batch.q_len)
+    w = np.ascontiguousarray(w, dtype=np.int32)
+    cap = batch.q_len.astype(np.uint64) + batch.t_len + 2
+    off = np.zeros(n, dtype=np.uint64)
+    if n:
+        off[1:] = np.cumsum(cap)[:-1]
+    buf = np.zeros(int(cap.sum()) + 1, dtype=np.uint32)
+    score = np.zeros(n, dtype=np.int32)
+    ncig = np.zeros(n, dtype=np.int32)
+    ma, mp = _mat(mat)
+    lib().orc_global_batch(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                           batch.t_off.ctypes.data, batch.t_len.ctypes.data, w.ctypes.data, mp, o_del, e_del, o_ins,
+                           e_ins, score.ctypes.data, off.ctypes.data, buf.ctypes.data, ncig.ctypes.data, nthreads)
+    parts = [buf[int(off[k]):int(off[k]) + int(ncig[k])] for k in range(n)]
+    return score, ncig, (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
+
+
+def brute_global(q, t, mat, o_del, e_del, o_ins, e_ins):
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    return lib().orc_brute_global(len(qa), qp, len(ta), tp, mp, o_del, e_del, o_ins, e_ins)
 
 
 def local_fwd(q, t, mat, o_del, e_del, o_ins, e_ins, minsc, endsc, max_sc, lanes):
