@@ -3,14 +3,15 @@
 // Two persistent kernels, each draining a device work queue of length-binned jobs:
 //
 //  * ext_group_kernel<K>: one job per 16-lane DPP row ("group"), four jobs per wavefront.  Row i
-//    of the DP is evaluated in parallel over the live window [beg, end): lane g owns the K
-//    consecutive window columns beg + g*K .. beg + g*K + K-1 (window-relative blocking; K odd keeps
-//    the 16 lanes' LDS accesses in distinct banks).  The eh arrays of the reference live in LDS over
-//    absolute columns 0..qlen, so columns outside the window keep their stale values exactly as in
-//    the reference.  M and E depend only on the previous row; F is the max-plus recurrence
+//    of the DP is evaluated in parallel over the live window [beg, end): lane g owns the kr
+//    consecutive window columns beg + g*kr .. beg + g*kr + kr-1 (window-relative blocking), where
+//    kr <= K is chosen per row as the smallest odd width that covers the widest window among the
+//    wavefront's four groups (odd keeps the 16 lanes' LDS accesses in distinct banks; K bounds
+//    the registers).  The eh arrays of the reference live in LDS over absolute columns 0..qlen, so
+//    columns outside the window keep their stale values exactly as in the reference.  M and E depend only on the previous row; F is the max-plus recurrence
 //    F(j+1) = max(F(j) - e_ins, max(M(j) - oe_ins, 0)), evaluated in two passes: each lane runs it
 //    over its block from F_in = 0, the blocks are combined by an exclusive max-scan across the row
-//    (F(l) = max_{m<l} c_m - (l-1-m)*K*e_ins), and a second pass applies each lane's incoming F.
+//    (F(l) = max_{m<l} c_m - (l-1-m)*kr*e_ins), and a second pass applies each lane's incoming F.
 //    Row maximum with the later-column tie rule is a max over keys (h << 13 | j).  The row
 //    epilogue (to-end score, z-drop, band tightening) is evaluated redundantly by the 16 lanes.
 //
@@ -73,15 +74,93 @@ __device__ __forceinline__ void fill_subst_table(int8_t *tbl, const ExtScoring &
     if (t < 40) tbl[t] = (t & 7) < 5 ? sc.mat[(t >> 3) * 5 + (t & 7)] : (int8_t)0;
 }
 
-// Block LDS layout (bytes): the eh pairs {H[j], E[j]} of every group, (qcap+2) columns each; then
-// every group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes);
-// then slack.  A lane always loads all K columns of its block, including columns past the window
-// (their values are not used); those loads may run past the group's own region into the next
-// one, and the slack keeps the last group's loads inside the allocation.
+// Block LDS layout (bytes): the eh pairs {H[j], E[j]} of every group, qcap + 2 columns each; then
+// every group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes); then slack.  A lane
+// always loads all columns of its block, including columns past the window (their values are not
+// used); those loads may run past the group's own region into the next one, and the slack keeps
+// the last group's loads inside the allocation.
 __host__ __device__ __forceinline__ int eh_bytes(int qcap) { return 8 * (qcap + 2); }
 __host__ __device__ __forceinline__ int plane_bytes(int qcap) { return 5 * qcap; }
 __host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap, int K) {
     return (size_t)groups * (eh_bytes(qcap) + plane_bytes(qcap)) + 16 * K * 8 + 64;
+}
+
+// The cells of one DP row for one group, with the row's window spread over the 16 lanes in blocks
+// of KR columns: lane g owns window columns base .. base + KR - 1, base = beg + g*KR.  Returns
+// the lane's row-maximum key (h << 13 | absolute column), 0 when the lane has no window column.
+// Only window columns store: the columns past the window keep their stale eh values, which a
+// later row reads when the band widens again (by up to two columns a row), as the reference's do.
+template <int KR>
+__device__ __forceinline__ int32_t row_cells(int2 *eh2, const int8_t *srow, int32_t beg, int32_t end, int g,
+                                             int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
+    const int32_t base = beg + g * KR;  // first window column of this lane
+    const int32_t nval = min(max(end - base, 0), KR);
+    const int2 *ehb = eh2 + base;
+    const int8_t *sb = srow + base;
+    // pass 1: the loads are unconditional and independent, so they issue back to back; columns
+    // past the window compute values that are never used
+    int2 v[KR];
+    int32_t s[KR];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        v[k] = ehb[k];
+        s[k] = sb[k];
+    }
+    // F from 0 at the block's first column, carried with its drift: G_k = F_k + k*e_ins, so
+    // F_{k+1} = max(F_k - e_ins, M_k - oe_ins, 0) becomes G_{k+1} = max(G_k, M_k + (k+1)*e_ins -
+    // oe_ins, (k+1)*e_ins), one add and one three-way max per column
+    int32_t mm[KR], gg[KR];
+    int32_t grun = 0;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        // M = H ? H + s : 0, as s*min(H, 1) + H (H >= 0): two operations, where a compare and
+        // a select would take three (the min is opaque so that the compiler keeps this form)
+        uint32_t z;
+        asm("v_min_u32 %0, %1, 1" : "=v"(z) : "v"(v[k].x));
+        const int32_t m = __mul24(s[k], (int32_t)z) + v[k].x;
+        mm[k] = m;
+        gg[k] = grun;
+        grun = max(max(grun, m + ((k + 1) * e_ins - oe_ins)), (k + 1) * e_ins);
+    }
+    grun -= KR * e_ins;
+    // F entering this lane's block: exclusive max-scan of U_l = c_l + l*KR*e_ins over the row
+    const int32_t Ke = KR * e_ins;
+    const int32_t U = grun + g * Ke;
+    const int32_t incl = row_scan_max_nonneg(U);
+    const int32_t excl = row_shift_in(incl, -Ke);
+    const int32_t f_in = excl - (g - 1) * Ke;
+
+    int32_t key = 0;
+    int2 *ehw = eh2 + base;
+#pragma unroll
+    for (int k = 0; k < KR; ++k) {
+        const int32_t f = max(f_in, gg[k]) - k * e_ins;
+        const int32_t h = max(max(mm[k], v[k].y), f);
+        const int32_t e = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
+        if (k < nval) {
+            // lane-relative column; the lane's first column is added once below
+            key = max(key, (h << kKeyShift) | k);
+            ehw[k].y = e;      // E(i+1, j)
+            ehw[k + 1].x = h;  // H(i, j): the diagonal of column j+1 next row
+        }
+    }
+    return nval > 0 ? key + base : 0;
+}
+
+// row_cells at the odd block width kr (lo <= kr <= hi, both odd), chosen by a binary search
+// over the wavefront-uniform kr so that each row runs one straight-line body.
+template <int LO, int HI>
+__device__ __forceinline__ int32_t row_cells_at(int kr, int2 *eh2, const int8_t *srow, int32_t beg, int32_t end,
+                                                int g, int32_t e_ins, int32_t oe_ins, int32_t e_del,
+                                                int32_t oe_del) {
+    if constexpr (LO >= HI) {
+        return row_cells<HI>(eh2, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+    } else {
+        constexpr int MID = LO + ((HI - LO) / 4) * 2;  // odd, LO <= MID < HI
+        if (kr <= MID)
+            return row_cells_at<LO, MID>(kr, eh2, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells_at<MID + 2, HI>(kr, eh2, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+    }
 }
 
 constexpr int kGroupsPerBlock = 16;
@@ -104,9 +183,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
 
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
-    const int32_t Ke = K * e_ins;
 
-    GroupJob J;
+    GroupJob J = {};
     bool active = false, exhausted = false;
     for (;;) {
         // ---- fetch and initialise a job for idle groups ------------------------------------
@@ -153,65 +231,34 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         wave_lds_sync();
         // the wavefront exits once none of its four groups has work left
         if (__ballot(active || !exhausted) == 0) break;
-        if (!active) continue;
 
         // ---- one DP row ------------------------------------------------------------------------
         const int32_t i = J.i;
         int32_t beg = J.beg, end = J.end;
-        if (beg < i - J.w) beg = i - J.w;
-        if (end > i + J.w + 1) end = i + J.w + 1;
-        if (end > J.qlen) end = J.qlen;
+        if (active) {
+            if (beg < i - J.w) beg = i - J.w;
+            if (end > i + J.w + 1) end = i + J.w + 1;
+            if (end > J.qlen) end = J.qlen;
+        }
+        // Block width of this row: the widest live window among the wavefront's four groups, in
+        // an odd number of columns per lane (at most K, since the bin's K covers the job's widest
+        // possible window).  The band narrows as rows go by, and the wavefront's time per row
+        // follows its widest group, so the row's columns are spread over every lane again.
+        const int32_t wwin = active ? max(end - beg, 0) : 0;
+        const int32_t wmax = max(max(__builtin_amdgcn_readlane(wwin, 0), __builtin_amdgcn_readlane(wwin, 16)),
+                                 max(__builtin_amdgcn_readlane(wwin, 32), __builtin_amdgcn_readlane(wwin, 48)));
+        const int32_t kr = min(((wmax + 15) >> 4) | 1, K);
+        if (!active) continue;
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
         const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
-
-        const int32_t base = beg + g * K;  // first window column of this lane
-        const int32_t nval = min(max(end - base, 0), K);
-        const int2 *ehb = (const int2 *)eh + base;
-        const int8_t *sb = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap) + base;
-        // pass 1: loads of all K columns are unconditional and independent, so they issue back to
-        // back; columns past the window compute values that are never used
-        int2 v[K];
-        int32_t s[K];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            v[k] = ehb[k];
-            s[k] = sb[k];
-        }
-        int32_t mm[K], gg[K];
-        int32_t grun = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int32_t m = v[k].x != 0 ? v[k].x + s[k] : 0;
-            mm[k] = m;
-            gg[k] = grun;
-            grun = max(max(grun - e_ins, m - oe_ins), 0);
-        }
-        // F entering this lane's block: exclusive max-scan of U_l = c_l + l*K*e_ins over the row
-        const int32_t U = grun + g * Ke;
-        const int32_t incl = row_scan_max_nonneg(U);
-        const int32_t excl = row_shift_in(incl, -Ke);
-        const int32_t f_in = excl - (g - 1) * Ke;
-
-        int32_t key = 0;
-        int2 *ehw = (int2 *)eh + base;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            const int32_t f = max(f_in - k * e_ins, gg[k]);
-            const int32_t h = max(max(mm[k], v[k].y), f);
-            const int32_t e = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
-            if (k < nval) {
-                // lane-relative column; the lane's first column is added once after the loop
-                key = max(key, (h << kKeyShift) | k);
-                ehw[k].y = e;       // E(i+1, j)
-                ehw[k + 1].x = h;   // H(i, j): the diagonal of column j+1 next row
-            }
-        }
+        const int8_t *srow = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap);
+        int32_t key = row_cells_at<1, K>(kr, (int2 *)eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
         if (g == 0) {
             eh[2 * beg] = h1_init;  // H(i, beg-1)
             eh[2 * end + 1] = 0;    // E at the sentinel column
         }
         wave_lds_sync();
-        key = row_allmax(nval > 0 ? key + base : 0);  // h << 13 | absolute column
+        key = row_allmax(key);  // h << 13 | absolute column
         const int32_t row_max = key >> kKeyShift;
         const int32_t mj = key & kKeyMask;
         const int2 at_end = *(const int2 *)(eh + 2 * end);  // H(i, end-1) (h1_init if empty)
