@@ -171,7 +171,7 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
 // more than the bin has work for.
 int ext_grid(const bwg_ctx *c, int kind, int param, bool narrow, int qcap, uint32_t count) {
     if (kind == kExtGroup) {
-        const int per_cu = ext_group_blocks_per_cu(param, qcap);
+        const int per_cu = ext_group_blocks_per_cu(param, qcap, narrow);
         return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ext_group_oversub,
                                                            ((int64_t)count + 15) / 16));
     }
@@ -216,8 +216,9 @@ std::vector<TuneCandidate> ext_candidates(int cls) {
         add(kExtThread, qmax <= 16 ? 16 : 32, 32);
     } else {
         const int r = cls - kExtShortClasses;
+        add(kExtGroup, kExtKs[r], 16);
         add(kExtGroup, kExtKs[r], 32);
-        if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1], 32);
+        if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1], 16);
     }
     return v;
 }
@@ -255,7 +256,7 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, const ExtScoring &esc, uint64_t sig
             const int grid = ext_grid(c, cd.kind, cd.param, narrow, qcap, S);
             if (e == hipSuccess) {
                 if (cd.kind == kExtGroup)
-                    e = launch_ext_group(st, cd.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S,
+                    e = launch_ext_group(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S,
                                          nullptr, d, esc, qcap, (bwg_ext_result *)b->d_out, grid);
                 else
                     e = launch_ext_thread(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1,
@@ -344,7 +345,6 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             const int r = std::min(kidx[k] + (tu.ext_k_step ? 1 : 0), kNumExtKs - 1);
             kind = kExtGroup, param = kExtKs[r];
         }
-        if (kind == kExtGroup) narrow = false;  // the group kernel keeps 32-bit state
         const int bi = find(kind, param, narrow);
         if (kind == kExtGroup) bins[bi].qcap = std::max(bins[bi].qcap, qcap_class(q));
         return bi;
@@ -727,7 +727,7 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
         const uint32_t *order = p->compact ? p->d_corder + it.offset : p->d_order + it.offset;
         const uint32_t *dcount = p->compact ? p->d_heads + p->n_bins + kCountStride * it.head : nullptr;
         if (it.kind == kExtGroup)
-            return launch_ext_group(st, it.param, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order, it.count,
+            return launch_ext_group(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order, it.count,
                                     dcount, p->d_heads + it.head, p->esc, it.qcap, (bwg_ext_result *)b->d_out,
                                     it.grid);
         return launch_ext_thread(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order,

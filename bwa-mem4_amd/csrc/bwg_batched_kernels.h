@@ -10,14 +10,14 @@
 namespace bwg {
 
 // Extension kernels.  The group kernel sizes its per-row block width itself (at most K columns per
-// lane); the thread kernel's `narrow` selects 16-bit column state (valid when h0 + qlen*max_sc <
+// lane); `narrow` selects 16-bit column state in both kernels (valid when h0 + qlen*max_sc <
 // 65536).
-size_t ext_group_lds_bytes(int K, int qcap);
-int ext_group_blocks_per_cu(int K, int qcap);
+size_t ext_group_lds_bytes(int K, int qcap, bool narrow);
+int ext_group_blocks_per_cu(int K, int qcap, bool narrow);
 // `count` jobs of `order` (or *dcount of them when dcount is not null: a compacted queue).
-hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
-                            const uint32_t *order, uint32_t count, const uint32_t *dcount, uint32_t *head,
-                            const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid);
+hipError_t launch_ext_group(hipStream_t st, int K, bool narrow, const uint8_t *arena, const uint8_t *ref,
+                            const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
+                            uint32_t *head, const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid);
 int ext_thread_blocks_per_cu(int qcap, bool narrow);
 hipError_t launch_ext_thread(hipStream_t st, int qcap, bool narrow, const uint8_t *arena, const uint8_t *ref,
                              const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,

@@ -74,15 +74,42 @@ __device__ __forceinline__ void fill_subst_table(int8_t *tbl, const ExtScoring &
     if (t < 40) tbl[t] = (t & 7) < 5 ? sc.mat[(t >> 3) * 5 + (t & 7)] : (int8_t)0;
 }
 
-// Block LDS layout (bytes): the eh pairs {H[j], E[j]} of every group, qcap + 2 columns each; then
-// every group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes); then slack.  A lane
+// The column state of one group, the reference's eh arrays over absolute columns 0..qcap+1.  Wide:
+// {H[j], E[j]} pairs of int32.  Narrow: H and E as two arrays of uint16, exact whenever H and E
+// stay below 65536; they never exceed h0 + qlen*max_sc, which the planner checks for narrow bins.
+// Narrow state halves the group's LDS, which is what limits the kernel's occupancy for wide
+// windows.
+template <bool NARROW> struct EhView;
+template <> struct EhView<false> {
+    int2 *p;
+    __device__ __forceinline__ EhView(void *base, int) : p((int2 *)base) {}
+    __device__ __forceinline__ int2 load(int j) const { return p[j]; }
+    __device__ __forceinline__ void set_h(int j, int32_t v) const { p[j].x = v; }
+    __device__ __forceinline__ void set_e(int j, int32_t v) const { p[j].y = v; }
+    __device__ __forceinline__ void init(int j, int32_t h) const { p[j] = make_int2(h, 0); }
+};
+template <> struct EhView<true> {
+    uint16_t *hp, *ep;
+    __device__ __forceinline__ EhView(void *base, int qcap)
+        : hp((uint16_t *)base), ep((uint16_t *)base + (qcap + 2)) {}
+    __device__ __forceinline__ int2 load(int j) const { return make_int2(hp[j], ep[j]); }
+    __device__ __forceinline__ void set_h(int j, int32_t v) const { hp[j] = (uint16_t)v; }
+    __device__ __forceinline__ void set_e(int j, int32_t v) const { ep[j] = (uint16_t)v; }
+    __device__ __forceinline__ void init(int j, int32_t h) const {
+        hp[j] = (uint16_t)h;
+        ep[j] = 0;
+    }
+};
+
+// Block LDS layout (bytes): the column state of every group, qcap + 2 columns each; then every
+// group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes); then slack.  A lane
 // always loads all columns of its block, including columns past the window (their values are not
 // used); those loads may run past the group's own region into the next one, and the slack keeps
 // the last group's loads inside the allocation.
-__host__ __device__ __forceinline__ int eh_bytes(int qcap) { return 8 * (qcap + 2); }
+__host__ __device__ __forceinline__ int eh_bytes(int qcap, bool narrow) { return (narrow ? 4 : 8) * (qcap + 2); }
 __host__ __device__ __forceinline__ int plane_bytes(int qcap) { return 5 * qcap; }
-__host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap, int K) {
-    return (size_t)groups * (eh_bytes(qcap) + plane_bytes(qcap)) + 16 * K * 8 + 64;
+__host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap, int K, bool narrow) {
+    return (size_t)groups * (eh_bytes(qcap, narrow) + plane_bytes(qcap)) + 16 * K * 8 + 64;
 }
 
 // The cells of one DP row for one group, with the row's window spread over the 16 lanes in blocks
@@ -90,12 +117,11 @@ __host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap,
 // the lane's row-maximum key (h << 13 | absolute column), 0 when the lane has no window column.
 // Only window columns store: the columns past the window keep their stale eh values, which a
 // later row reads when the band widens again (by up to two columns a row), as the reference's do.
-template <int KR>
-__device__ __forceinline__ int32_t row_cells(int2 *eh2, const int8_t *srow, int32_t beg, int32_t end, int g,
+template <int KR, class EH>
+__device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t beg, int32_t end, int g,
                                              int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
     const int32_t base = beg + g * KR;  // first window column of this lane
     const int32_t nval = min(max(end - base, 0), KR);
-    const int2 *ehb = eh2 + base;
     const int8_t *sb = srow + base;
     // pass 1: the loads are unconditional and independent, so they issue back to back; columns
     // past the window compute values that are never used
@@ -103,7 +129,7 @@ __device__ __forceinline__ int32_t row_cells(int2 *eh2, const int8_t *srow, int3
     int32_t s[KR];
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
-        v[k] = ehb[k];
+        v[k] = eh.load(base + k);
         s[k] = sb[k];
     }
     // F from 0 at the block's first column, carried with its drift: G_k = F_k + k*e_ins, so
@@ -131,7 +157,6 @@ __device__ __forceinline__ int32_t row_cells(int2 *eh2, const int8_t *srow, int3
     const int32_t f_in = excl - (g - 1) * Ke;
 
     int32_t key = 0;
-    int2 *ehw = eh2 + base;
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
         const int32_t f = max(f_in, gg[k]) - k * e_ins;
@@ -140,8 +165,8 @@ __device__ __forceinline__ int32_t row_cells(int2 *eh2, const int8_t *srow, int3
         if (k < nval) {
             // lane-relative column; the lane's first column is added once below
             key = max(key, (h << kKeyShift) | k);
-            ehw[k].y = e;      // E(i+1, j)
-            ehw[k + 1].x = h;  // H(i, j): the diagonal of column j+1 next row
+            eh.set_e(base + k, e);      // E(i+1, j)
+            eh.set_h(base + k + 1, h);  // H(i, j): the diagonal of column j+1 next row
         }
     }
     return nval > 0 ? key + base : 0;
@@ -149,23 +174,23 @@ __device__ __forceinline__ int32_t row_cells(int2 *eh2, const int8_t *srow, int3
 
 // row_cells at the odd block width kr (lo <= kr <= hi, both odd), chosen by a binary search
 // over the wavefront-uniform kr so that each row runs one straight-line body.
-template <int LO, int HI>
-__device__ __forceinline__ int32_t row_cells_at(int kr, int2 *eh2, const int8_t *srow, int32_t beg, int32_t end,
+template <int LO, int HI, class EH>
+__device__ __forceinline__ int32_t row_cells_at(int kr, const EH &eh, const int8_t *srow, int32_t beg, int32_t end,
                                                 int g, int32_t e_ins, int32_t oe_ins, int32_t e_del,
                                                 int32_t oe_del) {
     if constexpr (LO >= HI) {
-        return row_cells<HI>(eh2, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells<HI>(eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
     } else {
         constexpr int MID = LO + ((HI - LO) / 4) * 2;  // odd, LO <= MID < HI
         if (kr <= MID)
-            return row_cells_at<LO, MID>(kr, eh2, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
-        return row_cells_at<MID + 2, HI>(kr, eh2, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+            return row_cells_at<LO, MID>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells_at<MID + 2, HI>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
     }
 }
 
 constexpr int kGroupsPerBlock = 16;
 
-template <int K>
+template <int K, bool NARROW>
 __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restrict__ arena,
                                                         const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
@@ -178,8 +203,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;             // lane within the group
     const int gid = threadIdx.x >> 4;    // group within the block
-    int32_t *eh = (int32_t *)((char *)lds + gid * eh_bytes(qcap));
-    int8_t *planes = (int8_t *)lds + kGroupsPerBlock * eh_bytes(qcap) + gid * plane_bytes(qcap);
+    const EhView<NARROW> eh((char *)lds + gid * eh_bytes(qcap, NARROW), qcap);
+    int8_t *planes = (int8_t *)lds + kGroupsPerBlock * eh_bytes(qcap, NARROW) + gid * plane_bytes(qcap);
 
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
@@ -206,7 +231,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 // row -1 (sw.rs:263-279): H[0] = h0, H[j] = max(h0 - oe_ins - (j-1)*e_ins, 0)
                 for (int j = g; j <= J.qlen + 1; j += 16) {
                     const int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
-                    *(int2 *)(eh + 2 * j) = make_int2(j <= J.qlen ? h : 0, 0);
+                    eh.init(j, j <= J.qlen ? h : 0);
                 }
                 for (int j = g; j < J.qlen; j += 16) {
                     const int qc = q[j];
@@ -252,17 +277,17 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
         const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
         const int8_t *srow = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap);
-        int32_t key = row_cells_at<1, K>(kr, (int2 *)eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        int32_t key = row_cells_at<1, K>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
         if (g == 0) {
-            eh[2 * beg] = h1_init;  // H(i, beg-1)
-            eh[2 * end + 1] = 0;    // E at the sentinel column
+            eh.set_h(beg, h1_init);  // H(i, beg-1)
+            eh.set_e(end, 0);        // E at the sentinel column
         }
         wave_lds_sync();
         key = row_allmax(key);  // h << 13 | absolute column
         const int32_t row_max = key >> kKeyShift;
         const int32_t mj = key & kKeyMask;
-        const int2 at_end = *(const int2 *)(eh + 2 * end);  // H(i, end-1) (h1_init if empty)
-        const int2 at_beg = *(const int2 *)(eh + 2 * beg);
+        const int2 at_end = eh.load(end);  // H(i, end-1) (h1_init if empty)
+        const int2 at_beg = eh.load(beg);
         const int32_t h1_last = at_end.x;
 
         bool done = false;
@@ -293,7 +318,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                     const int j = nb + g;
                     bool live = false;
                     if (j < end) {
-                        const int2 x = *(const int2 *)(eh + 2 * j);
+                        const int2 x = eh.load(j);
                         live = (x.x | x.y) != 0;
                     }
                     const int32_t cand = row_allmin(live ? j : 0x7fffffff);
@@ -308,7 +333,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                     const int j = ne - g;
                     bool live = false;
                     if (j >= nb) {
-                        const int2 x = *(const int2 *)(eh + 2 * j);
+                        const int2 x = eh.load(j);
                         live = (x.x | x.y) != 0;
                     }
                     const int32_t cand = row_allmax(live ? j : -1);
@@ -643,26 +668,26 @@ __global__ __launch_bounds__(256) void ext_retry_filter_kernel(
     }
 }
 
-template <int K>
+template <int K, bool NARROW>
 hipError_t launch_group(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs, const uint32_t *order,
                         uint32_t count, const uint32_t *dcount, uint32_t *head, const ExtScoring &sc, int qcap,
                         bwg_ext_result *out, int grid) {
-    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
-    hipLaunchKernelGGL(ext_group_kernel<K>, dim3(grid), dim3(256), lds, st, arena, ref, jobs, order, count,
+    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K, NARROW);
+    hipLaunchKernelGGL((ext_group_kernel<K, NARROW>), dim3(grid), dim3(256), lds, st, arena, ref, jobs, order, count,
                        dcount, head, sc, qcap, out);
     return hipGetLastError();
 }
 
 // Occupancy queries are cached: they depend only on the kernel and its LDS size.
-template <int K>
+template <int K, bool NARROW>
 int group_occupancy(int qcap) {
     static std::atomic<int> cache[130];
     const int slot = std::min(qcap / 32, 129);
     int v = cache[slot].load(std::memory_order_relaxed);
     if (v > 0) return v;
     int blocks = 0;
-    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K>, 256, lds) != hipSuccess)
+    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K, NARROW);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K, NARROW>, 256, lds) != hipSuccess)
         blocks = 1;
     v = blocks > 0 ? blocks : 1;
     cache[slot].store(v, std::memory_order_relaxed);
@@ -684,22 +709,27 @@ int thread_occupancy() {
 
 }  // namespace
 
-size_t ext_group_lds_bytes(int K, int qcap) { return block_lds_bytes(kGroupsPerBlock, qcap, K); }
+size_t ext_group_lds_bytes(int K, int qcap, bool narrow) {
+    return block_lds_bytes(kGroupsPerBlock, qcap, K, narrow);
+}
 
-int ext_group_blocks_per_cu(int K, int qcap) {
+int ext_group_blocks_per_cu(int K, int qcap, bool narrow) {
     switch (K) {
-#define BWG_K(k) case k: return group_occupancy<k>(qcap);
+#define BWG_K(k) case k: return narrow ? group_occupancy<k, true>(qcap) : group_occupancy<k, false>(qcap);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return 1;
     }
 }
 
-hipError_t launch_ext_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
-                            const uint32_t *order, uint32_t count, const uint32_t *dcount, uint32_t *head,
-                            const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid) {
+hipError_t launch_ext_group(hipStream_t st, int K, bool narrow, const uint8_t *arena, const uint8_t *ref,
+                            const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
+                            uint32_t *head, const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid) {
     switch (K) {
-#define BWG_K(k) case k: return launch_group<k>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid);
+#define BWG_K(k)                                                                                              \
+    case k:                                                                                                   \
+        return narrow ? launch_group<k, true>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid) \
+                      : launch_group<k, false>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return hipErrorInvalidValue;
