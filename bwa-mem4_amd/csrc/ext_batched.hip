@@ -92,7 +92,15 @@ template <> struct EhView<true> {
     uint16_t *hp, *ep;
     __device__ __forceinline__ EhView(void *base, int qcap)
         : hp((uint16_t *)base), ep((uint16_t *)base + (qcap + 2)) {}
-    __device__ __forceinline__ int2 load(int j) const { return make_int2(hp[j], ep[j]); }
+    // The widened values are pinned where they are loaded: widening a 16-bit value in a later block
+    // (where the column's use sits behind a branch) costs an AND there, where ds_read_u16 has
+    // already zero-extended it.
+    __device__ __forceinline__ int2 load(int j) const {
+        int32_t h = hp[j], e = ep[j];
+        asm("" : "+v"(h));
+        asm("" : "+v"(e));
+        return make_int2(h, e);
+    }
     __device__ __forceinline__ void set_h(int j, int32_t v) const { hp[j] = (uint16_t)v; }
     __device__ __forceinline__ void set_e(int j, int32_t v) const { ep[j] = (uint16_t)v; }
     __device__ __forceinline__ void init(int j, int32_t h) const {
