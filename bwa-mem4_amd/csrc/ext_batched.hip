@@ -477,14 +477,23 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 // The reference's batched extension first tries a closed-form walk along the main diagonal
 // (crates/bwa-neon/src/batched.rs:602-697, ungapped_hit) and runs the banded DP only for the jobs it
 // declines.  One job per 16-lane group; block b takes the contiguous order-array slots
-// [b*chunk, (b+1)*chunk) in rounds of 16.  Lane g reads columns g, g+16, ... of the query and target
-// (16 consecutive bytes per step, the first 128 columns loaded up front), the group counts
-// mismatches and N bases with ballots and keeps the mismatch columns (< 128) as a 128-bit mask.  An
-// accepted job's result is written directly; the declined ones are collected in slot order and, at
-// the end of the chunk, appended to their bins' compacted queues with one atomic per bin (the
-// counters sit kCountStride words apart, on separate cache lines).  Slot ranges
-// [seg_off[b], seg_off[b+1]) belong to bin b; bins with skip[b] set (the plain bin) are left alone.
+// [b*chunk, (b+1)*chunk), and its 16 groups walk them independently (group q the slots q, q+16,
+// ...), so a group's next job's loads are not held back by the other groups.  Lane g reads columns
+// g, g+16, ... of the query and target (16 consecutive bytes per step, the first 128 columns loaded
+// up front), the group counts mismatches and N bases with ballots and keeps the mismatch columns
+// (< 128) as a 128-bit mask.  An accepted job's result is written directly; a declined one is
+// flagged in LDS and counted for its bin.  At the end of the chunk the declined jobs are appended,
+// in slot order, to their bins' compacted queues with one atomic per bin (the counters sit
+// kCountStride words apart, on separate cache lines).  Slot ranges [seg_off[b], seg_off[b+1])
+// belong to bin b; bins with skip[b] set (the plain bin) are left alone.
 constexpr int kUngappedChunkMax = 4096;
+constexpr int kUngappedBinsMax = 256;
+
+// Bin of `slot`, searching forward from `bin` (slots only grow).
+__device__ __forceinline__ int bin_of_slot(uint32_t slot, int bin, const uint32_t *seg_off, int n_bins) {
+    while (bin + 1 < n_bins && slot >= seg_off[bin + 1]) ++bin;
+    return bin;
+}
 
 __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__restrict__ arena,
                                                            const uint8_t *__restrict__ ref,
@@ -496,27 +505,25 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
                                                            uint32_t *__restrict__ ccount, int32_t a, int32_t b,
                                                            int32_t xmax, int32_t zdrop,
                                                            bwg_ext_result *__restrict__ out) {
-    __shared__ uint32_t s_list[kUngappedChunkMax];
-    __shared__ uint8_t s_lbin[kUngappedChunkMax];
-    __shared__ uint32_t s_decl[16], s_idx[16], s_bin[16];
-    __shared__ uint32_t s_n;
+    __shared__ uint8_t s_flag[kUngappedChunkMax];    // 1: the slot's job was declined
+    __shared__ uint32_t s_bincnt[kUngappedBinsMax];  // declined jobs per bin (bins from the chunk's first)
+    __shared__ uint32_t s_binpos[kUngappedBinsMax];  // their first compacted-queue position
+    __shared__ uint32_t s_wave[4];
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;
     const int gid = threadIdx.x >> 4;
     const uint32_t lo = blockIdx.x * chunk;
     const uint32_t hi = min(n_slots, lo + chunk);
-    if (threadIdx.x == 0) s_n = 0;
-    for (uint32_t base = lo; base < hi; base += 16u) {
-        int bin = 0;  // the round's first slot's bin (block-uniform), then this group's
-        while (bin + 1 < n_bins && base >= seg_off[bin + 1]) ++bin;
-        const uint32_t slot = base + gid;
+    const int bin0 = bin_of_slot(lo, 0, seg_off, n_bins);
+    for (int k = threadIdx.x; k < kUngappedBinsMax; k += blockDim.x) s_bincnt[k] = 0;
+    __syncthreads();
+
+    int bin = bin0;
+    for (uint32_t slot = lo + gid; slot < hi; slot += 16u) {
+        bin = bin_of_slot(slot, bin, seg_off, n_bins);
         bool declined = false;
-        uint32_t idx = 0;
-        if (slot < hi) {
-            while (bin + 1 < n_bins && slot >= seg_off[bin + 1]) ++bin;
-        }
-        if (slot < hi && !skip[bin]) {
-            idx = order[slot];
+        if (!skip[bin]) {
+            const uint32_t idx = order[slot];
             const ExtJob jb = jobs[idx];
             const uint8_t *q = arena + jb.q_off;
             const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
@@ -598,32 +605,46 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
             declined = bail;
         }
         if (g == 0) {
-            s_decl[gid] = declined ? 1u : 0u;
-            s_idx[gid] = idx;
-            s_bin[gid] = (uint32_t)bin;
+            s_flag[slot - lo] = declined ? 1 : 0;
+            if (declined) atomicAdd(&s_bincnt[bin - bin0], 1u);
         }
-        __syncthreads();
-        if (threadIdx.x == 0) {  // append the round's declined jobs in slot order
-            uint32_t n = s_n;
-            for (int k = 0; k < 16; ++k)
-                if (s_decl[k]) {
-                    s_list[n] = s_idx[k];
-                    s_lbin[n] = (uint8_t)s_bin[k];
-                    ++n;
-                }
-            s_n = n;
-        }
-        __syncthreads();
     }
-    if (threadIdx.x == 0) {  // one reservation per bin run of the chunk, then the chunk's entries
-        const uint32_t n = s_n;
-        for (uint32_t i0 = 0; i0 < n;) {
-            uint32_t i1 = i0;
-            const uint32_t bn = s_lbin[i0];
-            while (i1 < n && s_lbin[i1] == bn) ++i1;
-            const uint32_t pos = seg_off[bn] + atomicAdd(&ccount[kCountStride * bn], i1 - i0);
-            for (uint32_t k = i0; k < i1; ++k) corder[pos + (k - i0)] = s_list[k];
-            i0 = i1;
+    __syncthreads();
+
+    // one reservation per bin of the chunk
+    const uint32_t n = hi > lo ? hi - lo : 0;
+    const int nb = n ? bin_of_slot(hi - 1, bin0, seg_off, n_bins) - bin0 + 1 : 0;
+    for (int k = threadIdx.x; k < nb; k += blockDim.x) {
+        const uint32_t c = s_bincnt[k];
+        s_binpos[k] = c ? seg_off[bin0 + k] + atomicAdd(&ccount[kCountStride * (bin0 + k)], c) : 0u;
+    }
+    // exclusive scan of the declined flags in slot order: thread t owns slots [t*per, t*per + per)
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t t0 = min(n, threadIdx.x * per), t1 = min(n, t0 + per);
+    uint32_t cnt = 0;
+    for (uint32_t k = t0; k < t1; ++k) cnt += s_flag[k];
+    uint32_t incl = cnt;  // inclusive scan over the wavefront
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t y = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += y;
+    }
+    if (lane == 63) s_wave[threadIdx.x >> 6] = incl;
+    __syncthreads();
+    uint32_t before = incl - cnt;  // declined slots before t0
+    for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += s_wave[w];
+    // rank within its bin = declined slots before it in the chunk minus those of the earlier bins
+    if (cnt) {
+        int bb = bin_of_slot(lo + t0, bin0, seg_off, n_bins);
+        uint32_t bin_first = 0;  // declined slots of the chunk's bins before bb
+        for (int k = 0; k < bb - bin0; ++k) bin_first += s_bincnt[k];
+        for (uint32_t k = t0; k < t1; ++k) {
+            const int nbb = bin_of_slot(lo + k, bb, seg_off, n_bins);
+            for (; bb < nbb; ++bb) bin_first += s_bincnt[bb - bin0];
+            if (s_flag[k]) {
+                corder[s_binpos[bb - bin0] + (before - bin_first)] = order[lo + k];
+                ++before;
+            }
         }
     }
 }
