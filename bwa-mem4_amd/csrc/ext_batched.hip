@@ -477,8 +477,10 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 // The reference's batched extension first tries a closed-form walk along the main diagonal
 // (crates/bwa-neon/src/batched.rs:602-697, ungapped_hit) and runs the banded DP only for the jobs it
 // declines.  One job per 16-lane group; block b takes the contiguous order-array slots
-// [b*chunk, (b+1)*chunk), and its 16 groups walk them independently (group q the slots q, q+16,
-// ...), so a group's next job's loads are not held back by the other groups.  Lane g reads columns
+// [b*chunk, (b+1)*chunk).  The block first stages the chunk's order entries and then its job
+// descriptors in LDS (two gathers with every load in flight at once, where a group walking its jobs
+// would wait on each job's two dependent loads), and its 16 groups then walk the slots
+// independently (group q the slots q, q+16, ...).  Lane g reads columns
 // g, g+16, ... of the query and target (16 consecutive bytes per step, the first 128 columns loaded
 // up front), the group counts mismatches and N bases with ballots and keeps the mismatch columns
 // (< 128) as a 128-bit mask.  An accepted job's result is written directly; a declined one is
@@ -486,7 +488,7 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 // in slot order, to their bins' compacted queues with one atomic per bin (the counters sit
 // kCountStride words apart, on separate cache lines).  Slot ranges [seg_off[b], seg_off[b+1])
 // belong to bin b; bins with skip[b] set (the plain bin) are left alone.
-constexpr int kUngappedChunkMax = 4096;
+constexpr int kUngappedChunkMax = 512;
 constexpr int kUngappedBinsMax = 256;
 
 // Bin of `slot`, searching forward from `bin` (slots only grow).
@@ -505,6 +507,8 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
                                                            uint32_t *__restrict__ ccount, int32_t a, int32_t b,
                                                            int32_t xmax, int32_t zdrop,
                                                            bwg_ext_result *__restrict__ out) {
+    __shared__ __align__(16) ExtJob s_jobs[kUngappedChunkMax];  // the chunk's descriptors, in slot order
+    __shared__ uint32_t s_idx[kUngappedChunkMax];    // the chunk's order entries
     __shared__ uint8_t s_flag[kUngappedChunkMax];    // 1: the slot's job was declined
     __shared__ uint32_t s_bincnt[kUngappedBinsMax];  // declined jobs per bin (bins from the chunk's first)
     __shared__ uint32_t s_binpos[kUngappedBinsMax];  // their first compacted-queue position
@@ -515,7 +519,13 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
     const uint32_t lo = blockIdx.x * chunk;
     const uint32_t hi = min(n_slots, lo + chunk);
     const int bin0 = bin_of_slot(lo, 0, seg_off, n_bins);
+    const uint32_t n = hi > lo ? hi - lo : 0;
     for (int k = threadIdx.x; k < kUngappedBinsMax; k += blockDim.x) s_bincnt[k] = 0;
+    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s_idx[k] = order[lo + k];
+    __syncthreads();
+    static_assert(sizeof(ExtJob) == 32, "a descriptor is two 16-byte words");
+    for (uint32_t k = threadIdx.x; k < 2 * n; k += blockDim.x)
+        ((uint4 *)s_jobs)[k] = ((const uint4 *)(jobs + s_idx[k >> 1]))[k & 1];
     __syncthreads();
 
     int bin = bin0;
@@ -523,8 +533,8 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
         bin = bin_of_slot(slot, bin, seg_off, n_bins);
         bool declined = false;
         if (!skip[bin]) {
-            const uint32_t idx = order[slot];
-            const ExtJob jb = jobs[idx];
+            const uint32_t idx = s_idx[slot - lo];
+            const ExtJob jb = s_jobs[slot - lo];
             const uint8_t *q = arena + jb.q_off;
             const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
             const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
@@ -612,7 +622,6 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
     __syncthreads();
 
     // one reservation per bin of the chunk
-    const uint32_t n = hi > lo ? hi - lo : 0;
     const int nb = n ? bin_of_slot(hi - 1, bin0, seg_off, n_bins) - bin0 + 1 : 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         const uint32_t c = s_bincnt[k];
@@ -642,7 +651,7 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
             const int nbb = bin_of_slot(lo + k, bb, seg_off, n_bins);
             for (; bb < nbb; ++bb) bin_first += s_bincnt[bb - bin0];
             if (s_flag[k]) {
-                corder[s_binpos[bb - bin0] + (before - bin_first)] = order[lo + k];
+                corder[s_binpos[bb - bin0] + (before - bin_first)] = s_idx[k];
                 ++before;
             }
         }
