@@ -422,11 +422,21 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                 if (end > i + w + 1) end = i + w + 1;
                 if (end > qlen) end = qlen;
                 int32_t h1 = beg == 0 ? max(h0 - (sc.o_del + e_del * (i + 1)), 0) : 0;
+                // the next column's state and query code are loaded a column ahead: column j
+                // stores only slot j, so the loads do not wait on the store and their latency
+                // overlaps this column's arithmetic
                 int32_t j = beg;
+                W vn = j < end ? eh[j * 64 + lane] : W{};
+                uint32_t cn = j < end ? codes[j * 64 + lane] : 0u;
                 for (; j < end; ++j) {
-                    const W v = eh[j * 64 + lane];
+                    const W v = vn;
+                    const uint32_t qc = cn;
+                    if (j + 1 < end) {
+                        vn = eh[(j + 1) * 64 + lane];
+                        cn = codes[(j + 1) * 64 + lane];
+                    }
                     const int32_t hd = CS::h_of(v), ev = CS::e_of(v);
-                    const int32_t m = hd != 0 ? hd + subst(trow, codes[j * 64 + lane]) : 0;
+                    const int32_t m = hd != 0 ? hd + subst(trow, qc) : 0;
                     const int32_t h = max(max(m, ev), f);
                     const int32_t e = max(max(ev - e_del, m - oe_del), 0);
                     eh[j * 64 + lane] = CS::pack(h1, e);
