@@ -84,6 +84,8 @@ template <> struct EhView<false> {
     int2 *p;
     __device__ __forceinline__ EhView(void *base, int) : p((int2 *)base) {}
     __device__ __forceinline__ int2 load(int j) const { return p[j]; }
+    __device__ __forceinline__ int32_t load_h(int j) const { return p[j].x; }
+    __device__ __forceinline__ int32_t load_e(int j) const { return p[j].y; }
     __device__ __forceinline__ void set_h(int j, int32_t v) const { p[j].x = v; }
     __device__ __forceinline__ void set_e(int j, int32_t v) const { p[j].y = v; }
     __device__ __forceinline__ void init(int j, int32_t h) const { p[j] = make_int2(h, 0); }
@@ -100,6 +102,16 @@ template <> struct EhView<true> {
         asm("" : "+v"(h));
         asm("" : "+v"(e));
         return make_int2(h, e);
+    }
+    __device__ __forceinline__ int32_t load_h(int j) const {
+        int32_t h = hp[j];
+        asm("" : "+v"(h));
+        return h;
+    }
+    __device__ __forceinline__ int32_t load_e(int j) const {
+        int32_t e = ep[j];
+        asm("" : "+v"(e));
+        return e;
     }
     __device__ __forceinline__ void set_h(int j, int32_t v) const { hp[j] = (uint16_t)v; }
     __device__ __forceinline__ void set_e(int j, int32_t v) const { ep[j] = (uint16_t)v; }
@@ -294,9 +306,10 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         key = row_allmax(key);  // h << 13 | absolute column
         const int32_t row_max = key >> kKeyShift;
         const int32_t mj = key & kKeyMask;
-        const int2 at_end = eh.load(end);  // H(i, end-1) (h1_init if empty)
-        const int2 at_beg = eh.load(beg);
-        const int32_t h1_last = at_end.x;
+        // the edge slots: end holds H(i, end-1) (h1_init if the window is empty) and the E just
+        // reset to 0; beg holds h1_init and E(i+1, beg)
+        const int32_t h1_last = eh.load_h(end);
+        const int32_t e_beg = eh.load_e(beg);
 
         bool done = false;
         const int32_t j_exit = beg < end ? end : beg;
@@ -321,7 +334,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             // band tightening (sw.rs:464-480): first live column in [beg, end), last live column
             // scanning down from end.  The edges are usually live, so test them first.
             int32_t nb = beg;
-            if (beg < end && (at_beg.x | at_beg.y) == 0) {
+            if (beg < end && (h1_init | e_beg) == 0) {
                 for (;;) {
                     const int j = nb + g;
                     bool live = false;
@@ -336,7 +349,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 }
             }
             int32_t ne = end;
-            if ((at_end.x | at_end.y) == 0) {
+            if (h1_last == 0) {
                 for (;;) {
                     const int j = ne - g;
                     bool live = false;
