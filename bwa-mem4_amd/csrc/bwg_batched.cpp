@@ -798,8 +798,19 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
             e = hipEventRecord(ss.fork, ss.main);
             for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(ss.aux[k], ss.fork, 0);
         }
-        for (size_t r = i; r < j && e == hipSuccess; ++r) {
-            const int slot = (int)((r - i) % (kAuxStreams + 1));
+        // largest nominal work first, each onto the stream with the least work queued
+        // (longest-processing-time-first over the stage's items)
+        std::vector<size_t> ord(nitems);
+        std::iota(ord.begin(), ord.end(), i);
+        std::stable_sort(ord.begin(), ord.end(),
+                         [&](size_t x, size_t y) { return p->items[x].work > p->items[y].work; });
+        double load[kAuxStreams + 1] = {};
+        for (size_t r : ord) {
+            if (e != hipSuccess) break;
+            int slot = 0;
+            for (int k = 1; k <= kAuxStreams; ++k)
+                if (load[k] < load[slot]) slot = k;
+            load[slot] += std::max(p->items[r].work, 1.0);
             hipStream_t st = (!par || slot == 0) ? ss.main : ss.aux[slot - 1];
             e = launch_item(b, p, p->items[r], st);
             ++nl;
