@@ -47,6 +47,60 @@ def test_forced_extension_mappings_agree(gpu_ctx):
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
+def _long_query_batch(n, rng, qlo, qhi, h0_of):
+    """Reads of qlen in [qlo, qhi) against targets drawn from them with 3% substitutions and short
+    indels, followed by up to 80 random bases."""
+    qs, ts, h0s = [], [], []
+    for _ in range(n):
+        ql = int(rng.integers(qlo, qhi))
+        q = rng.integers(0, 4, ql, dtype=np.uint8)
+        t = []
+        k = 0
+        while k < ql:
+            u = rng.random()
+            if u < 0.03:
+                t.append(int(rng.integers(0, 4)))
+                k += 1
+            elif u < 0.035:
+                t.extend(int(x) for x in rng.integers(0, 4, int(rng.integers(1, 6))))
+            elif u < 0.04:
+                k += int(rng.integers(1, 6))
+            else:
+                t.append(int(q[k]))
+                k += 1
+        t.extend(int(x) for x in rng.integers(0, 4, int(rng.integers(0, 80))))
+        qs.append(q)
+        ts.append(np.asarray(t, dtype=np.uint8))
+        h0s.append(h0_of(ql))
+    return bwg.JobBatch.from_pairs(qs, ts, np.asarray(h0s, dtype=np.int32))
+
+
+@pytest.mark.parametrize("width", [16, 32])
+def test_long_queries_and_state_width_boundary(gpu_ctx, width):
+    """Queries up to 500 columns in the group kernels (windows up to 33 columns per lane at w=250),
+    with h0 + qlen*max_sc around 65536, where 16-bit column state stops being exact and the
+    planner switches those jobs to 32-bit state."""
+    rng = np.random.default_rng(91)
+    batches = [
+        (_long_query_batch(200, rng, 257, 500, lambda ql: int(rng.integers(1, 200))), 100),
+        (_long_query_batch(200, rng, 257, 500, lambda ql: int(rng.integers(1, 200))), 250),
+        (_long_query_batch(200, rng, 100, 500, lambda ql: int(65535 - ql + rng.integers(-3, 4))), 250),
+    ]
+    mat = bwg.default_mat()
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
+    try:
+        gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, width)
+        for bi, (batch, w) in enumerate(batches):
+            _say("long queries batch %d w=%d width=%d" % (bi, w, width))
+            got = gpu_ctx.extend_arrays(batch, bwg.ext_params(5, mat, 6, 1, 6, 1, w, 5, 100))
+            want = ob.extend_batch(batch, mat, 6, 1, 6, 1, w, 5, 100)
+            assert _first_bad(got, want) is None, (bi, _first_bad(got, want))
+    finally:
+        gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, 16)
+        gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
+
+
 def test_forced_rescue_mappings_agree(gpu_ctx):
     mat = bwg.default_mat()
     batch = bwg.generate(bwg.WORKLOAD_RESCUE, 3000, seed=22)
