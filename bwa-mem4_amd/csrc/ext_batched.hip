@@ -220,6 +220,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                                                         bwg_ext_result *__restrict__ out) {
     extern __shared__ int32_t lds[];
     const uint32_t count = dcount ? *dcount : count_static;  // a compacted queue's length is on the device
+    // an empty queue (a retry round that retries nothing) ends the grid before any queue atomic
+    if (count == 0) return;
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;             // lane within the group
     const int gid = threadIdx.x >> 4;    // group within the block
@@ -398,6 +400,7 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
                                                         uint32_t *__restrict__ head, ExtScoring sc,
                                                         bwg_ext_result *__restrict__ out) {
     const uint32_t count = dcount ? *dcount : count_static;
+    if (count == 0) return;  // an empty compacted queue: no queue atomics, no table set-up
     using CS = ColState<NARROW>;
     using W = typename CS::word;
     __shared__ W eh[(QCAP + 1) * 64];
