@@ -413,23 +413,33 @@ __global__ __launch_bounds__(256) void ksw_tracker_kernel(const KswJob *__restri
         const int32_t lo = te - half, hi = te + half;
         int32_t cs = 0, cr = -1;
         bool open = false;
-        for (int32_t i = 0; i < (int32_t)jb.tlen; ++i) {
-            const int32_t v = rm[64 * i];
-            if (v < sc.minsc) continue;
-            if (open && cr + 1 == i) {
-                if (cs < v) {
-                    cs = v;
-                    cr = i;
+        const int32_t tlen = (int32_t)jb.tlen;
+        // eight independent loads in flight per step; the merge rule then runs over them in order
+        constexpr int kAhead = 8;
+        for (int32_t i0 = 0; i0 < tlen; i0 += kAhead) {
+            int32_t vv[kAhead];
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) vv[u] = i0 + u < tlen ? rm[64 * (i0 + u)] : 0;
+#pragma unroll
+            for (int u = 0; u < kAhead; ++u) {
+                const int32_t i = i0 + u, v = vv[u];
+                if (i >= tlen) break;
+                if (v < sc.minsc) continue;
+                if (open && cr + 1 == i) {
+                    if (cs < v) {
+                        cs = v;
+                        cr = i;
+                    }
+                    continue;
                 }
-                continue;
+                if (open && (cr < lo || cr > hi) && cs > s2) {
+                    s2 = cs;
+                    t2 = cr;
+                }
+                cs = v;
+                cr = i;
+                open = true;
             }
-            if (open && (cr < lo || cr > hi) && cs > s2) {
-                s2 = cs;
-                t2 = cr;
-            }
-            cs = v;
-            cr = i;
-            open = true;
         }
         if (open && (cr < lo || cr > hi) && cs > s2) {
             s2 = cs;
