@@ -30,7 +30,7 @@ namespace {
 struct Item {
     int kind = 0;
     int param = 0;     // K, or the thread kernel's column capacity
-    bool narrow = false;  // 16-bit DP state (extension thread kernel, packed rescue kernel)
+    bool narrow = false;  // 16-bit DP state (extension, packed rescue kernel); packed scores (global)
     int qcap = 0;      // extension group: LDS sizing
     int stage = 0;     // items of one stage run concurrently; stages run in order
     uint32_t offset = 0, count = 0, head = 0;
@@ -687,7 +687,7 @@ int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         GlobalJob &j = jobs[k];
         j.z_off = zw;
         j.ops_off = ow;
-        zw += (uint64_t)j.tlen * 16 * ((j.k + 7) / 8);
+        zw += (uint64_t)((j.tlen + 7) & ~7u) * 16 * ((j.k + 7) / 8);  // rows in tiles of 8
         ow += (uint64_t)j.qlen + j.tlen + 2;
     }
     std::vector<Item> items;
@@ -695,7 +695,8 @@ int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         Item it = bins[r];
         it.head = (uint32_t)r;
         it.stage = 0;
-        const int per_cu = glb_group_blocks_per_cu(it.param, it.qcap);
+        it.narrow = glb_scores_packable(b->gsc);  // the packed-score kernel form
+        const int per_cu = glb_group_blocks_per_cu(it.param, it.qcap, it.narrow);
         it.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)it.count + 15) / 16));
         items.push_back(it);
     }
@@ -760,7 +761,7 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->ksc,
                                 (bwg_ksw_result *)b->d_out, p->d_rowmax, it.grid);
     case kGlbGroup:
-        return launch_glb_group(st, it.param, b->d_arena, b->d_ref, (const GlobalJob *)p->d_jobs,
+        return launch_glb_group(st, it.param, it.narrow, b->d_arena, b->d_ref, (const GlobalJob *)p->d_jobs,
                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->gsc, it.qcap, b->d_z,
                                 (int32_t *)b->d_out, it.grid);
     case kGlbTrace:

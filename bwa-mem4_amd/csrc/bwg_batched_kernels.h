@@ -66,8 +66,10 @@ hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t
                               const KswScoring &sc, const int32_t *rowmax, bwg_ksw_result *out);
 
 // Banded global alignment with traceback (global_batched.hip).  z_off is the job's first traceback
-// dword (tlen * 16 * ((K+7)/8) of them), ops_off its first scratch word (qlen + tlen + 2 of them),
-// k the group kernel's columns per lane for its bin.
+// dword (tlen rounded up to 8, times 16 * ((K+7)/8) of them), ops_off its first scratch word
+// (qlen + tlen + 2 of them), k the group kernel's columns per lane for its bin.  `packed` selects
+// the kernel form that keeps one byte per query column and the row's scores as 6-bit fields,
+// valid when glb_scores_packable(sc) (every score in [-32, 31]).
 struct GlobalScoring {
     int8_t mat[28];
     int32_t o_del, e_del, o_ins, e_ins;
@@ -78,11 +80,12 @@ struct GlobalJob {
     int32_t w;
     uint32_t k;
 };
-size_t glb_group_lds_bytes(int K, int qcap);
-int glb_group_blocks_per_cu(int K, int qcap);
-hipError_t launch_glb_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const GlobalJob *jobs,
-                            const uint32_t *order, uint32_t count, uint32_t *head, const GlobalScoring &sc,
-                            int qcap, uint32_t *z, int32_t *score, int grid);
+bool glb_scores_packable(const GlobalScoring &sc);
+size_t glb_group_lds_bytes(int K, int qcap, bool packed);
+int glb_group_blocks_per_cu(int K, int qcap, bool packed);
+hipError_t launch_glb_group(hipStream_t st, int K, bool packed, const uint8_t *arena, const uint8_t *ref,
+                            const GlobalJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
+                            const GlobalScoring &sc, int qcap, uint32_t *z, int32_t *score, int grid);
 // n_ops[k]: the job's CIGAR runs, back to front at ops + ops_off, or 0xffffffff when the walk left
 // the stored band (possible only when no alignment path lies inside it).
 hipError_t launch_glb_traceback(hipStream_t st, const GlobalJob *jobs, uint32_t n, const uint32_t *z, uint32_t *ops,

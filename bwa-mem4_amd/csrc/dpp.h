@@ -44,6 +44,22 @@ __device__ __forceinline__ int row_scan_max_nonneg(int x) {
     return x;
 }
 
+// Max over the whole wavefront of a value that is uniform within each 16-lane row (one value per
+// group), returned as a scalar.  Row broadcasts (lane 15 of a row into the next row, then lane 31
+// into rows 2 and 3) leave the maximum of all four rows in lane 63.  Each step is one max with a
+// DPP operand; lanes outside the step's row mask keep their value.  The s_nop gives the DPP read
+// of a register written by the previous VALU its two wait states.  The whole wavefront must be
+// active.
+__device__ __forceinline__ int wave_max_of_rows(int x) {
+    asm volatile(
+        "s_nop 1\n\t"
+        "v_max_i32_dpp %0, %0, %0 row_bcast:15 row_mask:0xa bank_mask:0xf\n\t"
+        "s_nop 1\n\t"
+        "v_max_i32_dpp %0, %0, %0 row_bcast:31 row_mask:0xc bank_mask:0xf"
+        : "+v"(x));
+    return __builtin_amdgcn_readlane(x, 63);
+}
+
 // Value of lane l-1 of the row; lane 0 of the row receives `fill`.
 __device__ __forceinline__ int row_shift_in(int x, int fill) { return dpp<kRowShr + 1, false>(fill, x); }
 

@@ -182,7 +182,8 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
         const int32_t f = max(f_in, gg[k]) - k * e_ins;
         const int32_t h = max(max(mm[k], v[k].y), f);
         const int32_t e = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
-        if (k < nval) {
+        // expected taken: the store runs under the exec mask with no branch around it
+        if (__builtin_expect(k < nval, 1)) {
             // lane-relative column; the lane's first column is added once below
             key = max(key, (h << kKeyShift) | k);
             eh.set_e(base + k, e);      // E(i+1, j)
@@ -281,19 +282,15 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
 
         // ---- one DP row ------------------------------------------------------------------------
         const int32_t i = J.i;
-        int32_t beg = J.beg, end = J.end;
-        if (active) {
-            if (beg < i - J.w) beg = i - J.w;
-            if (end > i + J.w + 1) end = i + J.w + 1;
-            if (end > J.qlen) end = J.qlen;
-        }
+        // an idle group's values are not used (its window width below is 0)
+        const int32_t beg = max(J.beg, i - J.w);
+        const int32_t end = min(min(J.end, i + J.w + 1), J.qlen);
         // Block width of this row: the widest live window among the wavefront's four groups, in
         // an odd number of columns per lane (at most K, since the bin's K covers the job's widest
         // possible window).  The band narrows as rows go by, and the wavefront's time per row
         // follows its widest group, so the row's columns are spread over every lane again.
         const int32_t wwin = active ? max(end - beg, 0) : 0;
-        const int32_t wmax = max(max(__builtin_amdgcn_readlane(wwin, 0), __builtin_amdgcn_readlane(wwin, 16)),
-                                 max(__builtin_amdgcn_readlane(wwin, 32), __builtin_amdgcn_readlane(wwin, 48)));
+        const int32_t wmax = wave_max_of_rows(wwin);
         const int32_t kr = min(((wmax + 15) >> 4) | 1, K);
         if (!active) continue;
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
@@ -326,7 +323,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
             J.best_i = i;
             J.best_j = mj;
             J.max_off = max(J.max_off, abs(mj - i));
-        } else if (sc.zdrop > 0) {
+        } else if (sc.zdrop > 0 && J.best - row_max > sc.zdrop) {
+            // the gap term is never negative, so a drop within zdrop cannot end the extension
             const int32_t di = i - J.best_i, dj = mj - J.best_j;
             const int32_t dd = di > dj ? di - dj : dj - di, ge = di > dj ? e_del : e_ins;
             const int32_t gap = sc.small_ext ? __mul24(dd, ge) : dd * ge;

@@ -8,11 +8,15 @@
 //    columns beg + g*K .. beg + g*K + K-1.  H (shifted by one column, as the reference's eh_h) and
 //    E live in LDS over absolute columns; F is the max-plus recurrence F(j+1) = max(F(j) - e_ins,
 //    M(j) - oe_ins) with no floor, evaluated by a per-lane pass, a max-scan across the row and a
-//    second pass.  Every evaluated cell leaves a 4-bit traceback code in HBM: bits 0-1 how H was
-//    reached (0 M, 1 E, 2 F; ties M over E over F), bit 2 "the deletion continues", bit 3 "the
-//    insertion continues", the same information as the reference's byte f<<4 | e<<2 | h.  The
-//    codes of a row are stored lane-major: lane g's K codes in (K+7)/8 dwords at
-//    (i*16 + g) * dwords, so each lane writes whole dwords.
+//    second pass.  Every evaluated cell leaves a 4-bit traceback code in HBM: bit 0 "E beat M",
+//    bit 1 "F beat max(M, E)" (so H came from M, E or F with ties M over E over F), bit 2 "the
+//    deletion continues", bit 3 "the insertion continues", the same information as the
+//    reference's byte f<<4 | e<<2 | h.  Each bit is the sign of a difference, shifted into the
+//    lane's code word with one funnel shift.  Lane g's K codes of a row fill (K+7)/8 dwords; in
+//    the dword holding n codes, code kk (0-based) is the nibble n-1-kk.  Rows are stored in tiles
+//    of 8: dword m of lane g in row i sits at ((i/8*16 + g) * dwords + m) * 8 + i%8, so the 8 rows
+//    of one lane's dword are adjacent and a traceback step along a diagonal (same lane, previous
+//    row) mostly reads the cache line of the step before.
 //  * glb_traceback_kernel: one job per thread walks the codes back from the band's last cell,
 //    merging runs as push_cigar does (sw.rs:515-524), and leaves the runs back to front in a
 //    per-job scratch region; glb_cigar_gather_kernel copies them front to back into the caller's
@@ -34,9 +38,24 @@ constexpr int kGlbGroupsPerBlock = 16;
 constexpr uint32_t kTracebackOffBand = 0xffffffffu;
 
 __host__ __device__ __forceinline__ int glb_eh_bytes(int qcap) { return 8 * (qcap + 2); }
-__host__ __device__ __forceinline__ int glb_plane_bytes(int qcap) { return 5 * qcap; }
-__host__ __device__ __forceinline__ size_t glb_block_lds_bytes(int qcap, int K) {
-    return (size_t)kGlbGroupsPerBlock * (glb_eh_bytes(qcap) + glb_plane_bytes(qcap)) + 16 * K * 8 + 64;
+// Scores per query column: five int8 planes (mat[c][q[j]] for target base c), or with packed
+// scores one byte 6*q[j] that selects a 6-bit field of the row's packed score word.
+__host__ __device__ __forceinline__ int glb_plane_bytes(int qcap, bool packed) { return packed ? qcap : 5 * qcap; }
+// Block LDS: the packed score words (32 bytes, packed form only), then every group's H/E columns,
+// then every group's score bytes, then slack for the last group's loads past its columns.
+__host__ __device__ __forceinline__ size_t glb_block_lds_bytes(int qcap, int K, bool packed) {
+    return (size_t)kGlbGroupsPerBlock * (glb_eh_bytes(qcap) + glb_plane_bytes(qcap, packed)) + 16 * K * 8 + 64 +
+           (packed ? 32 : 0);
+}
+
+// Position of dword m of lane g's codes in row i, in a job's code region (rows in tiles of 8).
+__device__ __forceinline__ size_t code_index(int32_t i, int32_t g, int32_t m, int32_t dw) {
+    return ((size_t)((i >> 3) * 16 + g) * dw + m) * 8 + (i & 7);
+}
+
+// Shift the sign bit of d into the low end of w: ({w, d} >> 31), one v_alignbit_b32.
+__device__ __forceinline__ uint32_t push_sign(uint32_t w, int32_t d) {
+    return __builtin_amdgcn_alignbit(w, (uint32_t)d, 31u);
 }
 
 // Inclusive prefix max over the 16 lanes of a row for values of any sign: a lane whose source is
@@ -49,7 +68,7 @@ __device__ __forceinline__ int row_scan_max_any(int x) {
     return x;
 }
 
-template <int K>
+template <int K, bool PACKED>
 __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restrict__ arena,
                                                         const uint8_t *__restrict__ ref,
                                                         const GlobalJob *__restrict__ jobs,
@@ -61,8 +80,18 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;
     const int gid = threadIdx.x >> 4;
-    int32_t *eh = (int32_t *)((char *)lds + gid * glb_eh_bytes(qcap));
-    int8_t *planes = (int8_t *)lds + kGlbGroupsPerBlock * glb_eh_bytes(qcap) + gid * glb_plane_bytes(qcap);
+    char *lb = (char *)lds + (PACKED ? 32 : 0);
+    uint32_t *tabs = (uint32_t *)lds;  // packed form: row c's scores mat[c][0..4] as 6-bit fields
+    int32_t *eh = (int32_t *)(lb + gid * glb_eh_bytes(qcap));
+    int8_t *planes = (int8_t *)lb + kGlbGroupsPerBlock * glb_eh_bytes(qcap) + gid * glb_plane_bytes(qcap, PACKED);
+    if constexpr (PACKED) {
+        if (threadIdx.x < 5) {
+            uint32_t tw = 0;
+            for (int c = 0; c < 5; ++c) tw |= ((uint32_t)sc.mat[threadIdx.x * 5 + c] & 63u) << (6 * c);
+            tabs[threadIdx.x] = tw;
+        }
+        __syncthreads();
+    }
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
     const int32_t Ke = K * e_ins;
@@ -97,8 +126,12 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
                 }
                 for (int j = g; j < qlen; j += 16) {
                     const int qc = q[j];
+                    if constexpr (PACKED) {
+                        planes[j] = (int8_t)(6 * qc);
+                    } else {
 #pragma unroll
-                    for (int c = 0; c < 5; ++c) planes[c * qcap + j] = sc.mat[c * 5 + qc];
+                        for (int c = 0; c < 5; ++c) planes[c * qcap + j] = sc.mat[c * 5 + qc];
+                    }
                 }
                 i = 0;
                 tcur = tlen > 0 ? t[0] : 0;
@@ -119,23 +152,30 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
         const int32_t base = beg + g * K;
         const int32_t nval = min(max(end - base, 0), K);
         const int2 *ehb = (const int2 *)eh + base;
-        const int8_t *sb = planes + tcur * qcap + base;
+        const int8_t *sb = planes + (PACKED ? 0 : tcur * qcap) + base;
+        const uint32_t rowtab = PACKED ? tabs[tcur] : 0u;
         int2 v[K];
         int32_t s[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             v[k] = ehb[k];
-            s[k] = sb[k];
+            if constexpr (PACKED)
+                s[k] = __builtin_amdgcn_sbfe(rowtab, (uint32_t)(uint8_t)sb[k], 6u);
+            else
+                s[k] = sb[k];
         }
-        // pass 1: M and the lane's own F chain from the identity
+        // pass 1: M and the lane's own F chain from the identity, carried with its drift
+        // G_k = F_k + k*e_ins: F_{k+1} = max(F_k - e_ins, M_k - oe_ins) becomes G_{k+1} =
+        // max(G_k, M_k + (k+1)*e_ins - oe_ins), one add and one max per column
         int32_t mm[K], gg[K];
         int32_t grun = kNeg;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
             mm[k] = v[k].x + s[k];
             gg[k] = grun;
-            grun = max(grun - e_ins, mm[k] - oe_ins);
+            grun = max(grun, mm[k] + ((k + 1) * e_ins - oe_ins));
         }
+        grun -= Ke;  // F leaving the lane's block
         // F entering this lane: the row starts at F = MINUS_INF, then the max-scan of the lanes'
         // exits shifted by K*e_ins per lane
         const int32_t incl = row_scan_max_any(grun + g * Ke);
@@ -147,19 +187,25 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
         int2 *ehw = (int2 *)eh + base;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            const int32_t f = max(f_in - k * e_ins, gg[k]);
+            const int32_t fd = max(f_in, gg[k]);  // F(i, j) + k*e_ins
+            const int32_t f = fd - k * e_ins;
             const int32_t e = v[k].y;
-            uint32_t code = mm[k] < e ? 1u : 0u;
-            int32_t h = max(mm[k], e);
-            if (h < f) {
-                code = 2u;
-                h = f;
-            }
+            const int32_t hme = max(mm[k], e);
+            const int32_t h = max(hme, f);
             const int32_t td = mm[k] - oe_del, en = e - e_del;
-            code |= en > td ? 4u : 0u;
-            code |= f - e_ins > mm[k] - oe_ins ? 8u : 0u;
-            zw[k / 8] |= code << (4 * (k % 8));
-            if (k < nval) {
+            // The code bits as signs of differences; every value here lies within 2^30 of zero,
+            // so no difference overflows.  Bit 3: F(i, j+1) extends F(i, j) rather than opening
+            // from M, F - e_ins > M - oe_ins with both sides moved by (k+1)*e_ins.  Bit 2: E(i+1, j)
+            // extends E(i, j).  Bits 1 and 0: F beat max(M, E), E beat M; M wins ties over E and
+            // both over F (sw.rs:654-661).
+            uint32_t w = zw[k / 8];
+            w = push_sign(w, mm[k] + ((k + 1) * e_ins - oe_ins) - fd);
+            w = push_sign(w, td - en);
+            w = push_sign(w, hme - f);
+            w = push_sign(w, mm[k] - e);
+            zw[k / 8] = w;
+            // expected taken: the store runs under the exec mask with no branch around it
+            if (__builtin_expect(k < nval, 1)) {
                 ehw[k].y = max(en, td);  // E(i+1, j)
                 ehw[k + 1].x = h;        // H(i, j), the diagonal of column j+1 next row
             }
@@ -169,10 +215,10 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
             eh[2 * end + 1] = kMinusInf;  // E at the band's end
         }
         if (nval > 0) {
-            uint32_t *zr = zj + (size_t)(i * 16 + g) * DW;
+            uint32_t *zr = zj + code_index(i, g, 0, DW);
 #pragma unroll
             for (int m = 0; m < DW; ++m)
-                if (8 * m < nval) zr[m] = zw[m];
+                if (8 * m < nval) zr[8 * m] = zw[m];
         }
         wave_lds_sync();
         ++i;
@@ -217,8 +263,10 @@ __global__ __launch_bounds__(256) void glb_traceback_kernel(const GlobalJob *__r
             break;
         }
         const int32_t g = off / K, kk = off - g * K;
-        const uint32_t code = (zj[(size_t)(i * 16 + g) * dw + kk / 8] >> (4 * (kk % 8))) & 15u;
-        which = which == 0 ? (code & 3u) : (which == 1 ? (code >> 2) & 1u : ((code >> 3) & 1u) * 2u);
+        const int32_t m = kk >> 3, nib = min(8, K - 8 * m) - 1 - (kk & 7);
+        const uint32_t code = (zj[code_index(i, g, m, dw)] >> (4 * nib)) & 15u;
+        const uint32_t src = (code & 2u) ? 2u : (code & 1u);  // 0 M, 1 E, 2 F
+        which = which == 0 ? src : (which == 1 ? (code >> 2) & 1u : ((code >> 3) & 1u) * 2u);
         if (which == 0) {
             push(0, 1);
             --i;
@@ -254,38 +302,52 @@ __global__ __launch_bounds__(256) void glb_cigar_gather_kernel(const GlobalJob *
     for (uint32_t r = 0; r < m; ++r) d[r] = src[m - 1 - r];
 }
 
-template <int K>
+template <int K, bool PACKED>
 int glb_occ(int qcap) {
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, glb_group_kernel<K>, 256,
-                                                     glb_block_lds_bytes(qcap, K)) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, glb_group_kernel<K, PACKED>, 256,
+                                                     glb_block_lds_bytes(qcap, K, PACKED)) != hipSuccess)
         blocks = 1;
     return blocks > 0 ? blocks : 1;
 }
 
+template <int K, bool PACKED>
+hipError_t launch_glb_group_k(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const GlobalJob *jobs,
+                              const uint32_t *order, uint32_t count, uint32_t *head, const GlobalScoring &sc,
+                              int qcap, uint32_t *z, int32_t *score, int grid) {
+    hipLaunchKernelGGL((glb_group_kernel<K, PACKED>), dim3(grid), dim3(256), glb_block_lds_bytes(qcap, K, PACKED), st,
+                       arena, ref, jobs, order, count, head, sc, qcap, z, score);
+    return hipGetLastError();
+}
+
 }  // namespace
 
-size_t glb_group_lds_bytes(int K, int qcap) { return glb_block_lds_bytes(qcap, K); }
+bool glb_scores_packable(const GlobalScoring &sc) {
+    for (int k = 0; k < 25; ++k)
+        if (sc.mat[k] < -32 || sc.mat[k] > 31) return false;
+    return true;
+}
 
-int glb_group_blocks_per_cu(int K, int qcap) {
+size_t glb_group_lds_bytes(int K, int qcap, bool packed) { return glb_block_lds_bytes(qcap, K, packed); }
+
+int glb_group_blocks_per_cu(int K, int qcap, bool packed) {
     switch (K) {
-#define BWG_K(k) case k: return glb_occ<k>(qcap);
+#define BWG_K(k) case k: return packed ? glb_occ<k, true>(qcap) : glb_occ<k, false>(qcap);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return 1;
     }
 }
 
-hipError_t launch_glb_group(hipStream_t st, int K, const uint8_t *arena, const uint8_t *ref, const GlobalJob *jobs,
-                            const uint32_t *order, uint32_t count, uint32_t *head, const GlobalScoring &sc,
-                            int qcap, uint32_t *z, int32_t *score, int grid) {
+hipError_t launch_glb_group(hipStream_t st, int K, bool packed, const uint8_t *arena, const uint8_t *ref,
+                            const GlobalJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
+                            const GlobalScoring &sc, int qcap, uint32_t *z, int32_t *score, int grid) {
     if (count == 0) return hipSuccess;
     switch (K) {
-#define BWG_K(k)                                                                                        \
-    case k:                                                                                             \
-        hipLaunchKernelGGL(glb_group_kernel<k>, dim3(grid), dim3(256), glb_block_lds_bytes(qcap, k), st, \
-                           arena, ref, jobs, order, count, head, sc, qcap, z, score);                   \
-        return hipGetLastError();
+#define BWG_K(k)                                                                                                    \
+    case k:                                                                                                         \
+        return packed ? launch_glb_group_k<k, true>(st, arena, ref, jobs, order, count, head, sc, qcap, z, score, grid) \
+                      : launch_glb_group_k<k, false>(st, arena, ref, jobs, order, count, head, sc, qcap, z, score, grid);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return hipErrorInvalidValue;

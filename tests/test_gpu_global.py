@@ -36,7 +36,10 @@ def test_region_jobs_match_oracle(gpu_ctx, seed):
     assert (n > 1).sum() > 500  # jobs with gaps in their CIGAR
 
 
-@pytest.mark.parametrize("a,b,o_del,e_del,o_ins,e_ins", [(2, 3, 5, 2, 3, 1), (1, 1, 0, 1, 4, 3), (3, 2, 2, 1, 7, 1)])
+# (31, 32): scores at both ends of the packed 6-bit range; (1, 40): a mismatch score outside it,
+# which runs the kernel form with five score planes per query column
+@pytest.mark.parametrize("a,b,o_del,e_del,o_ins,e_ins", [(2, 3, 5, 2, 3, 1), (1, 1, 0, 1, 4, 3), (3, 2, 2, 1, 7, 1),
+                                                         (31, 32, 6, 1, 6, 1), (1, 40, 6, 1, 6, 1)])
 def test_scoring_sweep_matches_oracle(gpu_ctx, a, b, o_del, e_del, o_ins, e_ins):
     _say("global scoring %s" % ((a, b, o_del, e_del, o_ins, e_ins),))
     batch, _ = bwg.generate_global_jobs(3000, seed=a * 10 + b, max_indel=30)
