@@ -52,11 +52,12 @@ hipError_t launch_ksw_group(hipStream_t st, int K, bool rev, const uint8_t *aren
 // Packed 16-bit form of the rescue pass (K columns per lane as K/2 pairs); exact for jobs with
 // qlen * (largest matrix score) < kKswPkMaxScore under a scoring that ksw_pk_scoring_ok accepts:
 // non-negative gap opens and open + extend below 2^14, so H*64 + 63 and every unfloored E and F
-// stay inside int16.
+// stay inside int16; insertion extension below 4000, so F carried with its drift (up to eight
+// extension penalties above F) does too.
 constexpr int64_t kKswPkMaxScore = 500;
 inline bool ksw_pk_scoring_ok(const KswScoring &sc) {
     return sc.o_del >= 0 && sc.o_ins >= 0 && sc.e_del >= 0 && sc.e_ins >= 0 && sc.o_del + sc.e_del < 16384 &&
-           sc.o_ins + sc.e_ins < 16384;
+           sc.o_ins + sc.e_ins < 16384 && sc.e_ins < 4000;
 }
 int ksw_pk_blocks_per_cu(int K, bool rev);
 hipError_t launch_ksw_pk(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref,

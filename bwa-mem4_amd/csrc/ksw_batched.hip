@@ -217,6 +217,14 @@ __device__ __forceinline__ pk16 pk_splat(int32_t v) { return (pk16){(short)v, (s
 __device__ __forceinline__ pk16 pk_max(pk16 a, pk16 b) { return __builtin_elementwise_max(a, b); }
 __device__ __forceinline__ uint32_t pk_bits(pk16 v) { return __builtin_bit_cast(uint32_t, v); }
 __device__ __forceinline__ pk16 pk_from(uint32_t w) { return __builtin_bit_cast(pk16, w); }
+// h*64 + k in each half, one v_pk_mad_u16 (the compiler otherwise emits a shift and an add); the
+// low 16 bits of the product and sum are the same for signed and unsigned halves.  op_sel_hi 0 on
+// the constant makes the upper half read the constant's low 16 bits (64) as well.
+__device__ __forceinline__ pk16 pk_key(pk16 h, pk16 k) {
+    uint32_t r;
+    asm("v_pk_mad_u16 %0, %1, 64, %2 op_sel_hi:[1,0,1]" : "=v"(r) : "v"(pk_bits(h)), "v"(pk_bits(k)));
+    return pk_from(r);
+}
 
 template <int KH, bool REV>
 __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__ arena,
@@ -326,6 +334,8 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         // column 0), upper half of pair 0 from this lane's last lower column, then pair p-1
         const int32_t prev_last = row_shift_in((int32_t)(pk_bits(hrow[KH - 1]) >> 16), 0);
         pk16 hd = pk_from((pk_bits(hrow[KH - 1]) << 16) | ((uint32_t)prev_last & 0xffffu));
+        // F along each half carried with its drift G_p = F_p + p*e_ins: F_{p+1} = max(F_p - e_ins,
+        // M_p - oe_ins) becomes G_{p+1} = max(G_p, M_p + (p+1)*e_ins - oe_ins), one op per pair
         pk16 dd[KH], dm[KH];
         pk16 grun = zero2;
 #pragma unroll
@@ -333,22 +343,23 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
             const pk16 d = pk_max(pk_max(hd + s2[p], erow[p]), zero2);
             hd = hrow[p];
             dd[p] = d;
-            dm[p] = d - oe_ins2;
-            grun = pk_max(grun - e_ins2, dm[p]);
+            dm[p] = d + pk_splat((p + 1) * e_ins - oe_ins);
+            grun = pk_max(grun, dm[p]);
         }
+        grun = grun - pk_splat(KHe);  // F leaving each half's last pair
         const int32_t c_lo = max((int32_t)grun.x, 0), c_hi = max((int32_t)grun.y, 0);
         const int32_t U = max(c_hi, c_lo - KHe) + g * Ke;
         const int32_t incl = row_scan_max_nonneg(U);
         const int32_t excl = row_shift_in(incl, -Ke);
         const int32_t f_in = excl - (g - 1) * Ke;
-        pk16 f = (pk16){(short)f_in, (short)max(f_in - KHe, c_lo)};
+        pk16 f = (pk16){(short)f_in, (short)max(f_in - KHe, c_lo)};  // F + p*e_ins at pair p
         pk16 key2 = zero2;
 #pragma unroll
         for (int p = 0; p < KH; ++p) {
-            const pk16 h = pk_max(dd[p], f);
-            key2 = pk_max(key2, h * pk_splat(64) + kk[p]);
+            const pk16 h = pk_max(dd[p], p == 0 ? f : f - pk_splat(p * e_ins));
+            key2 = pk_max(key2, pk_key(h, kk[p]));
             erow[p] = pk_max(erow[p] - e_del2, h - oe_del2);
-            f = pk_max(f - e_ins2, dm[p]);
+            f = pk_max(f, dm[p]);
             hrow[p] = h;
         }
         const int32_t kbest = max((int32_t)key2.x, (int32_t)key2.y);
