@@ -294,8 +294,12 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         const int32_t kr = min(((wmax + 15) >> 4) | 1, K);
         if (!active) continue;
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
-        const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
         const int8_t *srow = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap);
+        // This row's target base (requested a row ago) is consumed before the next one is
+        // requested: the vector-memory counter is in order, so a wait placed after the new request
+        // would wait for it too, a full memory round trip every row.
+        asm volatile("" ::"v"(J.tcur) : "memory");
+        const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
         int32_t key = row_cells_at<1, K>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
         if (g == 0) {
             eh.set_h(beg, h1_init);  // H(i, beg-1)

@@ -101,6 +101,12 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
     int32_t qlen = 0, tlen = 0, w = 0, i = 0, tcur = 0;
     uint32_t idx = 0;
     uint32_t *zj = nullptr;
+    // the lane's codes of the row before, not yet stored, for its pn window columns (0: none)
+    int32_t pn = 0;
+    uint32_t *pzr = nullptr;
+    uint32_t pzw[DW];
+#pragma unroll
+    for (int m = 0; m < DW; ++m) pzw[m] = 0u;
     for (;;) {
         if (!active && !exhausted) {
             uint32_t slot = 0;
@@ -148,12 +154,22 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
         const int32_t beg = max(i - w, 0);
         const int32_t end = min(i + w + 1, qlen);
         const int32_t h1_init = beg == 0 ? -(sc.o_del + sc.e_del * (i + 1)) : kMinusInf;
-        const int32_t tnext = i + 1 < tlen ? t[i + 1] : 0;
         const int32_t base = beg + g * K;
         const int32_t nval = min(max(end - base, 0), K);
         const int2 *ehb = (const int2 *)eh + base;
         const int8_t *sb = planes + (PACKED ? 0 : tcur * qcap) + base;
         const uint32_t rowtab = PACKED ? tabs[tcur] : 0u;
+        // The vector-memory counter is in order and counts stores too: this row's target base
+        // (requested a row ago) is consumed first, then the previous row's codes are stored, then
+        // the next base is requested, so the wait for a base never covers a request or a store made
+        // just before it.
+        asm volatile("" ::"v"(tcur) : "memory");
+        if (pn > 0) {
+#pragma unroll
+            for (int m = 0; m < DW; ++m)
+                if (8 * m < pn) pzr[8 * m] = pzw[m];
+        }
+        const int32_t tnext = i + 1 < tlen ? t[i + 1] : 0;
         int2 v[K];
         int32_t s[K];
 #pragma unroll
@@ -214,16 +230,21 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
             eh[2 * beg] = h1_init;        // H(i, beg-1)
             eh[2 * end + 1] = kMinusInf;  // E at the band's end
         }
-        if (nval > 0) {
-            uint32_t *zr = zj + code_index(i, g, 0, DW);
+        // this row's codes are stored at the start of the next row, or now if the job ends
+        pn = nval;
+        pzr = zj + code_index(i, g, 0, DW);
 #pragma unroll
-            for (int m = 0; m < DW; ++m)
-                if (8 * m < nval) zr[8 * m] = zw[m];
-        }
+        for (int m = 0; m < DW; ++m) pzw[m] = zw[m];
         wave_lds_sync();
         ++i;
         tcur = tnext;
         if (i >= tlen) {
+            if (pn > 0) {
+#pragma unroll
+                for (int m = 0; m < DW; ++m)
+                    if (8 * m < pn) pzr[8 * m] = pzw[m];
+            }
+            pn = 0;
             if (g == 0) score[idx] = eh[2 * qlen];  // H(tlen-1, qlen-1) as the last row left it
             active = false;
         }
