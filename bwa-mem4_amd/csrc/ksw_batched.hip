@@ -159,7 +159,7 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
         }
         key = row_allmax(key);
         const int32_t imax = key >> kKeyShift;
-        if (!REV && g == 0) rm[64 * i] = imax;
+        if (!REV && __builtin_expect(g == 0, 1)) rm[64 * i] = imax;  // under the exec mask, no branch
         bool done = false;
         if (imax > gmax) {
             gmax = imax;
@@ -208,10 +208,19 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
 //
 // The row maximum with the first-column tie rule is a 16-bit key h*64 + (63 - c) per half, c the
 // column within the lane's block, -32768 added for columns past the padded query so they never win;
-// a lane's best key becomes the usual 32-bit key h << 13 | 8191-j.  F enters the block's upper half
+// a lane's best key k16 becomes the 32-bit key h << 10 | (15-g) << 6 | (63-c), ordered by the row
+// maximum, then the lowest lane, then the lowest column in the lane: the lowest query column, as
+// lanes own consecutive column blocks.  It is k16 + (k16 >> 6)*960 + (15-g)*64, one shift, one
+// multiply-add and one add.  F enters the block's upper half
 // as max(F_in - KH*e_ins, c_lo), c_lo the lower half's exit from F = 0.  The query profile (five
 // target codes x KH column pairs) sits in LDS, lane-major, one 4-byte read per pair and row.
 typedef short pk16 __attribute__((ext_vector_type(2)));
+
+// Query column of a row maximum's 32-bit key h << 10 | (15-g) << 6 | (63-c): lane g owns columns
+// g*K .. g*K+K-1.
+__device__ __forceinline__ int32_t pk_key_column(int32_t key, int K) {
+    return (15 - ((key >> 6) & 15)) * K + 63 - (key & 63);
+}
 
 __device__ __forceinline__ pk16 pk_splat(int32_t v) { return (pk16){(short)v, (short)v}; }
 __device__ __forceinline__ pk16 pk_max(pk16 a, pk16 b) { return __builtin_elementwise_max(a, b); }
@@ -247,6 +256,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
     const int32_t Ke = K * e_ins, KHe = KH * e_ins;
+    const uint32_t g64 = (uint32_t)(15 - g) * 64u;  // this lane's field of the row-maximum key
     const pk16 e_ins2 = pk_splat(e_ins), e_del2 = pk_splat(e_del), oe_ins2 = pk_splat(oe_ins),
                oe_del2 = pk_splat(oe_del), zero2 = pk_splat(0);
 
@@ -332,8 +342,9 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         for (int p = 0; p < KH; ++p) s2[p] = pk_from(pr[p * 256]);
         // H(i-1, column-1): lower half of pair 0 from the previous lane's last column (0 left of
         // column 0), upper half of pair 0 from this lane's last lower column, then pair p-1
-        const int32_t prev_last = row_shift_in((int32_t)(pk_bits(hrow[KH - 1]) >> 16), 0);
-        pk16 hd = pk_from((pk_bits(hrow[KH - 1]) << 16) | ((uint32_t)prev_last & 0xffffu));
+        // ({this lane's last pair, the previous lane's last pair} >> 16): one funnel shift
+        const uint32_t prev_pair = (uint32_t)row_shift_in((int32_t)pk_bits(hrow[KH - 1]), 0);
+        pk16 hd = pk_from(__builtin_amdgcn_alignbit(pk_bits(hrow[KH - 1]), prev_pair, 16u));
         // F along each half carried with its drift G_p = F_p + p*e_ins: F_{p+1} = max(F_p - e_ins,
         // M_p - oe_ins) becomes G_{p+1} = max(G_p, M_p + (p+1)*e_ins - oe_ins), one op per pair
         pk16 dd[KH], dm[KH];
@@ -362,11 +373,10 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
             f = pk_max(f, dm[p]);
             hrow[p] = h;
         }
-        const int32_t kbest = max((int32_t)key2.x, (int32_t)key2.y);
-        const int32_t lkey = kbest > 0 ? ((kbest >> 6) << kKeyShift) | (kKeyMask - (g * K + 63 - (kbest & 63))) : 0;
-        const int32_t key = row_allmax(lkey);
-        const int32_t imax = key >> kKeyShift;
-        if (!REV && g == 0) rm[64 * i] = imax;
+        const uint32_t kbest = (uint32_t)max((int32_t)key2.x, (int32_t)key2.y);  // >= 0
+        const int32_t key = row_allmax((int32_t)(__umul24(kbest >> 6, 960u) + kbest + g64));
+        const int32_t imax = key >> 10;
+        if (!REV && __builtin_expect(g == 0, 1)) rm[64 * i] = imax;  // under the exec mask, no branch
         bool done = false;
         if (imax > gmax) {
             gmax = imax;
@@ -374,7 +384,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
                 if (gmax >= endsc) {
                     // first row reaching the forward score: the alignment starts rte/rqe back
                     if (g == 0 && gmax == endsc) {
-                        const int32_t rqe = kKeyMask - (key & kKeyMask);
+                        const int32_t rqe = pk_key_column(key, K);
                         out[idx].tb = te - i;
                         out[idx].qb = qe - rqe;
                     }
@@ -382,7 +392,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
                 }
             } else {
                 te = i;
-                qe = kKeyMask - (key & kKeyMask);
+                qe = pk_key_column(key, K);
             }
         }
         ++i;
