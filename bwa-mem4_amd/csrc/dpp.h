@@ -63,6 +63,35 @@ __device__ __forceinline__ int wave_max_of_rows(int x) {
 // Value of lane l-1 of the row; lane 0 of the row receives `fill`.
 __device__ __forceinline__ int row_shift_in(int x, int fill) { return dpp<kRowShr + 1, false>(fill, x); }
 
+// Job slots for the 16-lane groups of a persistent grid over a queue of `count` jobs sorted by
+// descending work.  With `deal`, slots below count - (groups in the grid) are dealt round-robin
+// (group q of the grid takes q, q + n_groups, ...), which needs no atomic: one contended queue
+// atomic per job switch stalls the whole wavefront for its round trip, a large share of a short
+// job.  The last n_groups slots, the shortest jobs, come from the queue counter, so that groups
+// that finish early take them and the grid's tail stays balanced.  Without `deal` (long jobs,
+// where per-wavefront speed differences outweigh the atomic) every slot comes from the counter.
+// Every lane of a group calls take() together.
+struct GroupSlots {
+    uint32_t next, stride, static_end;
+    __device__ __forceinline__ GroupSlots(uint32_t count, bool deal) {
+        const uint32_t groups_per_block = blockDim.x >> 4;
+        stride = gridDim.x * groups_per_block;
+        next = blockIdx.x * groups_per_block + (threadIdx.x >> 4);
+        static_end = deal && count > stride ? count - stride : 0u;
+    }
+    // The group's next slot; a slot >= count means the queue is exhausted.
+    __device__ __forceinline__ uint32_t take(uint32_t *head) {
+        if (next < static_end) {
+            const uint32_t s = next;
+            next += stride;
+            return s;
+        }
+        uint32_t s = 0;
+        if ((threadIdx.x & 15) == 0) s = static_end + atomicAdd(head, 1u);
+        return (uint32_t)__shfl((int)s, (int)(threadIdx.x & 63) & ~15, 64);
+    }
+};
+
 // Order LDS accesses of the lanes of one wavefront: everything before is visible to every lane
 // of the wave after.  A wavefront's LDS operations execute in issue order, so this only has to
 // stop the compiler from moving accesses across it.

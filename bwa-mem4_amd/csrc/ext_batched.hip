@@ -234,12 +234,11 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
 
     GroupJob J = {};
     bool active = false, exhausted = false;
+    GroupSlots slots(count, true);
     for (;;) {
         // ---- fetch and initialise a job for idle groups ------------------------------------
         if (!active && !exhausted) {
-            uint32_t slot = 0;
-            if (g == 0) slot = atomicAdd(head, 1u);
-            slot = __shfl(slot, lane & ~15, 64);
+            const uint32_t slot = slots.take(head);
             if (slot >= count) {
                 exhausted = true;
             } else {

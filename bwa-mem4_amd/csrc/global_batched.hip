@@ -107,11 +107,10 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
     uint32_t pzw[DW];
 #pragma unroll
     for (int m = 0; m < DW; ++m) pzw[m] = 0u;
+    GroupSlots slots(count, true);
     for (;;) {
         if (!active && !exhausted) {
-            uint32_t slot = 0;
-            if (g == 0) slot = atomicAdd(head, 1u);
-            slot = __shfl(slot, lane & ~15, 64);
+            const uint32_t slot = slots.take(head);
             if (slot >= count) {
                 exhausted = true;
             } else {

@@ -59,11 +59,10 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
     uint32_t prof[K], p4w[(K + 3) / 4];
     int32_t km[K], kc[K], hrow[K], erow[K];
 
+    GroupSlots slots(count, REV);  // the forward pass's long jobs balance better dynamically
     for (;;) {
         if (!active && !exhausted) {
-            uint32_t slot = 0;
-            if (g == 0) slot = atomicAdd(head, 1u);
-            slot = __shfl(slot, lane & ~15, 64);
+            const uint32_t slot = slots.take(head);
             if (slot >= count) {
                 exhausted = true;
             } else {
@@ -257,8 +256,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
     const int32_t Ke = K * e_ins, KHe = KH * e_ins;
     const uint32_t g64 = (uint32_t)(15 - g) * 64u;  // this lane's field of the row-maximum key
-    const pk16 e_ins2 = pk_splat(e_ins), e_del2 = pk_splat(e_del), oe_ins2 = pk_splat(oe_ins),
-               oe_del2 = pk_splat(oe_del), zero2 = pk_splat(0);
+    const pk16 e_del2 = pk_splat(e_del), oe_del2 = pk_splat(oe_del), zero2 = pk_splat(0);
 
     bool active = false, exhausted = false;
     TargetView t = {nullptr, 1};
@@ -267,11 +265,10 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
     int32_t *rm = nullptr;
     pk16 hrow[KH], erow[KH], kk[KH];
 
+    GroupSlots slots(count, REV);  // the forward pass's long jobs balance better dynamically
     for (;;) {
         if (!active && !exhausted) {
-            uint32_t slot = 0;
-            if (g == 0) slot = atomicAdd(head, 1u);
-            slot = __shfl(slot, lane & ~15, 64);
+            const uint32_t slot = slots.take(head);
             if (slot >= count) {
                 exhausted = true;
             } else {
