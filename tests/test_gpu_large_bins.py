@@ -1,0 +1,86 @@
+"""Bins with more jobs than their persistent grid has 16-lane groups: the group kernels then deal
+most job slots round-robin and take only the grid's last slots from the queue counter.  Every
+output is checked against the oracle.  Small batches never reach that path (their grid is sized
+to the bin), so these batches are built to fill one bin each."""
+import sys
+
+import numpy as np
+import pytest
+
+import bwg
+import oracle_binding as ob
+
+pytestmark = pytest.mark.gpu
+
+
+def _say(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def _pairs(n, qlen, tlen, seed, n_frac=0.0):
+    """n query/target pairs: the target starts with the query under 2% substitutions and runs on
+    at random; n_frac of the queries get one N base."""
+    rng = np.random.default_rng(seed)
+    qs, ts = [], []
+    for k in range(n):
+        q = rng.integers(0, 4, qlen).astype(np.uint8)
+        t = rng.integers(0, 4, tlen).astype(np.uint8)
+        t[:qlen] = q
+        m = rng.random(qlen) < 0.02
+        t[:qlen][m] = (t[:qlen][m] + 1) % 4
+        if rng.random() < n_frac:
+            q[rng.integers(0, qlen)] = 4
+        qs.append(q)
+        ts.append(t)
+    return qs, ts
+
+
+def _first_bad(got, want):
+    g, w = got.view(np.int32).reshape(len(got), -1), want.view(np.int32).reshape(len(want), -1)
+    bad = np.nonzero((g != w).any(axis=1))[0]
+    return None if bad.size == 0 else (int(bad[0]), g[bad[0]].tolist(), w[bad[0]].tolist())
+
+
+def test_extension_one_full_bin(gpu_ctx):
+    # 60k jobs of one window class (qlen 100, tlen 200 at w = 100), fast path off so that every
+    # job reaches the group kernel
+    _say("large bins: extension 60000 jobs, qlen 100")
+    qs, ts = _pairs(60000, 100, 200, seed=91)
+    rng = np.random.default_rng(92)
+    batch = bwg.JobBatch.from_pairs(qs, ts, h0=rng.integers(19, 120, len(qs)).astype(np.int32))
+    mat = bwg.default_mat()
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    gpu_ctx.set_option(bwg.OPT_UNGAPPED, 0)
+    try:
+        got = gpu_ctx.extend_arrays(batch, bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100))
+    finally:
+        gpu_ctx.set_option(bwg.OPT_UNGAPPED, 1)
+    want = ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=16)
+    assert _first_bad(got, want) is None, _first_bad(got, want)
+
+
+def test_rescue_one_full_bin(gpu_ctx):
+    # 40k 150-base mates against 180-base windows: one packed bin, whose reverse pass deals its
+    # slots; 2% of the mates carry an N base
+    _say("large bins: rescue 40000 jobs")
+    qs, ts = _pairs(40000, 150, 180, seed=93, n_frac=0.02)
+    batch = bwg.JobBatch.from_pairs(qs, ts)
+    mat = bwg.default_mat()
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    got = gpu_ctx.align2_arrays(batch, bwg.ksw_params(5, mat, 6, 1, 6, 1, 20, 1))
+    want = ob.align2_batch(batch, mat, 6, 1, 6, 1, 20, 1, nthreads=16)
+    assert _first_bad(got, want) is None, _first_bad(got, want)
+    assert (got["qb"] >= 0).sum() > 30000  # most jobs ran the reverse pass
+
+
+def test_global_one_full_bin(gpu_ctx):
+    # 40k 100-base regions with the reference's band: one bin of the global group kernel
+    _say("large bins: global 40000 regions")
+    batch, w = bwg.generate_global_jobs(40000, seed=95, read_lens=(100,))
+    mat = bwg.default_mat()
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    got_s, got_n, got_c = gpu_ctx.global_arrays(batch, bwg.global_params(5, mat, 6, 1, 6, 1), w)
+    want_s, want_n, want_c = ob.global_batch(batch, w, mat, 6, 1, 6, 1)
+    assert np.array_equal(got_s, want_s)
+    assert np.array_equal(got_n, want_n.astype(np.uint32))
+    assert np.array_equal(got_c, want_c)
