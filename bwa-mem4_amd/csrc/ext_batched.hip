@@ -412,10 +412,20 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
     const int lane = threadIdx.x;
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
+    // 64 consecutive slots per wavefront: dealt round-robin (wavefront w takes 64w, 64(w + waves),
+    // ...) up to the grid's last round, which comes from the queue counter (see GroupSlots)
+    const uint32_t stride = gridDim.x * 64u;
+    const uint32_t static_end = count > stride ? (count - stride) & ~63u : 0u;
+    uint32_t next = blockIdx.x * 64u;
     for (;;) {
         uint32_t slot = 0;
-        if (lane == 0) slot = atomicAdd(head, 64u);
-        slot = __shfl(slot, 0, 64);
+        if (next < static_end) {
+            slot = next;
+            next += stride;
+        } else {
+            if (lane == 0) slot = static_end + atomicAdd(head, 64u);
+            slot = __shfl(slot, 0, 64);
+        }
         if (slot >= count) break;
         const uint32_t my = slot + lane;
         if (my < count) {

@@ -59,6 +59,30 @@ def test_extension_one_full_bin(gpu_ctx):
     assert _first_bad(got, want) is None, _first_bad(got, want)
 
 
+def test_extension_short_queries_full_bin(gpu_ctx):
+    # 600k 24-base queries (one job per lane): more jobs than the thread kernel's grid has lanes,
+    # fast path off; the arena is built in one piece
+    _say("large bins: extension 600000 jobs, qlen 24")
+    n, L, T = 600000, 24, 124
+    rng = np.random.default_rng(94)
+    q = rng.integers(0, 4, (n, L)).astype(np.uint8)
+    t = rng.integers(0, 4, (n, T)).astype(np.uint8)
+    t[:, :L] = np.where(rng.random((n, L)) < 0.03, (q + 1) % 4, q)
+    arena = np.concatenate([q, t], axis=1).reshape(-1)
+    base = np.arange(n, dtype=np.uint64) * (L + T)
+    batch = bwg.JobBatch(arena, base, np.full(n, L), base + L, np.full(n, T),
+                         h0=rng.integers(19, 60, n).astype(np.int32))
+    mat = bwg.default_mat()
+    gpu_ctx.set_form(bwg.FORM_BATCHED)
+    gpu_ctx.set_option(bwg.OPT_UNGAPPED, 0)
+    try:
+        got = gpu_ctx.extend_arrays(batch, bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100))
+    finally:
+        gpu_ctx.set_option(bwg.OPT_UNGAPPED, 1)
+    want = ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=16)
+    assert _first_bad(got, want) is None, _first_bad(got, want)
+
+
 def test_rescue_one_full_bin(gpu_ctx):
     # 40k 150-base mates against 180-base windows: one packed bin, whose reverse pass deals its
     # slots; 2% of the mates carry an N base
