@@ -8,7 +8,8 @@
 //    kr <= K is chosen per row as the smallest odd width that covers the widest window among the
 //    wavefront's four groups (odd keeps the 16 lanes' LDS accesses in distinct banks; K bounds
 //    the registers).  The eh arrays of the reference live in LDS over absolute columns 0..qlen, so
-//    columns outside the window keep their stale values exactly as in the reference.  M and E depend only on the previous row; F is the max-plus recurrence
+//    columns outside the window keep their stale values exactly as in the reference.  M and E
+//    depend only on the previous row; F is the max-plus recurrence
 //    F(j+1) = max(F(j) - e_ins, max(M(j) - oe_ins, 0)), evaluated in two passes: each lane runs it
 //    over its block from F_in = 0, the blocks are combined by an exclusive max-scan across the row
 //    (F(l) = max_{m<l} c_m - (l-1-m)*kr*e_ins), and a second pass applies each lane's incoming F.
@@ -17,6 +18,9 @@
 //
 //  * ext_thread_kernel<QCAP>: one job per lane for short queries (qlen <= QCAP), the scalar
 //    recurrence with the eh arrays in LDS laid out [column][lane].
+//
+// Both deal most of their queue's slots round-robin to the grid (GroupSlots; the thread kernel in
+// 64-slot chunks per wavefront) and take the grid's last slots from the queue's counter.
 #include <algorithm>
 #include <atomic>
 
