@@ -158,7 +158,8 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
         }
         key = row_allmax(key);
         const int32_t imax = key >> kKeyShift;
-        if (!REV && __builtin_expect(g == 0, 1)) rm[64 * i] = imax;  // under the exec mask, no branch
+        if (!REV && __builtin_expect(g == 0, 1)) *rm = imax;  // under the exec mask, no branch
+        if (!REV) rm += 64;  // row i+1's slot (a running pointer: one 64-bit add a row)
         bool done = false;
         if (imax > gmax) {
             gmax = imax;
@@ -333,7 +334,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         // ---- one DP row ----------------------------------------------------------------------
         const int32_t tc = tnext;
         if (i + 1 < tlen) tnext = REV ? t[tlen - 2 - i] : t[i + 1];
-        const uint32_t *pr = prof + (size_t)tc * KH * 256 + threadIdx.x;
+        const uint32_t *pr = prof + __umul24((uint32_t)tc, (uint32_t)(KH * 256)) + threadIdx.x;
         pk16 s2[KH];
 #pragma unroll
         for (int p = 0; p < KH; ++p) s2[p] = pk_from(pr[p * 256]);
@@ -373,7 +374,8 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         const uint32_t kbest = (uint32_t)max((int32_t)key2.x, (int32_t)key2.y);  // >= 0
         const int32_t key = row_allmax((int32_t)(__umul24(kbest >> 6, 960u) + kbest + g64));
         const int32_t imax = key >> 10;
-        if (!REV && __builtin_expect(g == 0, 1)) rm[64 * i] = imax;  // under the exec mask, no branch
+        if (!REV && __builtin_expect(g == 0, 1)) *rm = imax;  // under the exec mask, no branch
+        if (!REV) rm += 64;  // row i+1's slot (a running pointer: one 64-bit add a row)
         bool done = false;
         if (imax > gmax) {
             gmax = imax;
