@@ -104,6 +104,8 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
     // the lane's codes of the row before, not yet stored, for its pn window columns (0: none)
     int32_t pn = 0;
     uint32_t *pzr = nullptr;
+    uint32_t *zrow = nullptr;  // this row's first code dword of the lane (code_index(i, g, 0, DW))
+    int32_t h1d = 0;           // -(o_del + e_del*(i+1)), carried row to row
     uint32_t pzw[DW];
 #pragma unroll
     for (int m = 0; m < DW; ++m) pzw[m] = 0u;
@@ -139,6 +141,8 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
                     }
                 }
                 i = 0;
+                h1d = -(sc.o_del + sc.e_del);  // H(i, -1) while the band touches column 0, row 0
+                zrow = zj + code_index(0, g, 0, DW);
                 tcur = tlen > 0 ? t[0] : 0;
                 active = tlen > 0;
                 if (!active && g == 0)
@@ -152,7 +156,7 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
         // ---- one DP row ----------------------------------------------------------------------
         const int32_t beg = max(i - w, 0);
         const int32_t end = min(i + w + 1, qlen);
-        const int32_t h1_init = beg == 0 ? -(sc.o_del + sc.e_del * (i + 1)) : kMinusInf;
+        const int32_t h1_init = beg == 0 ? h1d : kMinusInf;  // -(o_del + e_del*(i+1)) at column 0
         const int32_t base = beg + g * K;
         const int32_t nval = min(max(end - base, 0), K);
         const int2 *ehb = (const int2 *)eh + base;
@@ -231,7 +235,10 @@ __global__ __launch_bounds__(256) void glb_group_kernel(const uint8_t *__restric
         }
         // this row's codes are stored at the start of the next row, or now if the job ends
         pn = nval;
-        pzr = zj + code_index(i, g, 0, DW);
+        pzr = zrow;
+        // next row: the next dword of the tile, or the first row of the next tile
+        zrow += ((i + 1) & 7) ? 1 : 16 * DW * 8 - 7;
+        h1d -= e_del;
 #pragma unroll
         for (int m = 0; m < DW; ++m) pzw[m] = zw[m];
         wave_lds_sync();
