@@ -33,8 +33,12 @@ BWG_OK, BWG_EIO, BWG_E2BIG, BWG_EAGAIN, BWG_ENOMEM, BWG_ENODEV, BWG_EINVAL = 0, 
 FORM_PLAIN, FORM_BATCHED = 0, 1
 OPT_FORM, OPT_CONCURRENT_BINS, OPT_GRAPHS, OPT_EXT_THREAD_MAX_Q = 1, 2, 3, 4
 OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP, OPT_EXT_SCORE_WIDTH, OPT_UNGAPPED = 5, 6, 7, 8, 9
-OPT_KSW_SCORE_WIDTH = 10
+OPT_KSW_SCORE_WIDTH, OPT_LANES, OPT_POOL_CACHE_MB, OPT_POOL_PEAK_RESET = 10, 11, 12, 13
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
+STAT_POOL_IN_USE, STAT_POOL_CACHED, STAT_POOL_PEAK = 105, 106, 107
+# host wall time of the phases of the last synchronous extension / rescue call (ns)
+STAT_PHASES = {"validate": 110, "stage": 111, "plan": 112, "kernels": 113, "fetch": 114, "total": 115}
+MAX_JOB_LEN = 1 << 24
 
 # Every symbol include/bwg.h declares.
 EXPORTED_SYMBOLS = (
@@ -265,6 +269,10 @@ class Context:
 
     def stat(self, key):
         return int(_lib().bwg_get_stat(self.h, key))
+
+    def phases(self):
+        """Host wall time (seconds) of the phases of the last synchronous extension or rescue call."""
+        return {k: self.stat(v) * 1e-9 for k, v in STAT_PHASES.items()}
 
     def tune_report(self):
         """The autotuner's table: one line per tuned length class, chosen mapping starred."""
@@ -681,7 +689,8 @@ class Chunk(C.Structure):
 
 class ServerCounters(C.Structure):
     _fields_ = [("chunks", C.c_uint64), ("jobs", C.c_uint64), ("epochs", C.c_uint64),
-                ("peak_device_bytes", C.c_uint64), ("seconds", C.c_double), ("p50_ms", C.c_double),
+                ("peak_device_bytes", C.c_uint64), ("peak_pool_bytes", C.c_uint64),
+                ("seconds", C.c_double), ("p50_ms", C.c_double),
                 ("p99_ms", C.c_double), ("mean_ms", C.c_double), ("max_ms", C.c_double)]
 
 
@@ -725,6 +734,11 @@ class Server:
         TARGET_FROM_REF).  Returns a ticket; the arrays stay referenced until wait()."""
         is_ext = isinstance(params, ExtParams)
         out = np.zeros(len(batch), dtype=EXT_DTYPE if is_ext else KSW_DTYPE)
+        return self.submit_into(batch, params, out, t_off)
+
+    def submit_into(self, batch, params, out, t_off=None):
+        """submit() with a caller-owned result array (kept referenced by the caller)."""
+        is_ext = isinstance(params, ExtParams)
         t = np.ascontiguousarray(batch.t_off if t_off is None else t_off, dtype=np.uint64)
         ch = Chunk(CHUNK_EXTEND if is_ext else CHUNK_RESCUE,
                    C.addressof(params) if is_ext else None, None if is_ext else C.addressof(params),
@@ -741,6 +755,14 @@ class Server:
         _, _, _, out = self._live.pop(ticket)
         _check(rc, "bwg_server_wait")
         return out
+
+    def wait_rc(self, ticket):
+        """bwg_server_wait's return code (the inputs stay referenced by this object)."""
+        return _server_lib().bwg_server_wait(self.h, ticket)
+
+    def poll_rc(self, ticket):
+        """bwg_server_poll's return code."""
+        return _server_lib().bwg_server_poll(self.h, ticket)
 
     def counters(self, reset=False):
         c = ServerCounters()
@@ -806,9 +828,10 @@ def serve_run(ctx, server, rate_jobs_s, producers=8, duration_s=5.0, seed=42, us
     W.bwgw_serve_run.restype = C.c_int
     W.bwgw_serve_run.argtypes = [_P, _P, C.c_double, C.c_int, C.c_double, C.c_uint64, C.c_int, C.c_int,
                                  C.c_int, _P]
-    out = (C.c_double * 11)()
+    out = (C.c_double * 14)()
     _check(W.bwgw_serve_run(ctx.h, server.h, float(rate_jobs_s), int(producers), float(duration_s), seed,
                             1 if use_ref else 0, int(pool_chunks), 1 if verify else 0, out), "bwgw_serve_run")
     keys = ("p50_ms", "p99_ms", "mean_ms", "max_ms", "jobs_per_s", "chunks", "jobs", "epochs",
-            "mismatched_chunks", "failed_chunks", "peak_device_bytes")
+            "mismatched_chunks", "failed_chunks", "peak_device_bytes", "peak_pool_bytes",
+            "ext_jobs_submitted", "rescue_jobs_submitted")
     return dict(zip(keys, list(out)))

@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <chrono>
 #include <climits>
 #include <cstdlib>
 #include <cstdio>
@@ -33,12 +34,21 @@ int check_mat(int32_t m, const int8_t *mat) {
 }
 
 bool codes_ok(const uint8_t *p, size_t len) {
-    uint8_t acc = 0;
-    for (size_t k = 0; k < len; ++k) acc |= (uint8_t)(p[k] > 4);
-    return acc == 0;
+    // byte-wise max in blocks, so that the loop vectorises and ends early on a bad block
+    size_t k = 0;
+    for (; k + 4096 <= len; k += 4096) {
+        uint8_t mx = 0;
+        for (size_t r = 0; r < 4096; ++r) mx = std::max(mx, p[k + r]);
+        if (mx > 4) return false;
+    }
+    uint8_t mx = 0;
+    for (; k < len; ++k) mx = std::max(mx, p[k]);
+    return mx <= 4;
 }
 
-// Shared validation of the arena description.
+// Shared validation of the arena description.  The base codes are checked over the whole arena
+// first, one pass at memory speed; only an arena holding some code above 4 is checked job by job
+// (such a byte may lie outside every job).
 int check_jobs(size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
                const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len) {
     if (n == 0) return BWG_OK;
@@ -46,10 +56,10 @@ int check_jobs(size_t n, const uint8_t *arena, size_t arena_len, const uint64_t 
     for (size_t k = 0; k < n; ++k) {
         if (q_off[k] > arena_len || q_len[k] > arena_len - q_off[k]) return BWG_EINVAL;
         if (t_off[k] > arena_len || t_len[k] > arena_len - t_off[k]) return BWG_EINVAL;
-        if (q_len[k] > kMaxQlen || t_len[k] > kMaxTlen) return BWG_E2BIG;
-        if (!codes_ok(arena + q_off[k], q_len[k]) || !codes_ok(arena + t_off[k], t_len[k]))
-            return BWG_EINVAL;
     }
+    if (codes_ok(arena, arena_len)) return BWG_OK;
+    for (size_t k = 0; k < n; ++k)
+        if (!codes_ok(arena + q_off[k], q_len[k]) || !codes_ok(arena + t_off[k], t_len[k])) return BWG_EINVAL;
     return BWG_OK;
 }
 
@@ -88,7 +98,8 @@ int bwg_create(int device, bwg_ctx **out) {
     if (!c) return BWG_ENOMEM;
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
-    if (hipSetDevice(device) != hipSuccess || c->ss.create() != BWG_OK) {
+    if (hipSetDevice(device) != hipSuccess || c->ss.create() != BWG_OK ||
+        hipEventCreate(&c->na_ev[0]) != hipSuccess || hipEventCreate(&c->na_ev[1]) != hipSuccess) {
         bwg_destroy(c);
         return BWG_EIO;
     }
@@ -100,6 +111,11 @@ void bwg_destroy(bwg_ctx *c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     c->ss.destroy();
+    for (auto &l : c->lanes) l->destroy();
+    c->lanes.clear();
+    c->free_lanes.clear();
+    for (hipEvent_t ev : c->na_ev)
+        if (ev) (void)hipEventDestroy(ev);
     c->pool.release_all();
     if (c->d_ref2) (void)hipFree(c->d_ref2);
     if (c->d_contig_off) (void)hipFree(c->d_contig_off);
@@ -111,44 +127,60 @@ const char *bwg_name(const bwg_ctx *) { return kName; }
 
 int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
     if (!c) return BWG_EINVAL;
+    std::lock_guard<std::mutex> g(c->opt_mu);
+    Tuning &t = c->tuning;
     switch (key) {
     case BWG_OPT_FORM:
         if (v != BWG_FORM_PLAIN && v != BWG_FORM_BATCHED) return BWG_EINVAL;
-        c->form = (int)v;
+        t.form = (int)v;
         return BWG_OK;
     case BWG_OPT_CONCURRENT_BINS:
-        c->tuning.concurrent_bins = v != 0;
+        t.concurrent_bins = v != 0;
         return BWG_OK;
     case BWG_OPT_GRAPHS:
-        c->tuning.use_graphs = v != 0;
+        t.use_graphs = v != 0;
         return BWG_OK;
     case BWG_OPT_EXT_THREAD_MAX_Q:
         if (v < 0 || v > 32) return BWG_EINVAL;
-        c->tuning.ext_thread_max_q16 = (int)std::min<int64_t>(v, 16);
-        c->tuning.ext_thread_max_q32 = (int)v;
+        t.ext_thread_max_q16 = (int)std::min<int64_t>(v, 16);
+        t.ext_thread_max_q32 = (int)v;
         return BWG_OK;
     case BWG_OPT_AUTOTUNE:
-        c->tuning.autotune = v != 0;
+        t.autotune = v != 0;
         c->tuner.clear();
         return BWG_OK;
     case BWG_OPT_EXT_K_STEP:
-        c->tuning.ext_k_step = v != 0;
+        t.ext_k_step = v != 0;
         return BWG_OK;
     case BWG_OPT_KSW_K_STEP:
-        c->tuning.ksw_k_step = v != 0;
+        t.ksw_k_step = v != 0;
         return BWG_OK;
     case BWG_OPT_UNGAPPED:
-        c->tuning.ungapped = v != 0;
+        t.ungapped = v != 0;
         return BWG_OK;
     case BWG_OPT_EXT_SCORE_WIDTH:
         if (v != 16 && v != 32) return BWG_EINVAL;
-        c->tuning.ext_force_wide = v == 32;
+        t.ext_force_wide = v == 32;
         c->tuner.clear();
         return BWG_OK;
     case BWG_OPT_KSW_SCORE_WIDTH:
         if (v != 16 && v != 32) return BWG_EINVAL;
-        c->tuning.ksw_force_wide = v == 32;
+        t.ksw_force_wide = v == 32;
         c->tuner.clear();
+        return BWG_OK;
+    case BWG_OPT_LANES: {
+        if (v < 1 || v > 16) return BWG_EINVAL;
+        std::lock_guard<std::mutex> lg(c->lane_mu);
+        c->max_lanes = (int)v;  // lanes already created stay; no more than v are handed out anew
+        return BWG_OK;
+    }
+    case BWG_OPT_POOL_CACHE_MB:
+        if (v < 0) return BWG_EINVAL;
+        if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+        c->pool.set_cache_cap((size_t)v << 20);
+        return BWG_OK;
+    case BWG_OPT_POOL_PEAK_RESET:
+        c->pool.reset_peak();
         return BWG_OK;
     default: return BWG_EINVAL;
     }
@@ -167,22 +199,34 @@ int bwg_tune_report(const bwg_ctx *c, char *buf, size_t len) {
 
 int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     if (!c) return -1;
+    const Tuning t = c->tuning_snapshot();
     switch (key) {
-    case BWG_OPT_FORM: return c->form;
-    case BWG_OPT_CONCURRENT_BINS: return c->tuning.concurrent_bins;
-    case BWG_OPT_GRAPHS: return c->tuning.use_graphs;
-    case BWG_OPT_EXT_THREAD_MAX_Q: return c->tuning.ext_thread_max_q32;
-    case BWG_OPT_AUTOTUNE: return c->tuning.autotune;
-    case BWG_OPT_EXT_K_STEP: return c->tuning.ext_k_step;
-    case BWG_OPT_KSW_K_STEP: return c->tuning.ksw_k_step;
-    case BWG_OPT_EXT_SCORE_WIDTH: return c->tuning.ext_force_wide ? 32 : 16;
-    case BWG_OPT_UNGAPPED: return c->tuning.ungapped;
-    case BWG_OPT_KSW_SCORE_WIDTH: return c->tuning.ksw_force_wide ? 32 : 16;
+    case BWG_OPT_FORM: return t.form;
+    case BWG_OPT_CONCURRENT_BINS: return t.concurrent_bins;
+    case BWG_OPT_GRAPHS: return t.use_graphs;
+    case BWG_OPT_EXT_THREAD_MAX_Q: return t.ext_thread_max_q32;
+    case BWG_OPT_AUTOTUNE: return t.autotune;
+    case BWG_OPT_EXT_K_STEP: return t.ext_k_step;
+    case BWG_OPT_KSW_K_STEP: return t.ksw_k_step;
+    case BWG_OPT_EXT_SCORE_WIDTH: return t.ext_force_wide ? 32 : 16;
+    case BWG_OPT_UNGAPPED: return t.ungapped;
+    case BWG_OPT_KSW_SCORE_WIDTH: return t.ksw_force_wide ? 32 : 16;
+    case BWG_OPT_LANES: return c->max_lanes;
+    case BWG_OPT_POOL_CACHE_MB: return (int64_t)(c->pool.cache_cap() >> 20);
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
-    case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns;
-    case BWG_STAT_LAST_MAIN_NS: return c->last_main_ns;
-    case BWG_STAT_LAST_MAIN_LAUNCHES: return c->last_main_launches;
+    case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns.load();
+    case BWG_STAT_LAST_MAIN_NS: return c->last_main_ns.load();
+    case BWG_STAT_LAST_MAIN_LAUNCHES: return c->last_main_launches.load();
+    case BWG_STAT_POOL_IN_USE: return (int64_t)c->pool.bytes_in_use();
+    case BWG_STAT_POOL_CACHED: return (int64_t)c->pool.bytes_cached();
+    case BWG_STAT_POOL_PEAK: return (int64_t)c->pool.peak_in_use();
+    case BWG_STAT_PHASE_VALIDATE_NS: return c->phase_ns[kPhValidate].load();
+    case BWG_STAT_PHASE_STAGE_NS: return c->phase_ns[kPhStage].load();
+    case BWG_STAT_PHASE_PLAN_NS: return c->phase_ns[kPhPlan].load();
+    case BWG_STAT_PHASE_KERNELS_NS: return c->phase_ns[kPhKernels].load();
+    case BWG_STAT_PHASE_FETCH_NS: return c->phase_ns[kPhFetch].load();
+    case BWG_STAT_PHASE_TOTAL_NS: return c->phase_ns[kPhTotal].load();
     default: return -1;
     }
 }
@@ -220,7 +264,13 @@ int bwg_hbm_copy(bwg_ctx *c, size_t bytes, int reps, double *gb_per_s) {
 
 namespace {
 
-// Preconditions of ksw_extend2 (crates/bwa-extend/src/lib.rs:181-195, sw.rs:203-230).
+using Clock = std::chrono::steady_clock;
+inline int64_t ns_between(Clock::time_point a, Clock::time_point b) {
+    return std::chrono::duration_cast<std::chrono::nanoseconds>(b - a).count();
+}
+
+// Preconditions of ksw_extend2 (crates/bwa-extend/src/lib.rs:181-195, sw.rs:203-230) and the size
+// limits of include/bwg.h.
 int check_ext(const bwg_ext_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
               const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0) {
     int rc = check_mat(p->m, p->mat);
@@ -229,60 +279,32 @@ int check_ext(const bwg_ext_params *p, size_t n, const uint8_t *arena, size_t ar
     rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
     if (rc) return rc;
     if (n && !h0) return BWG_EINVAL;
-    for (size_t k = 0; k < n; ++k)
+    const ExtScoring sc = ext_scoring(*p);
+    for (size_t k = 0; k < n; ++k) {
         if (h0[k] <= 0 || q_len[k] < 1) return BWG_EINVAL;
+        if (!ext_job_fits(sc, q_len[k], t_len[k], h0[k])) return BWG_E2BIG;
+    }
     return BWG_OK;
 }
 
-}  // namespace
-
-extern "C" {
-
-int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
-                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
-                        const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
-                        bwg_batch **out) {
-    if (!c || !p || !out) return BWG_EINVAL;
-    *out = nullptr;
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+// Preconditions of ksw_align2 / batched_ksw_align2 (crates/bwa-extend/src/sw.rs:1108-1120,
+// crates/bwa-neon/src/matesw.rs:299-309) and the size limits.
+int check_ksw(const bwg_ksw_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
+              const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len) {
+    int rc = check_mat(p->m, p->mat);
     if (rc) return rc;
-    ExtScoring sc = ext_scoring(*p);
-    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, c->ss.main, out);
-    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));  // the caller may free its buffers
-    return rc;
-}
-
-int bwg_batch_stage_side(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,
-                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-                         const uint32_t *t_len, const int32_t *h0, const int32_t *prev, bwg_batch **out) {
-    if (!c || !p || !out) return BWG_EINVAL;
-    *out = nullptr;
-    if (max_band_try < 1 || max_band_try > 8 || p->w < 0) return BWG_EINVAL;
-    if (p->w > (INT32_MAX >> (max_band_try - 1))) return BWG_E2BIG;
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
     if (rc) return rc;
-    if (n && !prev) return BWG_EINVAL;
-    ExtScoring sc = ext_scoring(*p);
-    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    rc = stage_side(c, sc, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev, c->ss.main,
-                    out);
-    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
-    return rc;
+    const KswScoring sc = ksw_scoring(*p);
+    for (size_t k = 0; k < n; ++k)
+        if (!ksw_job_fits(sc, q_len[k], t_len[k])) return BWG_E2BIG;
+    return BWG_OK;
 }
 
-int bwg_batch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
-    if (!c || !b || (!w_out && b->n) || !b->side_tries) return BWG_EINVAL;
-    return fetch_side_bands(c, b, w_out);
-}
-
-int bwg_batch_stage_global(bwg_ctx *c, const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len,
-                           const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
-                           const int32_t *w, bwg_batch **out) {
-    if (!c || !p || !out || (n && !w)) return BWG_EINVAL;
-    *out = nullptr;
+int check_global(const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
+                 const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, const int32_t *w) {
+    if (n && !w) return BWG_EINVAL;
     int rc = check_mat(p->m, p->mat);
     if (rc) return rc;
     if (p->e_del < 1 || p->e_ins < 1 || p->o_del < 0 || p->o_ins < 0) return BWG_EINVAL;
@@ -297,43 +319,315 @@ int bwg_batch_stage_global(bwg_ctx *c, const bwg_global_params *p, size_t n, con
         if (p->o_del + (int64_t)p->e_del * span >= (1 << 28) || p->o_ins + (int64_t)p->e_ins * span >= (1 << 28))
             return BWG_E2BIG;
     }
+    return BWG_OK;
+}
+
+GlobalScoring global_scoring(const bwg_global_params &p) {
     GlobalScoring sc{};
-    for (int k = 0; k < 25; ++k) sc.mat[k] = p->mat[k];
-    sc.o_del = p->o_del;
-    sc.e_del = p->e_del;
-    sc.o_ins = p->o_ins;
-    sc.e_ins = p->e_ins;
+    for (int k = 0; k < 25; ++k) sc.mat[k] = p.mat[k];
+    sc.o_del = p.o_del;
+    sc.e_del = p.e_del;
+    sc.o_ins = p.o_ins;
+    sc.e_ins = p.e_ins;
+    return sc;
+}
+
+int check_matesw(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, const bwg_pe_orient pes[4], size_t n,
+                 const bwg_mate_anchor *anchors, size_t n_ma, const uint8_t *arena, size_t arena_len,
+                 MateRescueParams *mp) {
+    int rc = check_mat(p->m, p->mat);
+    if (rc) return rc;
+    if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
+    if (!c->d_ref2) return BWG_EINVAL;  // bwg_set_reference first
+    if (n > (size_t)(INT32_MAX / 4)) return BWG_E2BIG;
+    *mp = MateRescueParams{};
+    mp->l_pac = c->l_pac;
+    mp->min_seed_len = min_seed_len;
+    mp->n_contigs = c->n_contigs;
+    for (int r = 0; r < 4; ++r) {
+        mp->low[r] = pes[r].low;
+        mp->high[r] = pes[r].high;
+        mp->failed[r] = pes[r].failed ? 1 : 0;
+    }
+    const KswScoring sc = ksw_scoring(*p);
+    for (size_t k = 0; k < n; ++k) {
+        const bwg_mate_anchor &a = anchors[k];
+        if (a.ms_off > arena_len || a.l_ms > arena_len - a.ms_off || !codes_ok(arena + a.ms_off, a.l_ms))
+            return BWG_EINVAL;
+        if (a.ma_begin > a.ma_end || a.ma_end > n_ma) return BWG_EINVAL;
+        for (int r = 0; r < 4; ++r) {
+            if (mp->failed[r]) continue;
+            const int64_t bound = (int64_t)mp->high[r] - mp->low[r] + a.l_ms;
+            if (bound > (int64_t)kMaxJobLen || !ksw_job_fits(sc, a.l_ms, (uint32_t)std::max<int64_t>(bound, 0)))
+                return BWG_E2BIG;
+        }
+    }
+    return BWG_OK;
+}
+
+// Staged-batch calls: stage on the context's own streams under run_mu, then wait, so that the
+// caller may free its buffers on return.
+template <class Stage>
+int stage_on_context(bwg_ctx *c, Stage stage) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     std::lock_guard<std::mutex> g(c->run_mu);
-    rc = stage_global(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, w, c->ss.main, out);
+    int rc = stage(c->ss.main);
     if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
     return rc;
 }
 
+// One synchronous call on an execution lane: stage, plan, enqueue the kernels and the result
+// copies, wait, and record the phases (validation time measured by the caller).  `stage` makes the
+// batch on the lane's main stream; `fetch` enqueues (or performs) the copies of its outputs on it.
+template <class Stage, class Fetch>
+int run_on_lane(bwg_ctx *c, Clock::time_point t_begin, Clock::time_point t_valid, Stage stage, Fetch fetch) {
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    ExecLane *L = acquire_lane(c);
+    if (!L) return BWG_ENOMEM;
+    bwg_batch *b = nullptr;
+    int rc = stage(L->ss.main, &b);
+    const Clock::time_point t_staged = Clock::now();
+    if (!rc) rc = prepare_batch(c, b, b->tu.form, L->ss);
+    const Clock::time_point t_planned = Clock::now();
+    if (!rc) rc = enqueue_batch(c, b, b->tu.form, L->ss);
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(L->ss.main));
+    const Clock::time_point t_run = Clock::now();
+    if (!rc) {
+        record_run(c, b);
+        rc = fetch(b, L->ss.main);
+    }
+    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(L->ss.main));
+    const Clock::time_point t_end = Clock::now();
+    if (b) destroy_batch(c, b);
+    release_lane(c, L);
+    if (!rc) {
+        c->phase_ns[kPhValidate] = ns_between(t_begin, t_valid);
+        c->phase_ns[kPhStage] = ns_between(t_valid, t_staged);
+        c->phase_ns[kPhPlan] = ns_between(t_staged, t_planned);
+        c->phase_ns[kPhKernels] = ns_between(t_planned, t_run);
+        c->phase_ns[kPhFetch] = ns_between(t_run, t_end);
+        c->phase_ns[kPhTotal] = ns_between(t_begin, t_end);
+    }
+    return rc;
+}
+
+}  // namespace
+
+extern "C" {
+
+// ---- staged batches -------------------------------------------------------------------------
+
+int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                        bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (rc) return rc;
+    const ExtScoring sc = ext_scoring(*p);
+    return stage_on_context(c, [&](hipStream_t st) {
+        return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, out);
+    });
+}
+
+int bwg_batch_stage_side(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,
+                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
+                         const uint32_t *t_len, const int32_t *h0, const int32_t *prev, bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    if (max_band_try < 1 || max_band_try > 8 || p->w < 0) return BWG_EINVAL;
+    if (p->w > (INT32_MAX >> (max_band_try - 1))) return BWG_E2BIG;
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (rc) return rc;
+    if (n && !prev) return BWG_EINVAL;
+    const ExtScoring sc = ext_scoring(*p);
+    return stage_on_context(c, [&](hipStream_t st) {
+        return stage_side(c, sc, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev, st, out);
+    });
+}
+
+int bwg_batch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
+    if (!c || !b || (!w_out && b->n) || !b->side_tries) return BWG_EINVAL;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return fetch_side_bands(c, b, w_out, c->ss.main);
+}
+
+int bwg_batch_stage_global(bwg_ctx *c, const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len,
+                           const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                           const int32_t *w, bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_global(p, n, arena, arena_len, q_off, q_len, t_off, t_len, w);
+    if (rc) return rc;
+    const GlobalScoring sc = global_scoring(*p);
+    return stage_on_context(c, [&](hipStream_t st) {
+        return stage_global(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, w, st, out);
+    });
+}
+
 int bwg_batch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_cigar) {
     if (!c || !b || b->kind != 2 || (!n_cigar && b->n)) return BWG_EINVAL;
-    return fetch_cigar_lengths(c, b, n_cigar);
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return fetch_cigar_lengths(c, b, n_cigar, c->ss.main);
 }
 
 int bwg_batch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *cigar, size_t cap) {
     if (!c || !b || b->kind != 2 || (!cigar && cap)) return BWG_EINVAL;
-    return fetch_cigars(c, b, cigar, cap);
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return fetch_cigars(c, b, cigar, cap, c->ss.main);
+}
+
+int bwg_batch_stage_matesw(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, const bwg_pe_orient pes[4],
+                           size_t n, const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma,
+                           const uint8_t *arena, size_t arena_len, bwg_batch **out) {
+    if (!c || !p || !pes || !out || (n && !anchors) || (n_ma && !ma_rb)) return BWG_EINVAL;
+    *out = nullptr;
+    MateRescueParams mp;
+    int rc = check_matesw(c, p, min_seed_len, pes, n, anchors, n_ma, arena, arena_len, &mp);
+    if (rc) return rc;
+    const KswScoring sc = ksw_scoring(*p);
+    return stage_on_context(c, [&](hipStream_t st) {
+        return stage_matesw(c, sc, mp, n, anchors, ma_rb, n_ma, arena, arena_len, st, out);
+    });
+}
+
+int bwg_batch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win) {
+    if (!c || !b || !b->matesw || (!win && b->n)) return BWG_EINVAL;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return fetch_windows(c, b, win, c->ss.main);
+}
+
+int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
+                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                        const uint64_t *t_off, const uint32_t *t_len, bwg_batch **out) {
+    if (!c || !p || !out) return BWG_EINVAL;
+    *out = nullptr;
+    int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len);
+    if (rc) return rc;
+    const KswScoring sc = ksw_scoring(*p);
+    return stage_on_context(c, [&](hipStream_t st) {
+        return stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, st, out);
+    });
+}
+
+int bwg_batch_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
+    if (!c || !b || !hits) return BWG_EINVAL;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return batched_ungapped_hits(c, b, hits);
+}
+
+int bwg_batch_run(bwg_ctx *c, bwg_batch *b) {
+    if (!c || !b) return BWG_EINVAL;
+    return run_batch(c, b);
+}
+
+int bwg_batch_results(bwg_ctx *c, bwg_batch *b, void *out) {
+    if (!c || !b || (!out && b->n)) return BWG_EINVAL;
+    return fetch_results(c, b, out);
+}
+
+void bwg_batch_destroy(bwg_ctx *c, bwg_batch *b) {
+    if (!c || !b) return;
+    destroy_batch(c, b);
+}
+
+// ---- synchronous entry points (execution lanes) ---------------------------------------------
+
+int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                     size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                     const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                     bwg_ext_result *out) {
+    if (!c || !p) return BWG_EINVAL;
+    if (n == 0) return BWG_OK;
+    if (!out) return BWG_EINVAL;
+    const Clock::time_point t0 = Clock::now();
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (rc) return rc;
+    const ExtScoring sc = ext_scoring(*p);
+    return run_on_lane(
+        c, t0, Clock::now(),
+        [&](hipStream_t st, bwg_batch **b) {
+            return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, b);
+        },
+        [&](bwg_batch *b, hipStream_t st) { return enqueue_results(c, b, out, st); });
+}
+
+int bwg_extend_side_batch(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,
+                          size_t arena_len, const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
+                          const uint32_t *t_len, const int32_t *h0, const int32_t *prev, bwg_ext_result *out,
+                          int32_t *w_out) {
+    if (!c || !p || max_band_try < 1) return BWG_EINVAL;
+    if (n == 0) return BWG_OK;
+    if (!out || !w_out || !prev) return BWG_EINVAL;
+    const Clock::time_point t0 = Clock::now();
+    if (max_band_try > 8 || p->w < 0) return BWG_EINVAL;
+    if (p->w > (INT32_MAX >> (max_band_try - 1))) return BWG_E2BIG;
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (rc) return rc;
+    const ExtScoring sc = ext_scoring(*p);
+    return run_on_lane(
+        c, t0, Clock::now(),
+        [&](hipStream_t st, bwg_batch **b) {
+            return stage_side(c, sc, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev, st, b);
+        },
+        [&](bwg_batch *b, hipStream_t st) {
+            int r = enqueue_results(c, b, out, st);
+            return r ? r : fetch_side_bands(c, b, w_out, st);
+        });
+}
+
+int bwg_extend(bwg_ctx *c, const bwg_ext_params *p, const uint8_t *query, uint32_t qlen,
+               const uint8_t *target, uint32_t tlen, int32_t h0, bwg_ext_result *out) {
+    if (!c || !p || !out || (!query && qlen) || (!target && tlen)) return BWG_EINVAL;
+    std::vector<uint8_t> arena((size_t)qlen + tlen);
+    if (qlen) std::memcpy(arena.data(), query, qlen);
+    if (tlen) std::memcpy(arena.data() + qlen, target, tlen);
+    uint64_t qo = 0, to = qlen;
+    return bwg_extend_batch(c, p, 1, arena.data(), arena.size(), &qo, &qlen, &to, &tlen, &h0, out);
+}
+
+int bwg_ksw_align2_batch(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
+                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                         const uint64_t *t_off, const uint32_t *t_len, bwg_ksw_result *out) {
+    if (!c || !p) return BWG_EINVAL;
+    if (n == 0) return BWG_OK;
+    if (!out) return BWG_EINVAL;
+    const Clock::time_point t0 = Clock::now();
+    int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len);
+    if (rc) return rc;
+    const KswScoring sc = ksw_scoring(*p);
+    return run_on_lane(
+        c, t0, Clock::now(),
+        [&](hipStream_t st, bwg_batch **b) {
+            return stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, st, b);
+        },
+        [&](bwg_batch *b, hipStream_t st) { return enqueue_results(c, b, out, st); });
 }
 
 int bwg_global_batch(bwg_ctx *c, const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len,
                      const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                      const int32_t *w, int32_t *score, uint32_t *n_cigar, uint32_t *cigar, size_t cigar_cap) {
-    if (n == 0) return (c && p) ? BWG_OK : BWG_EINVAL;
+    if (!c || !p) return BWG_EINVAL;
+    if (n == 0) return BWG_OK;
     if (!score || !n_cigar) return BWG_EINVAL;
-    bwg_batch *b = nullptr;
-    int rc = bwg_batch_stage_global(c, p, n, arena, arena_len, q_off, q_len, t_off, t_len, w, &b);
+    const Clock::time_point t0 = Clock::now();
+    int rc = check_global(p, n, arena, arena_len, q_off, q_len, t_off, t_len, w);
     if (rc) return rc;
-    rc = bwg_batch_run(c, b);
-    if (!rc) rc = bwg_batch_results(c, b, score);
-    if (!rc) rc = bwg_batch_cigar_lengths(c, b, n_cigar);
-    if (!rc) rc = bwg_batch_cigars(c, b, cigar, cigar_cap);
-    bwg_batch_destroy(c, b);
-    return rc;
+    const GlobalScoring sc = global_scoring(*p);
+    return run_on_lane(
+        c, t0, Clock::now(),
+        [&](hipStream_t st, bwg_batch **b) {
+            return stage_global(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, w, st, b);
+        },
+        [&](bwg_batch *b, hipStream_t st) {
+            int r = enqueue_results(c, b, score, st);
+            if (!r) r = fetch_cigar_lengths(c, b, n_cigar, st);
+            // the run counts are filled even when the runs do not fit cigar_cap (BWG_E2BIG)
+            if (!r) r = (!cigar && cigar_cap) ? BWG_EINVAL : fetch_cigars(c, b, cigar, cigar_cap, st);
+            return r;
+        });
 }
 
 int bwg_set_reference(bwg_ctx *c, const uint8_t *fwd, int64_t l_pac, const bwg_contig *contigs,
@@ -388,154 +682,26 @@ int bwg_set_reference(bwg_ctx *c, const uint8_t *fwd, int64_t l_pac, const bwg_c
     return BWG_OK;
 }
 
-int bwg_batch_stage_matesw(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, const bwg_pe_orient pes[4],
-                           size_t n, const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma,
-                           const uint8_t *arena, size_t arena_len, bwg_batch **out) {
-    if (!c || !p || !pes || !out || (n && !anchors) || (n_ma && !ma_rb)) return BWG_EINVAL;
-    *out = nullptr;
-    int rc = check_mat(p->m, p->mat);
-    if (rc) return rc;
-    if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
-    if (!c->d_ref2) return BWG_EINVAL;  // bwg_set_reference first
-    if (n > (size_t)(INT32_MAX / 4)) return BWG_E2BIG;
-    MateRescueParams mp{};
-    mp.l_pac = c->l_pac;
-    mp.min_seed_len = min_seed_len;
-    mp.n_contigs = c->n_contigs;
-    for (int r = 0; r < 4; ++r) {
-        mp.low[r] = pes[r].low;
-        mp.high[r] = pes[r].high;
-        mp.failed[r] = pes[r].failed ? 1 : 0;
-    }
-    for (size_t k = 0; k < n; ++k) {
-        const bwg_mate_anchor &a = anchors[k];
-        if (a.l_ms > kMaxQlen) return BWG_E2BIG;
-        if (a.ms_off > arena_len || a.l_ms > arena_len - a.ms_off || !codes_ok(arena + a.ms_off, a.l_ms))
-            return BWG_EINVAL;
-        if (a.ma_begin > a.ma_end || a.ma_end > n_ma) return BWG_EINVAL;
-        for (int r = 0; r < 4; ++r)
-            if (!mp.failed[r] && (int64_t)mp.high[r] - mp.low[r] + a.l_ms > (int64_t)kMaxTlen) return BWG_E2BIG;
-    }
-    KswScoring sc = ksw_scoring(*p);
-    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    rc = stage_matesw(c, sc, mp, n, anchors, ma_rb, n_ma, arena, arena_len, c->ss.main, out);
-    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
-    return rc;
-}
-
-int bwg_batch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win) {
-    if (!c || !b || !b->matesw || (!win && b->n)) return BWG_EINVAL;
-    return fetch_windows(c, b, win);
-}
-
 int bwg_matesw_batch(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, const bwg_pe_orient pes[4], size_t n,
                      const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma, const uint8_t *arena,
                      size_t arena_len, bwg_rescue_window *win, bwg_ksw_result *out) {
-    if (n == 0) return (c && p && pes) ? BWG_OK : BWG_EINVAL;
-    if (!win || !out) return BWG_EINVAL;
-    bwg_batch *b = nullptr;
-    int rc = bwg_batch_stage_matesw(c, p, min_seed_len, pes, n, anchors, ma_rb, n_ma, arena, arena_len, &b);
+    if (!c || !p || !pes) return BWG_EINVAL;
+    if (n == 0) return BWG_OK;
+    if (!win || !out || !anchors || (n_ma && !ma_rb)) return BWG_EINVAL;
+    const Clock::time_point t0 = Clock::now();
+    MateRescueParams mp;
+    int rc = check_matesw(c, p, min_seed_len, pes, n, anchors, n_ma, arena, arena_len, &mp);
     if (rc) return rc;
-    rc = bwg_batch_run(c, b);
-    if (!rc) rc = bwg_batch_results(c, b, out);
-    if (!rc) rc = bwg_batch_windows(c, b, win);
-    bwg_batch_destroy(c, b);
-    return rc;
-}
-
-int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
-                        size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
-                        const uint64_t *t_off, const uint32_t *t_len, bwg_batch **out) {
-    if (!c || !p || !out) return BWG_EINVAL;
-    *out = nullptr;
-    int rc = check_mat(p->m, p->mat);
-    if (rc) return rc;
-    if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
-    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
-    if (rc) return rc;
-    KswScoring sc = ksw_scoring(*p);
-    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    rc = stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, c->ss.main, out);
-    if (!rc) rc = bwg_map_hip(hipStreamSynchronize(c->ss.main));
-    return rc;
-}
-
-int bwg_batch_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
-    if (!c || !b || !hits) return BWG_EINVAL;
-    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    return batched_ungapped_hits(c, b, hits);
-}
-
-int bwg_batch_run(bwg_ctx *c, bwg_batch *b) {
-    if (!c || !b) return BWG_EINVAL;
-    return run_batch(c, b);
-}
-
-int bwg_batch_results(bwg_ctx *c, bwg_batch *b, void *out) {
-    if (!c || !b || (!out && b->n)) return BWG_EINVAL;
-    return fetch_results(c, b, out);
-}
-
-void bwg_batch_destroy(bwg_ctx *c, bwg_batch *b) {
-    if (!c || !b) return;
-    destroy_batch(c, b);
-}
-
-int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
-                     size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
-                     const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
-                     bwg_ext_result *out) {
-    if (n == 0) return (c && p) ? BWG_OK : BWG_EINVAL;
-    bwg_batch *b = nullptr;
-    int rc = bwg_batch_stage_ext(c, p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, &b);
-    if (rc) return rc;
-    rc = bwg_batch_run(c, b);
-    if (!rc) rc = bwg_batch_results(c, b, out);
-    bwg_batch_destroy(c, b);
-    return rc;
-}
-
-int bwg_extend_side_batch(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,
-                          size_t arena_len, const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-                          const uint32_t *t_len, const int32_t *h0, const int32_t *prev, bwg_ext_result *out,
-                          int32_t *w_out) {
-    if (n == 0) return (c && p && max_band_try >= 1) ? BWG_OK : BWG_EINVAL;
-    if (!out || !w_out) return BWG_EINVAL;
-    bwg_batch *b = nullptr;
-    int rc = bwg_batch_stage_side(c, p, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev,
-                                  &b);
-    if (rc) return rc;
-    rc = bwg_batch_run(c, b);
-    if (!rc) rc = bwg_batch_results(c, b, out);
-    if (!rc) rc = bwg_batch_side_bands(c, b, w_out);
-    bwg_batch_destroy(c, b);
-    return rc;
-}
-
-int bwg_extend(bwg_ctx *c, const bwg_ext_params *p, const uint8_t *query, uint32_t qlen,
-               const uint8_t *target, uint32_t tlen, int32_t h0, bwg_ext_result *out) {
-    if (!c || !p || !out || (!query && qlen) || (!target && tlen)) return BWG_EINVAL;
-    std::vector<uint8_t> arena((size_t)qlen + tlen);
-    if (qlen) std::memcpy(arena.data(), query, qlen);
-    if (tlen) std::memcpy(arena.data() + qlen, target, tlen);
-    uint64_t qo = 0, to = qlen;
-    return bwg_extend_batch(c, p, 1, arena.data(), arena.size(), &qo, &qlen, &to, &tlen, &h0, out);
-}
-
-int bwg_ksw_align2_batch(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uint8_t *arena,
-                         size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
-                         const uint64_t *t_off, const uint32_t *t_len, bwg_ksw_result *out) {
-    if (n == 0) return (c && p) ? BWG_OK : BWG_EINVAL;
-    bwg_batch *b = nullptr;
-    int rc = bwg_batch_stage_ksw(c, p, n, arena, arena_len, q_off, q_len, t_off, t_len, &b);
-    if (rc) return rc;
-    rc = bwg_batch_run(c, b);
-    if (!rc) rc = bwg_batch_results(c, b, out);
-    bwg_batch_destroy(c, b);
-    return rc;
+    const KswScoring sc = ksw_scoring(*p);
+    return run_on_lane(
+        c, t0, Clock::now(),
+        [&](hipStream_t st, bwg_batch **b) {
+            return stage_matesw(c, sc, mp, n, anchors, ma_rb, n_ma, arena, arena_len, st, b);
+        },
+        [&](bwg_batch *b, hipStream_t st) {
+            int r = enqueue_results(c, b, out, st);
+            return r ? r : fetch_windows(c, b, win, st);
+        });
 }
 
 }  // extern "C"

@@ -169,19 +169,19 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
 
 // Persistent grid of a bin: enough workgroups to fill every CU at the kernel's occupancy, but no
 // more than the bin has work for.
-int ext_grid(const bwg_ctx *c, int kind, int param, bool narrow, int qcap, uint32_t count) {
+int ext_grid(const bwg_ctx *c, const Tuning &tu, int kind, int param, bool narrow, int qcap, uint32_t count) {
     if (kind == kExtGroup) {
         const int per_cu = ext_group_blocks_per_cu(param, qcap, narrow);
-        return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ext_group_oversub,
+        return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ext_group_oversub,
                                                            ((int64_t)count + 15) / 16));
     }
     const int per_cu = ext_thread_blocks_per_cu(param, narrow);
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)count + 63) / 64));
 }
 
-int ksw_grid(const bwg_ctx *c, int K, bool rev, uint32_t count, bool packed = false) {
+int ksw_grid(const bwg_ctx *c, const Tuning &tu, int K, bool rev, uint32_t count, bool packed = false) {
     const int per_cu = packed ? ksw_pk_blocks_per_cu(K, rev) : ksw_group_blocks_per_cu(K, rev);
-    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * c->tuning.ksw_group_oversub,
+    return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ksw_group_oversub,
                                                        ((int64_t)count + 15) / 16));
 }
 
@@ -253,7 +253,7 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, const ExtScoring &esc, uint64_t sig
             e = launch_zero_u32(st, d, 1);
             if (e == hipSuccess) e = hipEventRecord(e0, st);
             const bool narrow = cd.width == 16;
-            const int grid = ext_grid(c, cd.kind, cd.param, narrow, qcap, S);
+            const int grid = ext_grid(c, b->tu, cd.kind, cd.param, narrow, qcap, S);
             if (e == hipSuccess) {
                 if (cd.kind == kExtGroup)
                     e = launch_ext_group(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S,
@@ -289,7 +289,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
     const ExtScoring &sc = p->esc;
     const bool retry = p->retry_round > 0;
-    const Tuning &tu = c->tuning;
+    const Tuning &tu = b->tu;
     std::vector<Item> bins;
     auto find = [&](int kind, int param, bool narrow) {
         for (size_t k = 0; k < bins.size(); ++k)
@@ -315,7 +315,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0), choice_width(n_classes, 16);
     for (int cl = 0; cl < n_classes; ++cl) {
         if (!tu.autotune) break;
-        if (!retry && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+        if (tu.tune_new && !retry && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
             const int rc = tune_ext_class(c, b, sc, sig, cl, members[cl], st);
             if (rc) return rc;
         }
@@ -355,7 +355,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         Item &it = bins[r];
         it.head = (uint32_t)r;
         if (it.kind == kExtGroup || it.kind == kExtThread) {
-            it.grid = ext_grid(c, it.kind, it.param, it.narrow, it.qcap, it.count);
+            it.grid = ext_grid(c, tu, it.kind, it.param, it.narrow, it.qcap, it.count);
         } else {
             // plain fallback: compact descriptors with per-job scratch offsets
             size_t total = 0;
@@ -366,8 +366,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
                 p->ext_plain_jobs.push_back(j);
             }
             if (total >= (1ull << 32)) return BWG_E2BIG;
-            p->d_plain_jobs = c->pool.alloc(std::max<size_t>(1, it.count) * sizeof(ExtJob));
-            p->d_plain_scratch = c->pool.alloc(std::max<size_t>(1, total) * sizeof(int2));
+            p->d_plain_jobs = batch_alloc(c, b, std::max<size_t>(1, it.count) * sizeof(ExtJob));
+            p->d_plain_scratch = batch_alloc(c, b, std::max<size_t>(1, total) * sizeof(int2));
             if (!p->d_plain_jobs || !p->d_plain_scratch) return BWG_ENOMEM;
         }
     }
@@ -407,12 +407,12 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             u.kind = kExtRetryFilter;
         }
         bins.insert(bins.begin(), u);
-        p->d_corder = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
+        p->d_corder = (uint32_t *)batch_alloc(c, b, std::max<size_t>(1, n) * 4);
         if (!p->d_corder) return BWG_ENOMEM;
     }
     p->items = bins;
-    p->d_order = (uint32_t *)c->pool.alloc(std::max<size_t>(1, n) * 4);
-    p->d_heads = (uint32_t *)c->pool.alloc(std::max<size_t>((size_t)p->n_heads, p->meta.size()) * 4);
+    p->d_order = (uint32_t *)batch_alloc(c, b, std::max<size_t>(1, n) * 4);
+    p->d_heads = (uint32_t *)batch_alloc(c, b, std::max<size_t>((size_t)p->n_heads, p->meta.size()) * 4);
     if (!p->d_order || !p->d_heads) return BWG_ENOMEM;
     hipError_t e = hipMemcpyAsync(p->d_order, p->order.data(), n * 4, hipMemcpyHostToDevice, st);
     if (e == hipSuccess && p->compact)
@@ -440,7 +440,8 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
         uint32_t tmax = 0;
         for (uint32_t r = t0; r < t1; ++r) {
             jobs[r] = b->hksw[members[(size_t)r * members.size() / S]];
-            jobs[r].aux0 = (uint32_t)(rows + (r - t0));
+            jobs[r].aux0 = (uint32_t)(rows >> 6);
+            jobs[r].aux1 = r - t0;
             ident[r] = r;
             tmax = std::max(tmax, jobs[r].tlen);
         }
@@ -455,7 +456,7 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
         t.param = k2;
         ent.cands.push_back(t);
     }
-    if (cls < 8 && !c->tuning.ksw_force_wide) {  // 16-bit-exact jobs: the packed kernel too
+    if (cls < 8 && !b->tu.ksw_force_wide) {  // 16-bit-exact jobs: the packed kernel too
         TuneCandidate t;
         t.kind = kKswFwd;
         t.param = K;
@@ -481,10 +482,10 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
             if (e == hipSuccess) {
                 if (packed)
                     e = launch_ksw_pk(st, kk, false, b->d_arena, b->d_ref, dj, d + 1, S, d, b->ksc,
-                                      (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, kk, false, S, true));
+                                      (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, b->tu, kk, false, S, true));
                 else
                     e = launch_ksw_group(st, kk, false, b->d_arena, b->d_ref, dj, d + 1, S, d, b->ksc,
-                                         (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, kk, false, S));
+                                         (bwg_ksw_result *)b->d_out, drm, ksw_grid(c, b->tu, kk, false, S));
             }
             if (e == hipSuccess) e = hipEventRecord(e1, st);
             if (e == hipSuccess) e = hipEventSynchronize(e1);
@@ -511,7 +512,7 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
 int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
     const KswScoring &sc = b->ksc;
-    const Tuning &tu = c->tuning;
+    const Tuning &tu = b->tu;
     std::vector<Item> bins;
     auto find = [&](int kind, int param, bool packed) {
         for (size_t k = 0; k < bins.size(); ++k)
@@ -532,7 +533,7 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     std::vector<std::vector<uint32_t>> members(16);
     for (size_t k = 0; k < n; ++k) {
         const int qlen = (int)b->hksw[k].qlen;
-        const int K = sc.o_ins >= 0 ? ksw_k_for(widest_pad(qlen, sc.max_sc)) : -1;
+        const int K = sc.o_ins >= 0 && b->hksw[k].tlen <= kMaxFastTlen ? ksw_k_for(widest_pad(qlen, sc.max_sc)) : -1;
         const bool pk = pk_scoring && (int64_t)qlen * mm < kKswPkMaxScore;
         cls_of[k] = (int8_t)(K < 0 ? -1 : K / 2 - 1 + (pk ? 0 : 8));
         if (K > 0) members[cls_of[k]].push_back((uint32_t)k);
@@ -546,7 +547,7 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         chosen_pk[cl] = cl < 8 && !tu.ksw_force_wide;
         if (!tu.autotune) continue;
         // a rescue-window batch's descriptors are written on the device, so it only reads the table
-        if (!b->matesw && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+        if (tu.tune_new && !b->matesw && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
             const int rc = tune_ksw_class(c, b, sig, cl, members[cl], st);
             if (rc) return rc;
         }
@@ -564,7 +565,7 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     p->order = bin_jobs(n, bins, key_of, work_of);
     // Row-maximum slots for the fast-path jobs: tiles of 64 consecutive jobs of the order array,
     // row-major within a tile ([row][64]), so the tracker kernel's 64 lanes read consecutive words.
-    // A job's slot for row i is aux0 + 64*i.
+    // A job's slot for row i is row_slot(job) + 64*i.
     p->ksw_jobs = b->hksw;
     for (auto &it : bins)
         if (it.kind == kKswFwd) p->n_fast += it.count;
@@ -573,10 +574,12 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         uint32_t tmax = 0;
         const uint32_t t1 = std::min<uint32_t>(p->n_fast, t0 + 64);
         for (uint32_t r = t0; r < t1; ++r) tmax = std::max(tmax, p->ksw_jobs[p->order[r]].tlen);
-        for (uint32_t r = t0; r < t1; ++r) p->ksw_jobs[p->order[r]].aux0 = (uint32_t)(rows + (r - t0));
+        for (uint32_t r = t0; r < t1; ++r) {
+            p->ksw_jobs[p->order[r]].aux0 = (uint32_t)(rows >> 6);
+            p->ksw_jobs[p->order[r]].aux1 = r - t0;
+        }
         rows += 64ull * tmax;
     }
-    if (rows >= (1ull << 32)) return BWG_E2BIG;
     std::vector<Item> items;
     int heads = 0;
     for (auto &it : bins) {
@@ -584,13 +587,13 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             Item f = it;
             f.stage = 0;
             f.head = (uint32_t)heads++;
-            f.grid = ksw_grid(c, it.param, false, it.count, it.narrow);
+            f.grid = ksw_grid(c, tu, it.param, false, it.count, it.narrow);
             items.push_back(f);
             Item r = it;
             r.kind = kKswRev;
             r.stage = 2;
             r.head = (uint32_t)heads++;
-            r.grid = ksw_grid(c, it.param, true, it.count, it.narrow);
+            r.grid = ksw_grid(c, tu, it.param, true, it.count, it.narrow);
             items.push_back(r);
         } else {
             Item pl = it;
@@ -794,7 +797,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
         size_t j = i;
         while (j < p->items.size() && p->items[j].stage == p->items[i].stage) ++j;
         const int nitems = (int)(j - i);
-        const bool par = c->tuning.concurrent_bins && nitems > 1;
+        const bool par = b->tu.concurrent_bins && nitems > 1;
         if (par) {
             e = hipEventRecord(ss.fork, ss.main);
             for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(ss.aux[k], ss.fork, 0);
@@ -879,6 +882,8 @@ void free_plan(bwg_ctx *c, Plan *p) {
 
 }  // namespace
 
+int plan_batched(bwg_ctx *c, bwg_batch *b, hipStream_t st) { return plan_batch(c, b, st); }
+
 int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) {
     *launches = 0;
     int rc = plan_batch(c, b, ss.main);
@@ -886,7 +891,7 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     hipError_t e = hipSuccess;
     // Capture the launch sequence once the batch has run eagerly; the eager run is also the
     // reference every later replay is compared with by the tests.
-    if (c->tuning.use_graphs && !b->graph && b->batched_runs >= 1) {
+    if (b->tu.use_graphs && !b->graph && b->batched_runs >= 1) {
         hipGraph_t g = nullptr;
         e = hipStreamBeginCapture(ss.main, hipStreamCaptureModeThreadLocal);
         int64_t nl = 0;
@@ -910,6 +915,84 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     if (e == hipSuccess) e = hipEventRecord(b->ev_end, ss.main);
     b->batched_runs++;
     return bwg_map_hip(e);
+}
+
+size_t plan_device_bytes(bwg_ctx *c, int kind, size_t n, const uint32_t *q_len, const uint32_t *t_len,
+                         const int32_t *h0, const ExtScoring *esc, const KswScoring *ksc, bool exact) {
+    Tuning tu = c->tuning_snapshot();
+    if (exact) tu.tune_new = false;
+    auto W = [exact](size_t b) {
+        return exact ? bump_round(std::max<size_t>(b, 1)) : DevicePool::worst(std::max<size_t>(b, 64));
+    };
+    // order array, the compacted queues of the ungapped pre-pass, queue heads and bin metadata
+    size_t bytes = 2 * W(n * 4) + W(64 * (kCountStride + 3) * 4);
+    if (kind == 0) {
+        size_t n_plain = 0, plain_cols = 0;
+        std::vector<uint32_t> per_class(kExtShortClasses + kNumExtKs, 0);
+        for (size_t k = 0; k < n; ++k) {
+            ExtJob j{};
+            j.qlen = q_len[k];
+            j.tlen = t_len[k];
+            j.h0 = h0[k];
+            int ki = -1;
+            const int cl = ext_class_of(j, *esc, tu, &ki);
+            if (cl == -1) {
+                ++n_plain;
+                plain_cols += (size_t)q_len[k] + 1;
+            } else if (cl >= 0) {
+                per_class[cl]++;
+            }
+        }
+        if (n_plain) bytes += W(n_plain * sizeof(ExtJob)) + W(plain_cols * sizeof(int2));
+        bool tunes = false;
+        for (int cl = 0; cl < (int)per_class.size(); ++cl)
+            tunes |= tu.autotune && tu.tune_new && per_class[cl] >= kTuneMinJobs &&
+                     !c->tuner.lookup(scoring_sig(*esc), cl, nullptr);
+        if (tunes) bytes += W((kTuneSample + 1) * 4);
+        return bytes;
+    }
+    // rescue: the planned descriptor copy, the row-maximum tiles of the fast jobs (64 jobs a tile,
+    // each tile as deep as its longest target: at most n_fast rounded up to 64 times the longest),
+    // and the plain bin's per-job state
+    const bool pk_scoring = ksw_pk_scoring_ok(*ksc);
+    const int32_t mm = std::max<int32_t>(1, mat_max(ksc->mat));
+    size_t n_fast = 0, n_plain = 0, plain_bytes = 0;
+    uint64_t rows_fast = 0;
+    uint32_t tmax_fast = 0;
+    std::vector<uint32_t> members(16, 0), cls_tmax(16, 0);
+    for (size_t k = 0; k < n; ++k) {
+        const int qlen = (int)q_len[k];
+        const int K = ksc->o_ins >= 0 && t_len[k] <= kMaxFastTlen ? ksw_k_for(widest_pad(qlen, ksc->max_sc)) : -1;
+        if (K < 0) {
+            ++n_plain;
+            const size_t b = ((size_t)ksw_padded(qlen, ksc->max_sc) + 16) * 8 + (size_t)t_len[k] * 4;
+            plain_bytes += (b + 15) & ~(size_t)15;
+            continue;
+        }
+        ++n_fast;
+        rows_fast += t_len[k];
+        tmax_fast = std::max(tmax_fast, t_len[k]);
+        const int cl = K / 2 - 1 + (pk_scoring && (int64_t)qlen * mm < kKswPkMaxScore ? 0 : 8);
+        members[cl]++;
+        cls_tmax[cl] = std::max(cls_tmax[cl], t_len[k]);
+    }
+    // Within a bin the jobs are sorted by descending target (by a key exact to 1/2048), so a tile
+    // is no deeper than the shortest target of the tile before it: the tiles hold at most the jobs'
+    // own rows (plus the key's rounding) and one tile of the longest target per bin (at most 16
+    // fast bins) and one more.
+    const uint64_t tiles = std::min<uint64_t>((n_fast + 63) / 64 * 64 * (uint64_t)tmax_fast,
+                                              rows_fast + rows_fast / 1024 + 64ull * 17 * tmax_fast);
+    bytes += W(n * sizeof(KswJob)) + W(tiles * 4);
+    if (n_plain) bytes += W(n_plain * sizeof(KswJob)) + W(plain_bytes);
+    size_t tune = 0;
+    for (int cl = 0; cl < 16; ++cl) {
+        if (!tu.autotune || !tu.tune_new || members[cl] < kTuneMinJobs ||
+            c->tuner.lookup(scoring_sig(*ksc), cl, nullptr))
+            continue;
+        const size_t S = std::min<size_t>(members[cl], kTuneSample / 2);
+        tune = std::max(tune, W((S + 1) * 4) + W(S * sizeof(KswJob)) + W((S + 63) / 64 * 64 * (size_t)cls_tmax[cl] * 4));
+    }
+    return bytes + tune;
 }
 
 const void *glb_plan_jobs(const bwg_batch *b) { return b->plan ? ((const Plan *)b->plan)->d_jobs : nullptr; }

@@ -46,8 +46,11 @@ struct ExtJob {
 struct KswJob {
     uint64_t q_off, t_off;
     uint32_t qlen, tlen;
-    uint32_t aux0, aux1;
+    uint32_t aux0, aux1;  // batched: row-maximum tile and lane; plain: scratch offset / 16 in aux0
 };
+// First row-maximum slot of a batched rescue job: tiles of 64 jobs, row-major within a tile, so a
+// job's slot for row i is row_slot + 64*i.  aux0 counts 64-word row blocks, aux1 is the lane.
+__host__ __device__ inline size_t row_slot(const KswJob &j) { return ((size_t)j.aux0 << 6) | j.aux1; }
 
 // Scoring shared by all jobs of an extension batch, plus the values derived from it once.
 struct ExtScoring {

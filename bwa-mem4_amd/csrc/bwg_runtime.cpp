@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <iterator>
 #include <new>
 
 namespace bwg {
@@ -23,29 +24,25 @@ void *DevicePool::alloc(size_t bytes) {
     const size_t sz = round(bytes);
     std::lock_guard<std::mutex> g(mu_);
     auto it = free_.lower_bound(sz);
-    if (it != free_.end() && it->first <= sz + (sz >> 1)) {
-        void *p = it->second;
-        const size_t got = it->first;
+    void *p = nullptr;
+    size_t got = sz;
+    if (it != free_.end() && it->first <= sz + (sz >> 2)) {
+        p = it->second;
+        got = it->first;
         free_.erase(it);
         cached_ -= got;
-        live_[p] = got;
-        in_use_ += got;
-        return p;
-    }
-    void *p = nullptr;
-    if (hipMalloc(&p, sz) != hipSuccess) {
+    } else if (hipMalloc(&p, sz) != hipSuccess) {
         // drop the cache and retry once
-        for (auto &kv : free_) (void)hipFree(kv.second);
-        free_.clear();
-        cached_ = 0;
         (void)hipGetLastError();
+        trim_locked(0);
         if (hipMalloc(&p, sz) != hipSuccess) {
             (void)hipGetLastError();
             return nullptr;
         }
     }
-    live_[p] = sz;
-    in_use_ += sz;
+    live_[p] = got;
+    in_use_ += got;
+    if (in_use_.load() > peak_.load()) peak_.store(in_use_.load());
     return p;
 }
 
@@ -58,6 +55,23 @@ void DevicePool::free(void *p) {
     cached_ += it->second;
     in_use_ -= it->second;
     live_.erase(it);
+    if (cached_.load() > cap_.load()) trim_locked(cap_.load());
+}
+
+void DevicePool::trim_locked(size_t cap) {
+    // largest blocks first: they return the most memory per hipFree
+    while (!free_.empty() && cached_.load() > cap) {
+        auto it = std::prev(free_.end());
+        (void)hipFree(it->second);
+        cached_ -= it->first;
+        free_.erase(it);
+    }
+}
+
+void DevicePool::set_cache_cap(size_t cap) {
+    std::lock_guard<std::mutex> g(mu_);
+    cap_ = cap;
+    trim_locked(cap);
 }
 
 void DevicePool::release_all() {
@@ -66,7 +80,8 @@ void DevicePool::release_all() {
     for (auto &kv : live_) (void)hipFree(kv.first);
     free_.clear();
     live_.clear();
-    in_use_ = cached_ = 0;
+    in_use_ = 0;
+    cached_ = 0;
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -99,16 +114,63 @@ void StreamSet::destroy() {
     fork = nullptr;
 }
 
+int ExecLane::create() { return ss.create(); }
+
+void ExecLane::destroy() {
+    ss.destroy();
+    if (pin_out) (void)hipHostFree(pin_out);
+    pin_out = nullptr;
+    pin_out_cap = 0;
+}
+
+ExecLane *acquire_lane(bwg_ctx *c) {
+    std::unique_lock<std::mutex> g(c->lane_mu);
+    for (;;) {
+        if (!c->free_lanes.empty()) {
+            ExecLane *l = c->free_lanes.back();
+            c->free_lanes.pop_back();
+            return l;
+        }
+        if ((int)c->lanes.size() < c->max_lanes) {
+            std::unique_ptr<ExecLane> l(new (std::nothrow) ExecLane());
+            if (!l) return nullptr;
+            if (l->create() != BWG_OK) {
+                l->destroy();
+                return nullptr;
+            }
+            c->lanes.push_back(std::move(l));
+            return c->lanes.back().get();
+        }
+        c->lane_cv.wait(g);
+    }
+}
+
+void release_lane(bwg_ctx *c, ExecLane *l) {
+    {
+        std::lock_guard<std::mutex> g(c->lane_mu);
+        c->free_lanes.push_back(l);
+    }
+    c->lane_cv.notify_one();
+}
+
 // ---------------------------------------------------------------------------------------------
 // staging
 // ---------------------------------------------------------------------------------------------
 namespace {
 
-bwg_batch *new_batch(int kind, size_t n) {
+// A batch object; with `ev` its timing events are borrowed (the allocation-free path), else its own.
+bwg_batch *new_batch(const bwg_ctx *c, int kind, size_t n, const hipEvent_t *ev = nullptr) {
     bwg_batch *b = new (std::nothrow) bwg_batch();
     if (!b) return nullptr;
     b->kind = kind;
     b->n = n;
+    b->tu = c->tuning_snapshot();
+    if (ev) {
+        b->ev_start = ev[0];
+        b->ev_end = ev[1];
+        b->borrowed_events = true;
+        return b;
+    }
     if (hipEventCreate(&b->ev_start) != hipSuccess || hipEventCreate(&b->ev_end) != hipSuccess) {
         if (b->ev_start) (void)hipEventDestroy(b->ev_start);
         delete b;
@@ -120,9 +182,9 @@ bwg_batch *new_batch(int kind, size_t n) {
 int stage_buffers(bwg_ctx *c, bwg_batch *b, const uint8_t *arena, size_t arena_len, const void *jobs,
                   size_t job_bytes, size_t out_bytes, hipStream_t st) {
     b->arena_len = arena_len;
-    b->d_arena = (uint8_t *)c->pool.alloc(arena_len + 64);
-    b->d_jobs = c->pool.alloc(std::max<size_t>(job_bytes, 64));
-    b->d_out = c->pool.alloc(std::max<size_t>(out_bytes, 64));
+    b->d_arena = (uint8_t *)batch_alloc(c, b, arena_len + 64);
+    b->d_jobs = batch_alloc(c, b, std::max<size_t>(job_bytes, 64));
+    b->d_out = batch_alloc(c, b, std::max<size_t>(out_bytes, 64));
     if (!b->d_arena || !b->d_jobs || !b->d_out) return BWG_ENOMEM;
     hipError_t e = hipSuccess;
     if (arena_len) e = hipMemcpyAsync(b->d_arena, arena, arena_len, hipMemcpyHostToDevice, st);
@@ -132,13 +194,20 @@ int stage_buffers(bwg_ctx *c, bwg_batch *b, const uint8_t *arena, size_t arena_l
 
 }  // namespace
 
-int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
-              const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
-              bwg_batch **out) {
+void *batch_alloc(bwg_ctx *c, bwg_batch *b, size_t bytes) {
+    return b->bump ? b->bump->take(bytes) : c->pool.alloc(bytes);
+}
+
+namespace {
+
+int stage_ext_impl(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
+                   const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                   const int32_t *h0, const uint8_t *ref, Bump *bump, const hipEvent_t *ev, hipStream_t st,
+                   bwg_batch **out) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    bwg_batch *b = new_batch(0, n);
+    bwg_batch *b = new_batch(c, 0, n, ev);
     if (!b) return BWG_ENOMEM;
+    b->bump = bump;
     b->esc = sc;
     b->d_ref = ref;
     b->hext.resize(n);
@@ -159,6 +228,22 @@ int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, 
     }
     *out = b;
     return BWG_OK;
+}
+
+}  // namespace
+
+int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
+              const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
+              const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
+              bwg_batch **out) {
+    return stage_ext_impl(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, ref, nullptr, nullptr, st,
+                          out);
+}
+
+int stage_ext_bump(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
+                   const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                   const int32_t *h0, Bump *bump, const hipEvent_t ev[2], hipStream_t st, bwg_batch **out) {
+    return stage_ext_impl(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, bump, ev, st, out);
 }
 
 int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint8_t *arena, size_t arena_len,
@@ -188,7 +273,7 @@ int stage_matesw(bwg_ctx *c, const KswScoring &sc, const MateRescueParams &mp, s
                  size_t arena_len, hipStream_t st, bwg_batch **out) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     const size_t nj = 4 * n;
-    bwg_batch *b = new_batch(1, nj);
+    bwg_batch *b = new_batch(c, 1, nj);
     if (!b) return BWG_ENOMEM;
     b->ksc = sc;
     b->matesw = true;
@@ -257,7 +342,7 @@ int stage_global(bwg_ctx *c, const GlobalScoring &sc, size_t n, const uint8_t *a
                  const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                  const int32_t *w, hipStream_t st, bwg_batch **out) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    bwg_batch *b = new_batch(2, n);
+    bwg_batch *b = new_batch(c, 2, n);
     if (!b) return BWG_ENOMEM;
     b->gsc = sc;
     b->hglb.resize(n);
@@ -282,23 +367,22 @@ int stage_global(bwg_ctx *c, const GlobalScoring &sc, size_t n, const uint8_t *a
     return BWG_OK;
 }
 
-int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops) {
+int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops, hipStream_t st) {
     if (b->n == 0) return BWG_OK;
     if (!b->d_nops) return BWG_EINVAL;  // not run yet
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    hipError_t e = hipMemcpyAsync(n_ops, b->d_nops, b->n * 4, hipMemcpyDeviceToHost, c->ss.main);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    hipError_t e = hipMemcpyAsync(n_ops, b->d_nops, b->n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     if (e != hipSuccess) return bwg_map_hip(e);
     for (size_t k = 0; k < b->n; ++k)
         if (n_ops[k] == 0xffffffffu) return BWG_EINVAL;  // no alignment path inside the band
     return BWG_OK;
 }
 
-int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap) {
+int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap, hipStream_t st) {
     if (b->n == 0) return BWG_OK;
     std::vector<uint32_t> cnt(b->n);
-    int rc = fetch_cigar_lengths(c, b, cnt.data());
+    int rc = fetch_cigar_lengths(c, b, cnt.data(), st);
     if (rc) return rc;
     std::vector<uint64_t> off(b->n);
     uint64_t total = 0;
@@ -308,38 +392,33 @@ int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap) {
     }
     if (total > cap) return BWG_E2BIG;
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
     uint64_t *d_off = (uint64_t *)c->pool.alloc(b->n * 8);
     uint32_t *d_dst = (uint32_t *)c->pool.alloc(std::max<uint64_t>(1, total) * 4);
     hipError_t e = (d_off && d_dst) ? hipSuccess : hipErrorOutOfMemory;
-    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), b->n * 8, hipMemcpyHostToDevice, c->ss.main);
+    if (e == hipSuccess) e = hipMemcpyAsync(d_off, off.data(), b->n * 8, hipMemcpyHostToDevice, st);
     if (e == hipSuccess)
-        e = launch_glb_cigar_gather(c->ss.main, (const GlobalJob *)glb_plan_jobs(b), (uint32_t)b->n, b->d_ops,
-                                    b->d_nops, d_off, d_dst);
-    if (e == hipSuccess && total)
-        e = hipMemcpyAsync(ops, d_dst, total * 4, hipMemcpyDeviceToHost, c->ss.main);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+        e = launch_glb_cigar_gather(st, (const GlobalJob *)glb_plan_jobs(b), (uint32_t)b->n, b->d_ops, b->d_nops,
+                                    d_off, d_dst);
+    if (e == hipSuccess && total) e = hipMemcpyAsync(ops, d_dst, total * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     c->pool.free(d_off);
     c->pool.free(d_dst);
     return bwg_map_hip(e);
 }
 
-int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win) {
+int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win, hipStream_t st) {
     if (b->n == 0) return BWG_OK;
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    hipError_t e = hipMemcpyAsync(win, b->d_win, b->n * sizeof(bwg_rescue_window), hipMemcpyDeviceToHost,
-                                  c->ss.main);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    hipError_t e = hipMemcpyAsync(win, b->d_win, b->n * sizeof(bwg_rescue_window), hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return bwg_map_hip(e);
 }
 
-int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out) {
+int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out, hipStream_t st) {
     if (b->n == 0) return BWG_OK;
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    std::lock_guard<std::mutex> g(c->run_mu);
-    hipError_t e = hipMemcpyAsync(w_out, b->d_wout, b->n * 4, hipMemcpyDeviceToHost, c->ss.main);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->ss.main);
+    hipError_t e = hipMemcpyAsync(w_out, b->d_wout, b->n * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
     return bwg_map_hip(e);
 }
 
@@ -347,7 +426,7 @@ int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, 
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
-    bwg_batch *b = new_batch(1, n);
+    bwg_batch *b = new_batch(c, 1, n);
     if (!b) return BWG_ENOMEM;
     b->ksc = sc;
     b->d_ref = ref;
@@ -371,19 +450,15 @@ int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, 
     return BWG_OK;
 }
 
-size_t batch_device_bytes(int kind, size_t n, size_t arena_len, const uint32_t *q_len,
-                          const uint32_t *t_len, int32_t max_sc) {
-    size_t bytes = DevicePool::round(arena_len + 64) + 2 * DevicePool::round(std::max<size_t>(64, n * 32));
-    bytes += 2 * DevicePool::round(std::max<size_t>(64, n * 4)) + 4096;  // order array, queue heads
-    if (kind == 1) {
-        // row maxima: tiles of 64 jobs x longest target of the tile, bounded by n * max target
-        uint32_t tmax = 0;
-        for (size_t k = 0; k < n; ++k) tmax = std::max(tmax, t_len[k]);
-        bytes += DevicePool::round(std::max<size_t>(64, ((n + 63) / 64) * 64 * (size_t)tmax * 4));
-    }
-    (void)q_len;
-    (void)max_sc;
-    return bytes;
+size_t batch_device_bytes(bwg_ctx *c, int kind, size_t n, size_t arena_len, const uint32_t *q_len,
+                          const uint32_t *t_len, const int32_t *h0, const ExtScoring *esc,
+                          const KswScoring *ksc) {
+    // stage_buffers: arena, descriptors, results
+    const size_t job_bytes = n * (kind == 0 ? sizeof(ExtJob) : sizeof(KswJob));
+    const size_t out_bytes = n * (kind == 0 ? sizeof(bwg_ext_result) : sizeof(bwg_ksw_result));
+    size_t bytes = DevicePool::worst(arena_len + 64) + DevicePool::worst(std::max<size_t>(job_bytes, 64)) +
+                   DevicePool::worst(std::max<size_t>(out_bytes, 64));
+    return bytes + plan_device_bytes(c, kind, n, q_len, t_len, h0, esc, ksc);
 }
 
 // ---------------------------------------------------------------------------------------------
@@ -431,11 +506,20 @@ int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
 
 }  // namespace
 
+// Band-retry batches keep their rounds on the device, rescue-window batches compute their windows
+// there, and global alignment has only the batched kernels: those run the batched form always.
+static bool runs_plain(const bwg_batch *b, int form) {
+    return form == BWG_FORM_PLAIN && !b->side_tries && !b->matesw && b->kind != 2;
+}
+
+int prepare_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
+    if (b->n == 0) return BWG_OK;
+    return runs_plain(b, form) ? prepare_plain(c, b, ss.main) : plan_batched(c, b, ss.main);
+}
+
 int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
     if (b->n == 0) return BWG_OK;
-    // band-retry batches keep their rounds on the device, rescue-window batches compute their
-    // windows there, and global alignment has only the batched kernels
-    if (form == BWG_FORM_PLAIN && !b->side_tries && !b->matesw && b->kind != 2) {
+    if (runs_plain(b, form)) {
         int rc = prepare_plain(c, b, ss.main);
         if (rc) return rc;
         hipError_t e = hipEventRecord(b->ev_start, ss.main);
@@ -459,20 +543,26 @@ int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
     return rc;
 }
 
+void record_run(bwg_ctx *c, bwg_batch *b) {
+    float ms = 0.f;
+    (void)hipEventElapsedTime(&ms, b->ev_start, b->ev_end);
+    c->last_kernel_ns = (int64_t)(ms * 1e6);
+    c->last_main_ns = (int64_t)(ms * 1e6);
+    c->last_main_launches = b->last_launches;
+    c->launches += b->last_launches;
+    c->jobs += (int64_t)b->n;
+}
+
 int run_batch(bwg_ctx *c, bwg_batch *b) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     std::lock_guard<std::mutex> g(c->run_mu);
     if (b->n == 0) return BWG_OK;
-    int rc = enqueue_batch(c, b, c->form, c->ss);
+    b->tu = c->tuning_snapshot();
+    int rc = enqueue_batch(c, b, b->tu.form, c->ss);
     if (rc) return rc;
     hipError_t e = hipStreamSynchronize(c->ss.main);
     if (e != hipSuccess) return bwg_map_hip(e);
-    float ms = 0.f;
-    (void)hipEventElapsedTime(&ms, b->ev_start, b->ev_end);
-    c->last_kernel_ns = c->last_main_ns = (int64_t)(ms * 1e6);
-    c->last_main_launches = b->last_launches;
-    c->launches += b->last_launches;
-    c->jobs += (int64_t)b->n;
+    record_run(c, b);
     return BWG_OK;
 }
 
@@ -512,8 +602,10 @@ void destroy_batch(bwg_ctx *c, bwg_batch *b) {
     c->pool.free(b->d_nops);
     if (b->ref_owner) b->ref_owner->ref_users--;
     release_batched(c, b);
-    if (b->ev_start) (void)hipEventDestroy(b->ev_start);
-    if (b->ev_end) (void)hipEventDestroy(b->ev_end);
+    if (!b->borrowed_events) {
+        if (b->ev_start) (void)hipEventDestroy(b->ev_start);
+        if (b->ev_end) (void)hipEventDestroy(b->ev_end);
+    }
     delete b;
 }
 

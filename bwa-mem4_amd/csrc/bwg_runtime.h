@@ -2,9 +2,12 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <atomic>
+#include <condition_variable>
 #include <cstdint>
 #include <map>
+#include <memory>
 #include <mutex>
 #include <vector>
 
@@ -14,26 +17,78 @@
 
 namespace bwg {
 
-constexpr uint32_t kMaxQlen = 4096;
-constexpr uint32_t kMaxTlen = 65535;
+// Job sizes.  Every job up to kMaxJobLen bases of query and of target runs on the device: the
+// binned kernels take the shapes they are tuned for (extension queries below 8192 columns with
+// h0 + qlen*max_sc < 2^18; rescue queries padded to at most 256 columns and targets up to
+// kMaxFastTlen rows), and the plain kernels, whose per-job state lives in global memory, take
+// everything else.  The limit keeps every score, row index and gap product of the int32 recurrences
+// away from overflow (checked per job on the host, see bwg_api.cpp).
+constexpr uint32_t kMaxJobLen = 1u << 24;
+constexpr uint32_t kMaxFastTlen = 65535;
 constexpr int kAuxStreams = 3;
 
+// Device scratch of the allocation-free entry point: one array per GPU for the life of the process
+// (noalloc.hip).  A call that needs more than this is rejected with BWG_E2BIG.
+constexpr size_t kNoAllocScratchBytes = 64ull << 20;
+constexpr size_t kBumpAlign = 256;
+inline size_t bump_round(size_t b) { return (b + kBumpAlign - 1) & ~(kBumpAlign - 1); }
+
+// Bump allocation over a fixed device region (the no-alloc scratch): nothing is ever freed during
+// a call; the region is reused by the next call.
+struct Bump {
+    uint8_t *base = nullptr;
+    size_t cap = 0, used = 0;
+    void *take(size_t bytes) {
+        const size_t b = bump_round(std::max<size_t>(bytes, 1));
+        if (b > cap - used) return nullptr;
+        void *p = base + used;
+        used += b;
+        return p;
+    }
+};
+
+// Size limits of one job (BWG_E2BIG beyond them): both lengths up to kMaxJobLen, and the largest
+// score and gap product the int32 recurrences can form kept below 2^30.
+inline bool ext_job_fits(const ExtScoring &sc, uint32_t qlen, uint32_t tlen, int32_t h0) {
+    if (qlen > kMaxJobLen || tlen > kMaxJobLen) return false;
+    const int64_t top = (int64_t)h0 + (int64_t)qlen * std::max(sc.max_sc, 0) + std::max(sc.end_bonus, 0);
+    const int64_t gap = (int64_t)std::max(sc.e_del, sc.e_ins) * ((int64_t)std::max(qlen, tlen) + 1) +
+                        std::max(std::max(sc.o_del, sc.o_ins), 0);
+    return top < (1 << 30) && gap < (1 << 30);
+}
+inline bool ksw_job_fits(const KswScoring &sc, uint32_t qlen, uint32_t tlen) {
+    if (qlen > kMaxJobLen || tlen > kMaxJobLen) return false;
+    const int64_t top = ((int64_t)qlen + 16) * std::max<int32_t>(mat_max(sc.mat), 0);
+    const int64_t gap = (int64_t)std::max(sc.e_del, sc.e_ins) * ((int64_t)std::max(qlen, tlen) + 16) +
+                        std::max(std::max(sc.o_del, sc.o_ins), 0);
+    return top < (1 << 30) && gap < (1 << 30);
+}
+
 // Caching device allocator: freed blocks are kept per rounded size and reused, so a steady
-// stream of batches does not call hipMalloc/hipFree on the hot path.
+// stream of batches does not call hipMalloc/hipFree on the hot path.  A cached block is reused for
+// a request whose rounded size it exceeds by at most a quarter (kReuseSlack), and the cache is
+// trimmed, largest blocks first, whenever it would hold more than its cap.
 class DevicePool {
   public:
     void *alloc(size_t bytes);
     void free(void *p);
     void release_all();
-    size_t bytes_in_use() const { return in_use_; }
-    size_t bytes_cached() const { return cached_; }
+    size_t bytes_in_use() const { return in_use_.load(); }
+    size_t bytes_cached() const { return cached_.load(); }
+    size_t peak_in_use() const { return peak_.load(); }
+    void reset_peak() { peak_.store(in_use_.load()); }
+    void set_cache_cap(size_t cap);
+    size_t cache_cap() const { return cap_.load(); }
     static size_t round(size_t b);
+    // Device bytes one alloc(bytes) can hold, including the reuse slack: the admission estimate.
+    static size_t worst(size_t b) { return round(b) + (round(b) >> 2); }
 
   private:
+    void trim_locked(size_t cap);
     std::mutex mu_;
     std::multimap<size_t, void *> free_;
     std::map<void *, size_t> live_;
-    size_t in_use_ = 0, cached_ = 0;
+    std::atomic<size_t> in_use_{0}, cached_{0}, peak_{0}, cap_{size_t(8) << 30};
 };
 
 // A main stream plus helper streams used to run independent kernels of one batch concurrently
@@ -61,21 +116,52 @@ struct Tuning {
     bool ungapped = true;          // extension: the reference's ungapped-diagonal fast path first
     bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
     bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph
+    bool tune_new = true;          // a batch may run the autotuner (false: it only reads the table)
+    int form = BWG_FORM_BATCHED;   // BWG_OPT_FORM
 };
+
+// An execution lane of the synchronous entry points: its own stream set, so that calls from
+// several host threads stage, run and copy back concurrently instead of queueing behind one
+// another, plus pinned result staging.
+struct ExecLane {
+    StreamSet ss;
+    void *pin_out = nullptr;
+    size_t pin_out_cap = 0;
+    int create();
+    void destroy();
+};
+
+// Host time of the phases of the last synchronous call (bwg_extend_batch, bwg_ksw_align2_batch).
+enum Phase { kPhValidate, kPhStage, kPhPlan, kPhKernels, kPhFetch, kPhTotal, kNumPhases };
 
 }  // namespace bwg
 
 struct bwg_ctx {
     int device = 0;
     int num_cus = 256;
-    bwg::StreamSet ss;           // the context's own streams (synchronous entry points)
-    std::mutex run_mu;           // one synchronous batch on the context streams at a time
-    int form = BWG_FORM_BATCHED;
+    bwg::StreamSet ss;           // the context's own streams (staged batches)
+    std::mutex run_mu;           // one staged-batch operation on the context streams at a time
+    // options: written by bwg_set_option under opt_mu; every batch plans with a copy taken under
+    // the same lock (tuning_snapshot), so a concurrent option change never tears a plan
+    mutable std::mutex opt_mu;
     bwg::Tuning tuning;
+    bwg::Tuning tuning_snapshot() const {
+        std::lock_guard<std::mutex> g(opt_mu);
+        return tuning;
+    }
     bwg::Tuner tuner;
     std::atomic<int64_t> launches{0}, jobs{0};
-    int64_t last_kernel_ns = 0, last_main_ns = 0, last_main_launches = 0;
+    std::atomic<int64_t> last_kernel_ns{0}, last_main_ns{0}, last_main_launches{0};
+    std::atomic<int64_t> phase_ns[bwg::kNumPhases] = {};
     bwg::DevicePool pool;
+    // execution lanes of the synchronous entry points (created on demand, at most max_lanes)
+    std::mutex lane_mu;
+    std::condition_variable lane_cv;
+    std::vector<std::unique_ptr<bwg::ExecLane>> lanes;
+    std::vector<bwg::ExecLane *> free_lanes;
+    int max_lanes = 4;
+    // timing events of the allocation-free entry point, created with the context
+    hipEvent_t na_ev[2] = {nullptr, nullptr};
     // two-strand reference for mate rescue (bwg_set_reference): 2*l_pac bytes and the contigs
     uint8_t *d_ref2 = nullptr;
     int64_t l_pac = 0;
@@ -131,6 +217,12 @@ struct bwg_batch {
     // timing of the last run
     hipEvent_t ev_start = nullptr, ev_end = nullptr;
     int64_t last_launches = 0;
+    // the options the batch was planned with (a snapshot taken when it was staged)
+    bwg::Tuning tu;
+    // allocation-free batches: device buffers come from this region instead of the pool, and the
+    // timing events belong to the context
+    bwg::Bump *bump = nullptr;
+    bool borrowed_events = false;
 };
 
 namespace bwg {
@@ -147,36 +239,69 @@ int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, 
 int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint8_t *arena, size_t arena_len,
                const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out);
-int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out);
+int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out, hipStream_t st);
 // A mate-rescue round: n anchors, 4n jobs whose windows the device computes against the context's
 // two-strand reference.
 int stage_matesw(bwg_ctx *c, const KswScoring &sc, const MateRescueParams &mp, size_t n,
                  const bwg_mate_anchor *anchors, const int64_t *ma_rb, size_t n_ma, const uint8_t *arena,
                  size_t arena_len, hipStream_t st, bwg_batch **out);
-int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win);
+int fetch_windows(bwg_ctx *c, bwg_batch *b, bwg_rescue_window *win, hipStream_t st);
 // Banded global alignment with traceback: n jobs with their own bands.
 int stage_global(bwg_ctx *c, const GlobalScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
                  const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                  const int32_t *w, hipStream_t st, bwg_batch **out);
-int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops);
-int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap);
+// The fetch_* calls below enqueue their copies (and the CIGAR gather) on `st` and wait for them.
+int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops, hipStream_t st);
+int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap, hipStream_t st);
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out);
+// Host-side preparation of a batch for a form: the batched form's plan (binning, autotuning, plan
+// uploads), or the plain form's per-job scratch.  Done once; enqueue_batch does it if needed.
+int prepare_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss);
 // Enqueue the batch's kernels on the stream set (no host synchronisation).  b->ev_start/ev_end
 // bracket them on ss.main.
 int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss);
 // Synchronous run on the context streams: enqueue, wait, update the context statistics.
 int run_batch(bwg_ctx *c, bwg_batch *b);
+// After a batch enqueued on some stream set has completed: record its kernel time and launches in
+// the context statistics.
+void record_run(bwg_ctx *c, bwg_batch *b);
 int enqueue_results(bwg_ctx *c, bwg_batch *b, void *out, hipStream_t st);
 int fetch_results(bwg_ctx *c, bwg_batch *b, void *out);
 void destroy_batch(bwg_ctx *c, bwg_batch *b);
-// Device bytes a batch of this description holds once staged and planned (admission control).
-size_t batch_device_bytes(int kind, size_t n, size_t arena_len, const uint32_t *q_len,
-                          const uint32_t *t_len, int32_t max_sc);
+// Execution lanes of the synchronous entry points: take one (creating it, or waiting for one to
+// free up), give it back.
+ExecLane *acquire_lane(bwg_ctx *c);
+void release_lane(bwg_ctx *c, ExecLane *l);
+// Device memory of a batch: from its bump region when it has one, else from the context's pool.
+void *batch_alloc(bwg_ctx *c, bwg_batch *b, size_t bytes);
+// The allocation-free entry point's scratch (noalloc.hip): its device address is looked up once
+// per device (noalloc_setup, with that device current); calls on one device are serialised by
+// noalloc_call_mutex, since they share the array.
+int noalloc_setup(int device);
+uint8_t *noalloc_scratch(int device);
+std::mutex &noalloc_call_mutex(int device);
+// Stage an extension batch whose buffers come from `bump` and whose events are `ev`.
+int stage_ext_bump(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
+                   const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
+                   const int32_t *h0, Bump *bump, const hipEvent_t ev[2], hipStream_t st, bwg_batch **out);
+// Upper bound of the device bytes a batch of this description holds once staged and planned,
+// tuning included (admission control of the server).  Mirrors the allocations of stage_ext /
+// stage_ksw and of the planner (bwg_batched.cpp).
+size_t batch_device_bytes(bwg_ctx *c, int kind, size_t n, size_t arena_len, const uint32_t *q_len,
+                          const uint32_t *t_len, const int32_t *h0, const ExtScoring *esc,
+                          const KswScoring *ksc);
 
 // batched form (bwg_batched.cpp)
 int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches);
+int plan_batched(bwg_ctx *c, bwg_batch *b, hipStream_t st);
+// The planner's share of batch_device_bytes: order arrays, queues, plain-bin scratch, rescue row
+// maxima, and the autotuner's sample buffers when the batch would tune a class.
+// With `exact` set (allocation-free extension batches, which never tune): the bytes the plan takes
+// from a bump region.
+size_t plan_device_bytes(bwg_ctx *c, int kind, size_t n, const uint32_t *q_len, const uint32_t *t_len,
+                         const int32_t *h0, const ExtScoring *esc, const KswScoring *ksc, bool exact = false);
 // The planned descriptors of a global-alignment batch (device), null before its first run.
 const void *glb_plan_jobs(const bwg_batch *b);
 void release_batched(bwg_ctx *c, bwg_batch *b);

@@ -34,6 +34,7 @@ struct Chunk {
     Clock::time_point t_submit;
     size_t bytes = 0;
     int status = BWG_EAGAIN;  // BWG_EAGAIN while pending
+    bool waited = false;      // a bwg_server_wait owns the ticket and releases it
 };
 
 struct Slot {
@@ -95,9 +96,13 @@ struct bwg_server {
 
 namespace {
 
-size_t chunk_bytes(const Chunk *k) {
-    return bwg::batch_device_bytes(k->c.kind == BWG_CHUNK_EXTEND ? 0 : 1, k->c.n, k->c.arena_len, k->c.q_len,
-                                   k->c.t_len, 1);
+size_t chunk_bytes(bwg_ctx *ctx, const Chunk *k) {
+    if (k->c.kind == BWG_CHUNK_EXTEND) {
+        const bwg::ExtScoring sc = bwg::ext_scoring(k->ext);
+        return bwg::batch_device_bytes(ctx, 0, k->c.n, k->c.arena_len, k->c.q_len, k->c.t_len, k->c.h0, &sc, nullptr);
+    }
+    const bwg::KswScoring sc = bwg::ksw_scoring(k->ksw);
+    return bwg::batch_device_bytes(ctx, 1, k->c.n, k->c.arena_len, k->c.q_len, k->c.t_len, nullptr, nullptr, &sc);
 }
 
 int validate_chunk(const bwg_server *s, const bwg_chunk *c) {
@@ -109,9 +114,14 @@ int validate_chunk(const bwg_server *s, const bwg_chunk *c) {
     } else {
         return BWG_EINVAL;
     }
+    const bwg::ExtScoring esc = c->kind == BWG_CHUNK_EXTEND ? bwg::ext_scoring(*c->ext) : bwg::ExtScoring{};
+    const bwg::KswScoring ksc = c->kind == BWG_CHUNK_RESCUE ? bwg::ksw_scoring(*c->ksw) : bwg::KswScoring{};
     for (size_t k = 0; k < c->n; ++k) {
         if (c->q_off[k] > c->arena_len || c->q_len[k] > c->arena_len - c->q_off[k]) return BWG_EINVAL;
-        if (c->q_len[k] > bwg::kMaxQlen || c->t_len[k] > bwg::kMaxTlen) return BWG_E2BIG;
+        if (c->kind == BWG_CHUNK_EXTEND && c->h0[k] <= 0) return BWG_EINVAL;
+        if (c->kind == BWG_CHUNK_EXTEND ? !bwg::ext_job_fits(esc, c->q_len[k], c->t_len[k], c->h0[k])
+                                        : !bwg::ksw_job_fits(ksc, c->q_len[k], c->t_len[k]))
+            return BWG_E2BIG;
         for (uint32_t i = 0; i < c->q_len[k]; ++i)
             if (c->arena[c->q_off[k] + i] > 4) return BWG_EINVAL;
         const uint64_t to = c->t_off[k] & bwg::kTOffMask;
@@ -123,7 +133,7 @@ int validate_chunk(const bwg_server *s, const bwg_chunk *c) {
             for (uint32_t i = 0; i < c->t_len[k]; ++i)
                 if (c->arena[to + i] > 4) return BWG_EINVAL;
         }
-        if (c->kind == BWG_CHUNK_EXTEND && (c->h0[k] <= 0 || c->q_len[k] < 1)) return BWG_EINVAL;
+        if (c->kind == BWG_CHUNK_EXTEND && c->q_len[k] < 1) return BWG_EINVAL;
     }
     return BWG_OK;
 }
@@ -297,6 +307,7 @@ int bwg_server_create(bwg_ctx *ctx, uint64_t hbm_budget_bytes, int n_slots, bwg_
             return BWG_EIO;
         }
     }
+    ctx->pool.reset_peak();
     for (int k = 0; k < n_slots; ++k) s->workers.emplace_back([s, k] { s->slot_loop(k); });
     *out = s;
     return BWG_OK;
@@ -331,7 +342,7 @@ int bwg_server_submit(bwg_server *s, const bwg_chunk *chunk, uint64_t *ticket) {
     k->c = *chunk;
     if (chunk->kind == BWG_CHUNK_EXTEND) k->ext = *chunk->ext;
     else k->ksw = *chunk->ksw;
-    k->bytes = chunk_bytes(k);
+    k->bytes = chunk_bytes(s->ctx, k);
     if (k->bytes > s->budget) {
         delete k;
         return BWG_E2BIG;
@@ -358,7 +369,8 @@ int bwg_server_poll(bwg_server *s, uint64_t ticket) {
     auto it = s->chunks.find(ticket);
     if (it == s->chunks.end()) return BWG_EINVAL;
     const int st = it->second->status;
-    if (st != BWG_EAGAIN) {
+    // a waiting thread holds the chunk: it alone releases the ticket
+    if (st != BWG_EAGAIN && !it->second->waited) {
         delete it->second;
         s->chunks.erase(it);
     }
@@ -371,6 +383,8 @@ int bwg_server_wait(bwg_server *s, uint64_t ticket) {
     auto it = s->chunks.find(ticket);
     if (it == s->chunks.end()) return BWG_EINVAL;
     Chunk *k = it->second;
+    if (k->waited) return BWG_EINVAL;  // one waiter per ticket
+    k->waited = true;
     s->cv_complete.wait(g, [&] { return k->status != BWG_EAGAIN; });
     const int st = k->status;
     delete k;
@@ -386,6 +400,7 @@ int bwg_server_stats(bwg_server *s, bwg_server_counters *out, int reset) {
     out->jobs = s->n_jobs;
     out->epochs = s->n_epochs;
     out->peak_device_bytes = s->peak_bytes;
+    out->peak_pool_bytes = s->ctx->pool.peak_in_use();
     out->seconds = std::chrono::duration<double>(Clock::now() - s->stats_t0).count();
     if (!s->lat_ms.empty()) {
         std::vector<double> v = s->lat_ms;
@@ -402,6 +417,7 @@ int bwg_server_stats(bwg_server *s, bwg_server_counters *out, int reset) {
         s->lat_ms.clear();
         s->n_chunks = s->n_jobs = s->n_epochs = 0;
         s->peak_bytes = s->in_flight_bytes;
+        s->ctx->pool.reset_peak();
         s->stats_t0 = Clock::now();
     }
     return BWG_OK;

@@ -83,7 +83,7 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
                     te = f.te;  // forward coordinates, kept for the start computation
                     qe = f.qe;
                 } else {
-                    rm = rowmax + jb.aux0;
+                    rm = rowmax + row_slot(jb);
                     if (!run && g == 0) {
                         bwg_ksw_result r;
                         r.score = 0;
@@ -290,7 +290,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
                     te = f.te;  // forward coordinates, kept for the start computation
                     qe = f.qe;
                 } else {
-                    rm = rowmax + jb.aux0;
+                    rm = rowmax + row_slot(jb);
                     if (!run && g == 0) {
                         bwg_ksw_result r;
                         r.score = 0;
@@ -428,7 +428,7 @@ __global__ __launch_bounds__(256) void ksw_tracker_kernel(const KswJob *__restri
     const int32_t gmax = r.score, te = r.te;
     int32_t s2 = -1, t2 = -1;
     if (gmax > 0 && jb.qlen > 0) {
-        const int32_t *rm = rowmax + jb.aux0;
+        const int32_t *rm = rowmax + row_slot(jb);
         const int32_t half = (gmax + sc.max_sc - 1) / sc.max_sc;
         const int32_t lo = te - half, hi = te + half;
         int32_t cs = 0, cr = -1;

@@ -1,7 +1,9 @@
 // Load driver for the serving loop: Poisson arrivals of alignment chunks from concurrent producers.
 //
 // A pool of chunks is generated up front (sizes log-uniform in [1k, 64k] jobs; 80% of chunks
-// long-tail extension work, 20% mate-rescue work; targets given either as arena bytes or as
+// long-tail extension work, 20% mate-rescue work: the kind is drawn per chunk, independently of its
+// size, so the expected share of jobs is 80/20 too, and the submitted job counts per kind are
+// reported; targets given either as arena bytes or as
 // windows of a genome resident in the server).  Each producer thread submits chunks from the pool
 // at exponentially distributed intervals so the producers together offer `rate` jobs/s, and a
 // waiter thread per producer collects its chunks.  With verification on, every chunk's results in
@@ -57,8 +59,9 @@ struct Pending {
 
 extern "C" {
 
-// out[0..10]: p50 ms, p99 ms, mean ms, max ms, completed jobs/s, chunks, jobs, epochs,
-// mismatching chunks, failed submissions, peak device bytes in flight.
+// out[0..13]: p50 ms, p99 ms, mean ms, max ms, completed jobs/s, chunks, jobs, epochs,
+// mismatching chunks, failed submissions, peak admitted device bytes in flight, peak device bytes
+// the context's batches held, extension jobs submitted, rescue jobs submitted.
 int bwgw_serve_run(bwg_ctx *ctx, bwg_server *srv, double rate_jobs_s, int producers, double duration_s,
                    uint64_t seed, int use_ref, int pool_chunks, int verify, double *out) {
     const uint64_t genome_seed = seed;
@@ -133,6 +136,7 @@ int bwgw_serve_run(bwg_ctx *ctx, bwg_server *srv, double rate_jobs_s, int produc
     bwg_server_stats(srv, &st, 1);
     // ---- producers and waiters ----
     std::atomic<int> mismatches{0}, failures{0};
+    std::atomic<uint64_t> ext_jobs{0}, resc_jobs{0};
     std::atomic<bool> stop_submit{false};
     const auto t0 = Clock::now();
     const auto t_end = t0 + std::chrono::duration_cast<Clock::duration>(std::chrono::duration<double>(duration_s));
@@ -190,6 +194,7 @@ int bwgw_serve_run(bwg_ctx *ctx, bwg_server *srv, double rate_jobs_s, int produc
                     delete x;
                     continue;
                 }
+                (pc.kind == BWG_CHUNK_EXTEND ? ext_jobs : resc_jobs) += pc.n;
                 {
                     std::lock_guard<std::mutex> g(mu);
                     q.push_back(x);
@@ -218,6 +223,9 @@ int bwgw_serve_run(bwg_ctx *ctx, bwg_server *srv, double rate_jobs_s, int produc
     out[8] = (double)mismatches.load();
     out[9] = (double)failures.load();
     out[10] = (double)st.peak_device_bytes;
+    out[11] = (double)st.peak_pool_bytes;
+    out[12] = (double)ext_jobs.load();
+    out[13] = (double)resc_jobs.load();
     (void)stop_submit;
     return BWG_OK;
 }

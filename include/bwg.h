@@ -30,7 +30,7 @@ extern "C" {
 enum {
     BWG_OK = 0,
     BWG_EIO = -5,      /* a HIP runtime call or kernel failed */
-    BWG_E2BIG = -7,    /* a job exceeds a supported size (see bwg_limits) */
+    BWG_E2BIG = -7,    /* a job exceeds the size limits stated below, or an output buffer is too small */
     BWG_EAGAIN = -11,  /* serving: the HBM budget cannot admit the chunk right now */
     BWG_ENOMEM = -12,  /* device or host allocation failed */
     BWG_ENODEV = -19,  /* no usable gfx950 device */
@@ -69,8 +69,20 @@ const char *bwg_strerror(int code);
 
 /* Validation applied to every job (BWG_EINVAL on failure), the preconditions of ksw_extend2 and
  * ksw_align2: m == 5; e_del, e_ins >= 1; base codes <= 4; extension h0 > 0 and qlen >= 1;
- * rescue max_sc >= 1.  Size limits (BWG_E2BIG): extension qlen <= 4096 and tlen <= 65535; rescue
- * qlen <= 4096 and tlen <= 65535. */
+ * rescue max_sc >= 1.  Size limits (BWG_E2BIG): query and target up to 2^24 bases each, with the
+ * largest score (extension h0 + qlen*max_sc + end_bonus, rescue (qlen+16)*max(mat)) and the largest
+ * gap cost (max(e_del,e_ins) * (max(qlen,tlen)+16) + max(o_del,o_ins)) below 2^30, so that no
+ * int32 quantity of the recurrences can overflow.  Jobs beyond the binned kernels' shapes run in
+ * the plain kernels (per-job state in global memory), with identical results.
+ *
+ * Threading.  The synchronous entry points (bwg_extend_batch, bwg_extend, bwg_ksw_align2_batch,
+ * bwg_extend_side_batch, bwg_global_batch, bwg_matesw_batch) may be called concurrently from any
+ * number of host threads on one context: each call runs on one of the context's execution lanes
+ * (BWG_OPT_LANES, default 4; each its own set of HIP streams), so up to that many calls stage,
+ * run and copy back at the same time and the rest wait for a lane.  The staged-batch calls
+ * (bwg_batch_*) share the context's own streams and run one at a time.  bwg_set_option may be
+ * called at any time; a batch plans with the options in force when it was staged (staged batches)
+ * or called (synchronous entry points). */
 
 /* Banded seed extension (ksw_extend2 semantics) for n jobs sharing `p`.  Thread-safe; may be
  * called concurrently from many host threads. */
@@ -112,11 +124,26 @@ enum {
     BWG_OPT_EXT_SCORE_WIDTH = 8,  /* extension DP state bits: 16 where exact (default), or 32 always */
     BWG_OPT_UNGAPPED = 9,         /* batched extension: ungapped-diagonal fast path first (default 1) */
     BWG_OPT_KSW_SCORE_WIDTH = 10, /* rescue DP state bits: 16 where exact (default), or 32 always */
+    BWG_OPT_LANES = 11,           /* execution lanes of the synchronous entry points (1..16, default 4) */
+    BWG_OPT_POOL_CACHE_MB = 12,   /* device blocks the context keeps cached for reuse, MiB (default 8192) */
+    BWG_OPT_POOL_PEAK_RESET = 13, /* any value: restart BWG_STAT_POOL_PEAK at the bytes now in use */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
     BWG_STAT_LAST_MAIN_NS = 103,   /* device time of the last batch's dominant DP kernel */
-    BWG_STAT_LAST_MAIN_LAUNCHES = 104 /* launches of that dominant kernel in the last batch */
+    BWG_STAT_LAST_MAIN_LAUNCHES = 104, /* launches of that dominant kernel in the last batch */
+    BWG_STAT_POOL_IN_USE = 105,    /* device bytes the context's batches hold now */
+    BWG_STAT_POOL_CACHED = 106,    /* device bytes freed by batches and cached for reuse */
+    BWG_STAT_POOL_PEAK = 107,      /* largest BWG_STAT_POOL_IN_USE since creation or the last reset */
+    /* host wall time of the phases of the last synchronous extension or rescue call, ns: host-side
+     * validation; descriptor build and host-to-device copies; planning (binning, autotuning, plan
+     * uploads); device kernels (HIP events); result copy to the host; the whole call */
+    BWG_STAT_PHASE_VALIDATE_NS = 110,
+    BWG_STAT_PHASE_STAGE_NS = 111,
+    BWG_STAT_PHASE_PLAN_NS = 112,
+    BWG_STAT_PHASE_KERNELS_NS = 113,
+    BWG_STAT_PHASE_FETCH_NS = 114,
+    BWG_STAT_PHASE_TOTAL_NS = 115
 };
 int bwg_set_option(bwg_ctx *ctx, int key, int64_t value);
 int64_t bwg_get_stat(const bwg_ctx *ctx, int key);
@@ -278,20 +305,26 @@ typedef struct {
     void *out;                   /* n bwg_ext_result or bwg_ksw_result */
 } bwg_chunk;
 typedef struct {
-    uint64_t chunks, jobs, epochs, peak_device_bytes;
+    uint64_t chunks, jobs, epochs;
+    uint64_t peak_device_bytes;  /* largest admitted estimate of the epochs in flight (<= budget) */
+    uint64_t peak_pool_bytes;    /* largest device bytes the context's batches actually held */
     double seconds, p50_ms, p99_ms, mean_ms, max_ms;  /* chunk latency: submit to results available */
 } bwg_server_counters;
 typedef struct bwg_server bwg_server;
-/* hbm_budget_bytes bounds the device memory of the epochs in flight; n_slots (1..8) epochs can be
- * on the device at once. */
+/* hbm_budget_bytes bounds the device memory of the epochs in flight: a chunk is admitted when an
+ * upper bound of what its batch allocates (staging, plan, plain-kernel state, row maxima, autotuner
+ * samples) fits beside the epochs already running; n_slots (1..8) epochs can be on the device at
+ * once. */
 int bwg_server_create(bwg_ctx *ctx, uint64_t hbm_budget_bytes, int n_slots, bwg_server **out);
 /* Upload the reference (codes 0..4) that BWG_TARGET_FROM_REF targets index; only while idle. */
 int bwg_server_set_reference(bwg_server *s, const uint8_t *ref, uint64_t len);
 /* Validate and queue a chunk (any thread).  BWG_E2BIG if it alone exceeds the budget. */
 int bwg_server_submit(bwg_server *s, const bwg_chunk *chunk, uint64_t *ticket);
-/* BWG_EAGAIN while the chunk is pending, else its final status (and the ticket is released). */
+/* BWG_EAGAIN while the chunk is pending, else its final status; the ticket is then released,
+ * unless a bwg_server_wait is in progress on it (that wait releases it). */
 int bwg_server_poll(bwg_server *s, uint64_t ticket);
-/* Block until the chunk completes; returns its status and releases the ticket. */
+/* Block until the chunk completes; returns its status and releases the ticket.  One wait per
+ * ticket: a second concurrent wait returns BWG_EINVAL. */
 int bwg_server_wait(bwg_server *s, uint64_t ticket);
 /* Counters and chunk-latency percentiles since creation or the last reset. */
 int bwg_server_stats(bwg_server *s, bwg_server_counters *out, int reset);
