@@ -17,7 +17,7 @@ BUILD = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 HIP_SOURCES = ["plain_kernels.hip", "ext_batched.hip", "ksw_batched.hip", "rescue_windows.hip", "global_batched.hip",
-               "bwg_batched.cpp", "bwg_runtime.cpp", "bwg_api.cpp", "bwg_server.cpp", "bwg_tuner.cpp"]
+               "noalloc.hip", "bwg_batched.cpp", "bwg_runtime.cpp", "bwg_api.cpp", "bwg_server.cpp", "bwg_tuner.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]

@@ -50,6 +50,7 @@ EXPORTED_SYMBOLS = (
     "bwg_batch_ungapped_hits", "bwg_extend_side_batch", "bwg_batch_stage_side", "bwg_batch_side_bands",
     "bwg_set_reference", "bwg_matesw_batch", "bwg_batch_stage_matesw", "bwg_batch_windows",
     "bwg_global_batch", "bwg_batch_stage_global", "bwg_batch_cigar_lengths", "bwg_batch_cigars",
+    "bwg_extend_batch_noalloc",
 )
 
 # bwg_mate_anchor, bwg_rescue_window, bwg_pe_orient and bwg_contig as numpy records
@@ -104,6 +105,8 @@ def _lib():
         L.bwg_strerror.argtypes = [C.c_int]
         L.bwg_extend_batch.restype = C.c_int
         L.bwg_extend_batch.argtypes = [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P]
+        L.bwg_extend_batch_noalloc.restype = C.c_int
+        L.bwg_extend_batch_noalloc.argtypes = [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P]
         L.bwg_extend.restype = C.c_int
         L.bwg_extend.argtypes = [_P, _P, _P, C.c_uint32, _P, C.c_uint32, C.c_int32, _P]
         L.bwg_ksw_align2_batch.restype = C.c_int

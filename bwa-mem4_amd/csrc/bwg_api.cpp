@@ -98,8 +98,10 @@ int bwg_create(int device, bwg_ctx **out) {
     if (!c) return BWG_ENOMEM;
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
+    // the allocation-free entry point's scratch address is looked up here, with the device current
     if (hipSetDevice(device) != hipSuccess || c->ss.create() != BWG_OK ||
-        hipEventCreate(&c->na_ev[0]) != hipSuccess || hipEventCreate(&c->na_ev[1]) != hipSuccess) {
+        hipEventCreate(&c->na_ev[0]) != hipSuccess || hipEventCreate(&c->na_ev[1]) != hipSuccess ||
+        noalloc_setup(device) != BWG_OK) {
         bwg_destroy(c);
         return BWG_EIO;
     }
@@ -552,6 +554,46 @@ int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_
             return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, b);
         },
         [&](bwg_batch *b, hipStream_t st) { return enqueue_results(c, b, out, st); });
+}
+
+int bwg_extend_batch_noalloc(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                             size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                             const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                             bwg_ext_result *out) {
+    if (!c || !p) return BWG_EINVAL;
+    if (n == 0) return BWG_OK;
+    if (!out) return BWG_EINVAL;
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    if (rc) return rc;
+    const ExtScoring sc = ext_scoring(*p);
+    uint8_t *base = noalloc_scratch(c->device);
+    if (!base) return BWG_EIO;
+    // everything the call places in the scratch: the staged arena, descriptors and results
+    // (stage_buffers) and the plan's arrays (plan_ext); it never tunes
+    const size_t need = bump_round(arena_len + 64) + bump_round(std::max<size_t>(n * sizeof(ExtJob), 64)) +
+                        bump_round(std::max<size_t>(n * sizeof(bwg_ext_result), 64)) +
+                        plan_device_bytes(c, 0, n, q_len, t_len, h0, &sc, nullptr, true);
+    if (need > kNoAllocScratchBytes) return BWG_E2BIG;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g_dev(noalloc_call_mutex(c->device));  // one call per device: shared array
+    std::lock_guard<std::mutex> g_ctx(c->run_mu);                       // the context's own streams
+    Bump bump;
+    bump.base = base;
+    bump.cap = kNoAllocScratchBytes;
+    bwg_batch *b = nullptr;
+    rc = stage_ext_bump(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, &bump, c->na_ev, c->ss.main, &b);
+    if (rc) return rc;
+    // the batched form, planned from the tuning table as it stands, launched eagerly
+    b->tu.tune_new = false;
+    b->tu.use_graphs = false;
+    b->tu.form = BWG_FORM_BATCHED;
+    rc = enqueue_batch(c, b, BWG_FORM_BATCHED, c->ss);
+    if (!rc) rc = enqueue_results(c, b, out, c->ss.main);
+    const int rs = bwg_map_hip(hipStreamSynchronize(c->ss.main));
+    if (!rc) rc = rs;
+    if (!rc) record_run(c, b);
+    destroy_batch(c, b);
+    return rc;
 }
 
 int bwg_extend_side_batch(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, size_t n, const uint8_t *arena,

@@ -91,6 +91,21 @@ int bwg_extend_batch(bwg_ctx *ctx, const bwg_ext_params *p, size_t n, const uint
                      const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
                      bwg_ext_result *out);
 
+/* bwg_extend_batch without any device allocation.  No counterpart in the reference: it is for
+ * callers that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same arguments, semantics and results as bwg_extend_batch;
+ * the device memory beyond the caller's inputs and outputs (the staged arena, descriptors and
+ * results, the bin queues, the plain kernel's state) comes from one 64 MiB array per GPU that the
+ * library holds for the life of the process.  A call needing more (about 250k long-tail jobs)
+ * returns BWG_E2BIG.  It runs the batched form on the context's own streams, plans with the
+ * autotuner's table as it stands (it never tunes) and replays no graph.  Calls on one GPU share
+ * the array, which keeps the last call's contents: the library serialises them (overlapping calls
+ * would race on it). */
+int bwg_extend_batch_noalloc(bwg_ctx *ctx, const bwg_ext_params *p, size_t n, const uint8_t *arena,
+                             size_t arena_len, const uint64_t *q_off, const uint32_t *q_len,
+                             const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+                             bwg_ext_result *out);
+
 /* One extension (SwBackend::extend). */
 int bwg_extend(bwg_ctx *ctx, const bwg_ext_params *p, const uint8_t *query, uint32_t qlen,
                const uint8_t *target, uint32_t tlen, int32_t h0, bwg_ext_result *out);
