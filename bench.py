@@ -4,16 +4,26 @@
 Headline workload: 1M long-tail extension jobs (reads of 100/150/250 bp, query lengths 70% < 30,
 25% 30-150, 5% 150-250, target = query + 100, band 100, h0 U[19,250]), run through the batched
 form with the batch resident in HBM.  A step is one pass of the extension kernels over the whole
-batch.  Alongside it the same batch is timed in the plain form, the 200k-job mate-rescue workload
-is timed in both forms, and the CPU oracle is timed on the host cores.
+batch.  Alongside it:
+
+- the same batch in the plain form, and per-bin device times with the bins run one after another
+  (each bin's own fraction of the integer-VALU roofline);
+- the 200k-job mate-rescue workload (minsc 20) in both forms, checked bit-exact against the oracle,
+  with its own roofline, executed-cell count and CPU baseline;
+- the small extension workload, band retry, a mate-rescue round, global alignment;
+- the reference's call shapes: 16 host threads making 2000-job extension calls and 100-job rescue
+  calls on one context, directly and through the server;
+- the synchronous call from host buffers (PCIe included) with its phases;
+- the Poisson serving sweep with 8 producers;
+- the CPU oracle on the host cores.
 
 Prints one JSON line.
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
@@ -27,45 +37,133 @@ METRIC = "extension GCUPS + jobs/s, batched vs plain GPU (bit-exact); p50/p99 ch
 K_EXT = 16   # integer ops per extension cell (crates/bwa-extend/src/sw.rs:377-412)
 K_RESC = 13  # integer ops per rescue cell (crates/bwa-extend/src/sw.rs:993-1029)
 VALU_PEAK_TOPS = 256 * 64 * 2.4e9 / 1e12  # int32 lane-ops/s on 256 CUs at 2.4 GHz
+VALU_PK16_PEAK_TOPS = 2 * VALU_PEAK_TOPS  # packed 16-bit lane-ops/s
 HBM_PEAK_GBPS = 8000.0
+EXT_P = (6, 1, 6, 1, 100, 5, 100)  # o_del e_del o_ins e_ins w end_bonus zdrop
+RESC_P = (6, 1, 6, 1, 20, 1)      # o_del e_del o_ins e_ins minsc max_sc (config 2: minsc 20)
 
 
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
+def oracle():
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import oracle_binding as ob
+    return ob
+
+
+def host_info():
+    """CPU model, online CPUs, CPUs this process may run on, and the worker threads the CPU legs
+    use: the CPUs available to the process, at most 16 (a one-GPU box's CPU share)."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    online = os.cpu_count() or 1
+    try:
+        avail = len(os.sched_getaffinity(0))
+    except AttributeError:
+        avail = online
+    return {"cpu_model": model, "cpus_online": online, "cpus_available": avail, "threads": min(16, avail)}
+
+
 def time_steps(ctx, staged, steps, warmup):
     for _ in range(warmup):
         staged.run()
-    kern_ns, main_ns, launches = [], [], []
+    kern_ns = []
     t0 = time.perf_counter()
     for _ in range(steps):
         staged.run()
         kern_ns.append(ctx.stat(bwg.STAT_LAST_KERNEL_NS))
-        main_ns.append(ctx.stat(bwg.STAT_LAST_MAIN_NS))
-        launches.append(ctx.stat(bwg.STAT_LAST_MAIN_LAUNCHES))
     wall = time.perf_counter() - t0
-    return wall / steps, float(np.mean(kern_ns)) * 1e-9, float(np.mean(main_ns)) * 1e-9, float(np.mean(launches))
+    return wall / steps, float(np.mean(kern_ns)) * 1e-9
 
 
-def cpu_baseline(ext_batch, mat, n_threads):
-    """The oracle on the host cores over the whole extension batch, with the same algorithm the
-    batched GPU form runs: the ungapped-diagonal fast path, then ksw_extend2 for the declined jobs
-    (one pthread per core, static partition).  Returns the results, the time, the workload's
-    evaluated-cell count (the GCUPS denominator) and the cells of the declined jobs (the cells the
-    DP kernels evaluate)."""
-    sys.path.insert(0, os.path.join(ROOT, "tests"))
-    import oracle_binding as ob
-    t0 = time.perf_counter()
-    out, hit = ob.ungapped_batch(ext_batch, mat, 6, 1, 6, 1, 100)
-    idx = np.nonzero(~hit)[0]
-    rest = bwg.JobBatch(ext_batch.arena, ext_batch.q_off[idx], ext_batch.q_len[idx], ext_batch.t_off[idx],
-                        ext_batch.t_len[idx], ext_batch.h0[idx])
-    out[idx] = ob.extend_batch(rest, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=n_threads)
-    dt = time.perf_counter() - t0
-    full, job_cells = ob.extend_batch_job_cells(ext_batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=n_threads)
-    assert np.array_equal(full, out)
-    return out, int(job_cells.sum()), dt, int(job_cells[idx].sum()), int(hit.sum())
+def serial_bin_times(ctx, staged, reps=3):
+    """Each launch's device time with the batch's bins run one after another on one stream
+    (HIP events around every launch), averaged over `reps` runs; and the report of the last."""
+    ctx.set_option(bwg.OPT_CONCURRENT_BINS, 0)
+    ctx.set_option(bwg.OPT_ITEM_TIMING, 1)
+    try:
+        acc = None
+        for _ in range(reps):
+            staged.run()
+            rep = staged.report()
+            us = np.array([r["us"] for r in rep])
+            acc = us if acc is None else acc + us
+        for r, t in zip(rep, acc / reps):
+            r["us"] = float(t)
+        total_s = ctx.stat(bwg.STAT_LAST_KERNEL_NS) * 1e-9
+    finally:
+        ctx.set_option(bwg.OPT_ITEM_TIMING, 0)
+        ctx.set_option(bwg.OPT_CONCURRENT_BINS, 1)
+    return rep, total_s
+
+
+def ext_bin_table(rep, bins, job_cells, dp_mask):
+    """Per extension DP bin: jobs, jobs the DP ran (declined by the ungapped pre-pass), their
+    evaluated cells, serial device time and fraction of the int32 VALU roofline."""
+    out = []
+    for r in rep:
+        sel = bins == r["index"]
+        if not sel.any() and r["kernel"] != "ext_ungapped":
+            continue
+        row = {"kernel": r["kernel"], "jobs": int(sel.sum()), "us": round(r["us"], 2)}
+        if r["kernel"] == "ext_ungapped":
+            row["jobs"] = r["jobs"]
+        else:
+            dsel = sel & dp_mask
+            cells = int(job_cells[dsel].sum())
+            row.update(dp_jobs=int(dsel.sum()), dp_cells=cells)
+            if r["us"] > 0:
+                ops = cells * K_EXT / (r["us"] * 1e-6) / 1e12
+                row["frac"] = round(ops / VALU_PEAK_TOPS, 4)
+        out.append(row)
+    return out
+
+
+def resc_bin_table(rep, bins, batch, res, max_sc=1):
+    """Per rescue launch: forward bins with their nominal (qlen x tlen) and padded cells and the
+    fraction of the roofline of their arithmetic (packed 16-bit or int32); the tracker; the
+    reverse bins with the cells of the reversed prefixes up to the forward score."""
+    q = batch.q_len.astype(np.int64)
+    t = batch.t_len.astype(np.int64)
+    lanes = np.where(q * max_sc < 250, 16, 8)
+    qpad = (q + lanes - 1) // lanes * lanes
+    rq = np.minimum(res["qe"].astype(np.int64) + 1, q)
+    rl = np.where(rq * max_sc < 250, 16, 8)
+    rpad = (rq + rl - 1) // rl * rl
+    rrows = np.where(res["tb"] >= 0, res["te"].astype(np.int64) - res["tb"] + 1, res["te"].astype(np.int64) + 1)
+    ran_rev = (res["score"] >= RESC_P[4]) & (res["qe"] >= 0)
+    fwd_of = {}
+    out = []
+    for r in rep:
+        row = {"kernel": r["kernel"], "jobs": r["jobs"], "us": round(r["us"], 2)}
+        sel = bins == r["index"]
+        packed = "pk<" in r["kernel"]
+        peak = VALU_PK16_PEAK_TOPS if packed else VALU_PEAK_TOPS
+        if sel.any():
+            fwd_of[r["kernel"].split(">")[0]] = sel
+            row["cells"] = int((q[sel] * t[sel]).sum())
+            row["padded_cells"] = int((qpad[sel] * t[sel]).sum())
+            if r["us"] > 0:
+                row["frac"] = round(row["padded_cells"] * K_RESC / (r["us"] * 1e-6) / 1e12 / peak, 4)
+                row["peak_tops"] = peak
+        elif "reverse" in r["kernel"]:
+            sel = fwd_of.get(r["kernel"].split(",reverse")[0].split(">")[0], np.zeros(len(q), bool)) & ran_rev
+            row["cells"] = int((rpad[sel] * rrows[sel]).sum())
+            if r["us"] > 0:
+                row["frac"] = round(row["cells"] * K_RESC / (r["us"] * 1e-6) / 1e12 / peak, 4)
+                row["peak_tops"] = peak
+        out.append(row)
+    rev_cells = int((rpad[ran_rev] * rrows[ran_rev]).sum())
+    return out, int((qpad * t).sum()), rev_cells
 
 
 def traffic_from_profile(jobs):
@@ -83,7 +181,90 @@ def traffic_from_profile(jobs):
     return t.get("bytes_per_batch")
 
 
+def pct(v, f):
+    return float(np.percentile(np.asarray(v), f)) if len(v) else None
+
+
+def call_shapes(ctx, threads, ext_calls, resc_calls, reps, check):
+    """The reference's call shapes on one context: `threads` host threads each making extension
+    calls of 2000 jobs, then rescue calls of 100 jobs, directly (bwg_extend_batch /
+    bwg_ksw_align2_batch on the context's execution lanes) and through the server (submit + wait
+    per call).  Returns jobs/s and call-latency percentiles; every result is compared with the same
+    call made alone beforehand."""
+    ep = bwg.ext_params(5, bwg.default_mat(), *EXT_P)
+    kp = bwg.ksw_params(5, bwg.default_mat(), *RESC_P)
+    # the calls one at a time first: the reference results, and the mean host phases of a call
+    out = {}
+    alone_e, alone_r = [], []
+    for kind, calls, alone in (("ext", ext_calls, alone_e), ("resc", resc_calls, alone_r)):
+        ph = []
+        t0 = time.perf_counter()
+        for b in calls:
+            alone.append(ctx.extend_arrays(b, ep) if kind == "ext" else ctx.align2_arrays(b, kp))
+            ph.append(ctx.phases())
+        wall = time.perf_counter() - t0
+        out["%s_one_thread" % kind] = {
+            "calls": len(calls), "jobs_per_call": len(calls[0]), "jobs_per_s": sum(len(b) for b in calls) / wall,
+            "phases_ms": {k: round(float(np.mean([p[k] for p in ph])) * 1e3, 3) for k in ph[0]}}
+        log("call shapes %s one thread: %.3g jobs/s, mean phases (ms) %s"
+            % (kind, out["%s_one_thread" % kind]["jobs_per_s"], out["%s_one_thread" % kind]["phases_ms"]))
+    if check:
+        ob = oracle()
+        mat = bwg.default_mat()
+        for b, a in zip(ext_calls[:16], alone_e[:16]):
+            assert np.array_equal(a, ob.extend_batch(b, mat, *EXT_P)), "call-shape extension vs oracle"
+        for b, a in zip(resc_calls[:32], alone_r[:32]):
+            assert np.array_equal(a, ob.align2_batch(b, mat, *RESC_P)), "call-shape rescue vs oracle"
+    srv = bwg.Server(ctx, hbm_budget_bytes=64 << 30, slots=3)
+    try:
+        for kind, calls, alone, params in (("ext", ext_calls, alone_e, ep), ("resc", resc_calls, alone_r, kp)):
+            for route in ("lanes", "direct", "server"):
+                # lanes: synchronous calls each on an execution lane (coalescing off); direct: the
+                # same calls with coalescing on (the default); server: explicit submit + wait
+                ctx.set_option(bwg.OPT_COALESCE, 0 if route == "lanes" else 1)
+                lat, bad = [], []
+                lock = threading.Lock()
+
+                def worker(t):
+                    mine = []
+                    for r in range(reps):
+                        k = (t * reps + r) % len(calls)
+                        t0 = time.perf_counter()
+                        if route != "server":
+                            got = (ctx.extend_arrays(calls[k], params) if kind == "ext"
+                                   else ctx.align2_arrays(calls[k], params))
+                        else:
+                            got = srv.wait(srv.submit(calls[k], params))
+                        mine.append(time.perf_counter() - t0)
+                        if not np.array_equal(got, alone[k]):
+                            bad.append(k)
+                    with lock:
+                        lat.extend(mine)
+
+                th = [threading.Thread(target=worker, args=(t,)) for t in range(threads)]
+                t0 = time.perf_counter()
+                for x in th:
+                    x.start()
+                for x in th:
+                    x.join()
+                wall = time.perf_counter() - t0
+                if bad:
+                    log("call shapes %s/%s: %d results differ from the call alone" % (kind, route, len(bad)))
+                    sys.exit(1)
+                njobs = sum(len(calls[(t * reps + r) % len(calls)]) for t in range(threads) for r in range(reps))
+                out["%s_%s" % (kind, route)] = {
+                    "threads": threads, "calls": threads * reps, "jobs_per_call": len(calls[0]),
+                    "jobs_per_s": njobs / wall, "p50_call_ms": pct(lat, 50) * 1e3, "p99_call_ms": pct(lat, 99) * 1e3}
+                log("call shapes %s %s: %d threads x %d calls of %d jobs: %.3g jobs/s, call p50 %.2f ms p99 %.2f ms"
+                    % (kind, route, threads, reps, len(calls[0]), njobs / wall, pct(lat, 50) * 1e3,
+                       pct(lat, 99) * 1e3))
+    finally:
+        srv.close()
+    return out
+
+
 def main():
+    hi = host_info()
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
@@ -92,18 +273,21 @@ def main():
     ap.add_argument("--rescue-jobs", type=int, default=200_000)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-check", action="store_true")
-    ap.add_argument("--cpu-threads", type=int, default=16)
+    ap.add_argument("--cpu-threads", type=int, default=hi["threads"])
     ap.add_argument("--forms", default="batched,plain", help="comma list of forms to time")
-    ap.add_argument("--only", default="ext,small,resc,side,matesw,global,e2e",
+    ap.add_argument("--only", default="ext,bins,small,resc,side,matesw,global,e2e,calls",
                     help="comma list of workloads to time")
     ap.add_argument("--global-jobs", type=int, default=200_000,
                     help="region-to-CIGAR jobs of the banded global alignment workload")
     ap.add_argument("--matesw-anchors", type=int, default=200_000,
                     help="anchors of the mate-rescue round (4 orientation jobs each)")
-    ap.add_argument("--serve-rates", default="5e4,2e5,2e6,1e7,3e7,6e7",
+    ap.add_argument("--serve-rates", default="1e3,5e4,2e5,2e6,1e7,3e7,6e7",
                     help="offered loads (jobs/s) of the serving sweep; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="seconds per offered load (longer at low loads, for at least ~40 chunks)")
+    ap.add_argument("--serve-min-chunks", type=int, default=500,
+                    help="chunks at the config-4 point (200k jobs/s) that its p50/p99 are taken over")
+    ap.add_argument("--call-reps", type=int, default=8, help="calls per thread of the call-shape leg")
     ap.add_argument("--small-jobs", type=int, default=100_000,
                     help="jobs of the small plain-extension workload (0 to skip)")
     ap.add_argument("--serial-bins", action="store_true",
@@ -117,13 +301,38 @@ def main():
         sys.exit(2)
 
     mat = bwg.default_mat()
-    ext_p = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
-    ksw_p = bwg.ksw_params(5, mat, 6, 1, 6, 1, 20, 1)
+    ext_p = bwg.ext_params(5, mat, *EXT_P)
+    ksw_p = bwg.ksw_params(5, mat, *RESC_P)
+    only = args.only.split(",")
+    log("host: %s, %d CPUs online, %d available, CPU legs on %d threads"
+        % (hi["cpu_model"], hi["cpus_online"], hi["cpus_available"], args.cpu_threads))
     log("generating %d long-tail extension jobs, %d small extension jobs and %d rescue jobs"
         % (args.jobs, args.small_jobs, args.rescue_jobs))
     ext = bwg.generate(bwg.WORKLOAD_EXT_TAIL, args.jobs, seed=42)
     resc = bwg.generate(bwg.WORKLOAD_RESCUE, args.rescue_jobs, seed=42)
     small = bwg.generate(bwg.WORKLOAD_EXT_SMALL, args.small_jobs, seed=42) if args.small_jobs else None
+
+    # the oracle's per-job cells and ungapped-diagonal hits: the GCUPS and roofline denominators
+    cpu = cpu_resc = None
+    job_cells = hit = None
+    ob = oracle()  # the checker, the cell counts (the GCUPS unit) and the CPU baseline
+    if not args.no_cpu or "bins" in only:
+        log("oracle over the extension batch on %d threads (cells and the CPU baseline)" % args.cpu_threads)
+        t0 = time.perf_counter()
+        cpu_out, hit = ob.ungapped_batch(ext, mat, 6, 1, 6, 1, 100)
+        idx = np.nonzero(~hit)[0]
+        rest = bwg.JobBatch(ext.arena, ext.q_off[idx], ext.q_len[idx], ext.t_off[idx], ext.t_len[idx], ext.h0[idx])
+        cpu_out[idx] = ob.extend_batch(rest, mat, *EXT_P, nthreads=args.cpu_threads)
+        cpu_dt = time.perf_counter() - t0
+        full, job_cells = ob.extend_batch_job_cells(ext, mat, *EXT_P, nthreads=args.cpu_threads)
+        assert np.array_equal(full, cpu_out)
+        cpu = {"value": args.jobs / cpu_dt, "unit": "jobs/s", "cores": args.cpu_threads, "kind": "port",
+               "gcups": float(job_cells.sum()) / cpu_dt / 1e9, "cpu_model": hi["cpu_model"],
+               "cpus_online": hi["cpus_online"], "cpus_available": hi["cpus_available"],
+               "sample": "all %d long-tail extension jobs: the ungapped-diagonal fast path, then the "
+                         "scalar C oracle's ksw_extend2 for the declined jobs, one thread per core, "
+                         "static partition" % args.jobs}
+        log("cpu oracle: %.3g jobs/s on %d threads" % (cpu["value"], args.cpu_threads))
 
     ctx = bwg.Context(0)
     if args.serial_bins:
@@ -133,67 +342,91 @@ def main():
         copy_gbs = ctx.hbm_copy(args.calib_copy_mb << 20, reps=1)
         log("calibration copy: hbm_copy_kernel, %d MiB read + %d MiB written per launch, %.0f GB/s"
             % (args.calib_copy_mb, args.calib_copy_mb, copy_gbs))
-    res = {}
-    outs = {}
+    res, outs = {}, {}
+    ext_bins = resc_bins = None
+    resc_exec = None
     forms = [f for f in ((bwg.FORM_BATCHED, "batched"), (bwg.FORM_PLAIN, "plain")) if f[1] in args.forms.split(",")]
-    only = args.only.split(",")
     for form, fname in forms:
-        try:
-            ctx.set_form(form)
-            if "ext" not in only:
-                raise KeyError
+        ctx.set_form(form)
+        if "ext" in only:
             log("extension %s: staging" % fname)
             st = ctx.stage(ext, ext_p)
-            st.run()  # first run builds the form's plan and buffers
-            per, kern, main, nl = time_steps(ctx, st, args.steps if form == bwg.FORM_BATCHED else max(1, args.steps // 3), args.warmup)
+            st.run()  # first run builds the form's plan and buffers (and tunes)
+            if form == bwg.FORM_BATCHED:
+                ctx.hbm_copy(1 << 20, reps=1)  # trace marker: the timed steps follow this copy kernel
+            per, kern = time_steps(ctx, st, args.steps if form == bwg.FORM_BATCHED else max(1, args.steps // 3),
+                                   args.warmup)
             outs["ext_" + fname] = st.results()
             hits = st.ungapped_hits()
-            st.close()
-            res["ext_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl, ungapped_hits=hits)
+            res["ext_" + fname] = dict(step_s=per, kernel_s=kern, ungapped_hits=hits)
             log("extension %s: %.3f ms/step (kernels %.3f ms), %d jobs resolved by the ungapped fast path"
                 % (fname, per * 1e3, kern * 1e3, hits))
+            if form == bwg.FORM_BATCHED and "bins" in only and job_cells is not None:
+                rep, tot = serial_bin_times(ctx, st)
+                ext_bins = ext_bin_table(rep, st.job_bins(), job_cells, ~hit)
+                res["ext_serial"] = dict(kernel_s=tot)
+                for row in ext_bins:
+                    log("  serial bin %-22s jobs %7d  %8.1f us  frac %s" % (row["kernel"], row["jobs"], row["us"],
+                                                                            row.get("frac")))
+            st.close()
             if form == bwg.FORM_BATCHED:
                 # the same batch with every job through the banded DP, for comparison
                 ctx.set_option(bwg.OPT_UNGAPPED, 0)
                 st = ctx.stage(ext, ext_p)
                 st.run()
-                per0, kern0, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+                per0, kern0 = time_steps(ctx, st, max(1, args.steps // 2), 1)
                 assert np.array_equal(st.results(), outs["ext_batched"]), "fast path changes results"
                 st.close()
                 ctx.set_option(bwg.OPT_UNGAPPED, 1)
                 res["ext_batched_dp_only"] = dict(step_s=per0, kernel_s=kern0)
                 log("extension batched, fast path off: %.3f ms/step (kernels %.3f ms)" % (per0 * 1e3, kern0 * 1e3))
-        except KeyError:
-            pass
-        except bwg.BwgError as e:
-            log("form %s unavailable: %s" % (fname, e))
-        try:
-            if "small" not in only or small is None:
-                raise KeyError
+        if "small" in only and small is not None:
             st = ctx.stage(small, ext_p)
             st.run()
-            per, kern, main, nl = time_steps(ctx, st, args.steps, args.warmup)
+            per, kern = time_steps(ctx, st, args.steps, args.warmup)
             outs["small_" + fname] = st.results()
             st.close()
-            res["small_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
+            res["small_" + fname] = dict(step_s=per, kernel_s=kern)
             log("small extension %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
-        except KeyError:
-            pass
-        try:
-            if "resc" not in only:
-                raise KeyError
+        if "resc" in only:
             log("rescue %s: staging" % fname)
             st = ctx.stage(resc, ksw_p)
             st.run()
-            per, kern, main, nl = time_steps(ctx, st, max(1, args.steps // 2) if form == bwg.FORM_BATCHED else 2, 1)
+            per, kern = time_steps(ctx, st, max(1, args.steps // 2) if form == bwg.FORM_BATCHED else 2, 1)
             outs["resc_" + fname] = st.results()
-            st.close()
-            res["resc_" + fname] = dict(step_s=per, kernel_s=kern, main_s=main, main_launches=nl)
+            res["resc_" + fname] = dict(step_s=per, kernel_s=kern)
             log("rescue %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
-        except KeyError:
-            pass
-        except bwg.BwgError as e:
-            log("form %s unavailable: %s" % (fname, e))
+            if form == bwg.FORM_BATCHED and "bins" in only:
+                rep, tot = serial_bin_times(ctx, st)
+                resc_bins, fwd_pad, rev_cells = resc_bin_table(rep, st.job_bins(), resc, outs["resc_batched"])
+                resc_exec = dict(forward_padded_cells=fwd_pad, reverse_cells=rev_cells, serial_kernel_s=tot)
+                for row in resc_bins:
+                    log("  serial launch %-28s jobs %7d  %8.1f us  frac %s" % (row["kernel"], row["jobs"], row["us"],
+                                                                               row.get("frac")))
+            st.close()
+
+    # rescue: both forms agree, and the oracle (config 2's minsc 20) agrees with them
+    if "resc_batched" in outs and "resc_plain" in outs and not np.array_equal(outs["resc_batched"], outs["resc_plain"]):
+        log("rescue forms disagree")
+        sys.exit(1)
+    resc_head = "resc_batched" if "resc_batched" in outs else ("resc_plain" if "resc_plain" in outs else None)
+    if resc_head and not (args.no_cpu and args.no_check):
+        log("oracle over the %d rescue jobs on %d threads" % (args.rescue_jobs, args.cpu_threads))
+        t0 = time.perf_counter()
+        want = ob.align2_batch(resc, mat, *RESC_P, nthreads=args.cpu_threads)
+        dt = time.perf_counter() - t0
+        ok = np.array_equal(want, outs[resc_head])
+        log("rescue bit-exact vs oracle (minsc %d): %s" % (RESC_P[4], ok))
+        if not ok:
+            bad = np.nonzero(want != outs[resc_head])[0]
+            log("rescue mismatches: %d, first %d" % (bad.size, bad[0]))
+            sys.exit(1)
+        nominal = float(np.sum(resc.q_len.astype(np.int64) * resc.t_len))
+        cpu_resc = {"value": args.rescue_jobs / dt, "unit": "jobs/s", "cores": args.cpu_threads, "kind": "port",
+                    "gcell_s": nominal / dt / 1e9,
+                    "sample": "all %d rescue jobs: the scalar C oracle's batched ksw_align2 (forward and "
+                              "reverse passes, tracker), one thread per core" % args.rescue_jobs}
+
     side = {}
     if "side" in only:
         # one side of the seed extensions with bwa's band doubling, all rounds on the device: the
@@ -206,7 +439,7 @@ def main():
             log("side batch w=%d: staging" % w)
             st = ctx.stage_side(ext, p, prev, 2)
             st.run()
-            per, kern, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+            per, kern = time_steps(ctx, st, max(1, args.steps // 2), 1)
             bands = st.side_bands()
             if w == 100 and "ext_batched" in outs:
                 assert np.array_equal(st.results(), outs["ext_batched"]), "side batch differs from the plain batch"
@@ -225,7 +458,7 @@ def main():
         msw_p = bwg.ksw_params(5, mat, 6, 1, 6, 1, rnd.min_seed_len, 1)  # minsc = min_seed_len * a
         st = ctx.stage_matesw(rnd, msw_p)
         st.run()
-        per, kern, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+        per, kern = time_steps(ctx, st, max(1, args.steps // 2), 1)
         win, out = st.windows(), st.results()
         st.close()
         valid = (win["flags"] & bwg.WIN_VALID) != 0
@@ -235,8 +468,6 @@ def main():
         log("mate rescue: %.3f ms/step (kernels %.3f ms), %d windows aligned, %.0f G cells/s"
             % (per * 1e3, kern * 1e3, matesw["jobs"], cells / per / 1e9))
         if not args.no_check:
-            sys.path.insert(0, os.path.join(ROOT, "tests"))
-            import oracle_binding as ob
             sub_n = min(10_000, args.matesw_anchors)
             part = bwg.RescueRound(rnd.fwd, rnd.contigs, rnd.pes, rnd.anchors[:sub_n], rnd.ma_rb[:rnd.n_ma],
                                    rnd.arena, rnd.min_seed_len)
@@ -255,7 +486,7 @@ def main():
         gp = bwg.global_params(5, mat, 6, 1, 6, 1)
         st = ctx.stage_global(gb, gp, gw)
         st.run()
-        per, kern, _, _ = time_steps(ctx, st, max(1, args.steps // 2), 1)
+        per, kern = time_steps(ctx, st, max(1, args.steps // 2), 1)
         gscore = st.results()
         gn, gc = st.cigars()
         st.close()
@@ -265,38 +496,56 @@ def main():
         log("global alignment: %.3f ms/step (kernels %.3f ms), %.3g jobs/s" % (per * 1e3, kern * 1e3,
                                                                                  args.global_jobs / per))
         if not args.no_check:
-            sys.path.insert(0, os.path.join(ROOT, "tests"))
-            import oracle_binding as ob
             ws, wn, wc = ob.global_batch(gb, gw, mat, 6, 1, 6, 1, nthreads=args.cpu_threads)
             ok = np.array_equal(ws, gscore) and np.array_equal(wn.astype(np.uint32), gn) and np.array_equal(wc, gc)
             log("global alignment: bit-exact vs oracle: %s" % ok)
             if not ok:
                 sys.exit(1)
-    e2e = None
+    e2e = e2e_phases = None
     if "e2e" in only and "ext" in only:
-        # PCIe-inclusive: the synchronous entry point from host buffers (stage, run, copy back)
+        # PCIe-inclusive: the synchronous entry point from host buffers (stage, plan, run, copy
+        # back), with the host time of each phase of the last call
         ctx.set_form(bwg.FORM_BATCHED)
         ctx.extend_arrays(ext, ext_p)
         t0 = time.perf_counter()
         reps = 3
         for _ in range(reps):
-            ctx.extend_arrays(ext, ext_p)
+            got = ctx.extend_arrays(ext, ext_p)
         e2e = args.jobs * reps / (time.perf_counter() - t0)
-        log("extension from host buffers (PCIe-inclusive): %.3g jobs/s" % e2e)
+        e2e_phases = {k: round(v * 1e3, 3) for k, v in ctx.phases().items()}
+        e2e_phases["device_kernels"] = round(ctx.stat(bwg.STAT_LAST_KERNEL_NS) * 1e-6, 3)
+        if "ext_batched" in outs and not np.array_equal(got, outs["ext_batched"]):
+            log("synchronous call differs from the staged batch")
+            sys.exit(1)
+        log("extension from host buffers (PCIe-inclusive): %.3g jobs/s; phases of the last call (ms): %s"
+            % (e2e, e2e_phases))
+    calls = None
+    if "calls" in only:
+        ctx.set_form(bwg.FORM_BATCHED)
+        threads = 16
+        ext_calls = [bwg.generate(bwg.WORKLOAD_EXT_TAIL, 2000, seed=1000 + k) for k in range(64)]
+        resc_calls = [bwg.generate(bwg.WORKLOAD_RESCUE, 100, seed=2000 + k) for k in range(128)]
+        calls = call_shapes(ctx, threads, ext_calls, resc_calls, args.call_reps, not args.no_check)
     serving = []
     rates = [float(x) for x in args.serve_rates.split(",") if x.strip()]
     if rates:
         ctx.set_form(bwg.FORM_BATCHED)
         srv = bwg.Server(ctx, hbm_budget_bytes=64 << 30, slots=3)
         for rate in rates:
-            # chunks average ~15k jobs: run long enough at low loads for ~40 of them
-            secs = max(args.serve_seconds, min(15.0, 40 * 15000.0 / rate))
+            # chunks average ~15k jobs: the config-4 point (200k jobs/s) runs for at least
+            # serve_min_chunks chunks; the others for ~40 chunks within 20 s, and at least
+            # serve_seconds (so the lowest rates complete only a few chunks: their count is reported)
+            if abs(rate - 2e5) < 1:
+                secs = args.serve_min_chunks * 15500.0 / rate
+            else:
+                secs = max(args.serve_seconds, min(20.0, 40 * 15000.0 / rate))
             r = bwg.serve_run(ctx, srv, rate, producers=8, duration_s=secs, seed=42,
                               use_ref=True, pool_chunks=64, verify=True)
             r["offered_jobs_per_s"] = rate
+            r["seconds"] = secs
             serving.append(r)
-            log("serving at %.3g jobs/s: p50 %.2f ms p99 %.2f ms, completed %.3g jobs/s, %d chunks, "
-                "%d mismatched" % (rate, r["p50_ms"], r["p99_ms"], r["jobs_per_s"], r["chunks"],
+            log("serving at %.3g jobs/s for %.0f s: p50 %.2f ms p99 %.2f ms over %d chunks, completed %.3g jobs/s, "
+                "%d mismatched" % (rate, secs, r["p50_ms"], r["p99_ms"], r["chunks"], r["jobs_per_s"],
                                    r["mismatched_chunks"]))
             if r["mismatched_chunks"] or r["failed_chunks"]:
                 log("serving results differ from the chunks run alone")
@@ -310,32 +559,22 @@ def main():
         assert np.array_equal(outs["ext_batched"], outs["ext_plain"]), "forms disagree"
     if "small_batched" in outs and "small_plain" in outs:
         assert np.array_equal(outs["small_batched"], outs["small_plain"]), "forms disagree"
-
-    cpu = None
-    cells = None
+    if cpu and not args.no_check and head in outs:
+        ok = np.array_equal(cpu_out, outs[head])
+        log("extension bit-exact vs oracle: %s" % ok)
+        if not ok:
+            bad = np.nonzero(cpu_out != outs[head])[0]
+            log("mismatches: %d, first %d" % (bad.size, bad[0]))
+            sys.exit(1)
     small_cells = None
-    dp_cells = cpu_hits = None
-    if not args.no_cpu:
-        log("cpu oracle over the extension batch on %d threads" % args.cpu_threads)
-        cpu_out, cells, cpu_dt, dp_cells, cpu_hits = cpu_baseline(ext, mat, args.cpu_threads)
-        if not args.no_check and head in outs:
-            ok = np.array_equal(cpu_out, outs[head])
-            log("bit-exact vs oracle: %s" % ok)
-            if not ok:
-                bad = np.nonzero(cpu_out != outs[head])[0]
-                log("mismatches: %d, first %d" % (bad.size, bad[0]))
-                sys.exit(1)
-        cpu = {"value": args.jobs / cpu_dt, "unit": "jobs/s", "cores": args.cpu_threads,
-               "kind": "port", "gcups": cells / cpu_dt / 1e9,
-               "sample": "all %d long-tail extension jobs: the ungapped-diagonal fast path, then the "
-                         "scalar C oracle's ksw_extend2 for the declined jobs, static partition" % args.jobs}
-        small_key = "small_batched" if "small_batched" in outs else "small_plain"
-        if small_key in outs:
-            s_out, small_cells, _, _, _ = cpu_baseline(small, mat, args.cpu_threads)
-            ok = np.array_equal(s_out, outs[small_key])
-            log("small workload bit-exact vs oracle: %s" % ok)
-            if not ok:
-                sys.exit(1)
+    small_key = "small_batched" if "small_batched" in outs else "small_plain"
+    if ob is not None and small_key in outs:
+        s_full, s_cells = ob.extend_batch_job_cells(small, mat, *EXT_P, nthreads=args.cpu_threads)
+        small_cells = int(s_cells.sum())
+        ok = np.array_equal(s_full, outs[small_key])
+        log("small workload bit-exact vs oracle: %s" % ok)
+        if not ok:
+            sys.exit(1)
 
     h = res[head]
     jobs_s = args.jobs / h["step_s"]
@@ -348,13 +587,15 @@ def main():
                                "end_bonus=5, h0 U[19,250], seed 42" % args.jobs,
                    "form": head.split("_")[1]},
     }
-    if cells:
+    if job_cells is not None:
+        cells = int(job_cells.sum())
+        dp_cells = int(job_cells[~hit].sum())
         line["gcups"] = cells / h["step_s"] / 1e9
         line["gcups_kernel"] = cells / h["kernel_s"] / 1e9
         line["cells"] = cells
         # the DP kernels evaluate the cells of the jobs the ungapped fast path declines
         if h.get("ungapped_hits"):
-            assert h["ungapped_hits"] == cpu_hits, "device and oracle fast paths disagree"
+            assert h["ungapped_hits"] == int(hit.sum()), "device and oracle fast paths disagree"
             line["ungapped_hits"] = h["ungapped_hits"]
             line["dp_cells"] = dp_cells
         ops = (dp_cells if h.get("ungapped_hits") else cells) * K_EXT / h["kernel_s"] / 1e12
@@ -366,15 +607,25 @@ def main():
                             "hbm": {"achieved": alg_bytes / h["kernel_s"] / 1e9, "peak": HBM_PEAK_GBPS,
                                     "unit": "GB/s", "frac": alg_bytes / h["kernel_s"] / 1e9 / HBM_PEAK_GBPS,
                                     "alg_bytes": alg_bytes}}
+        if "ext_batched_dp_only" in res:
+            line["batched_dp_only_roofline_frac"] = (cells * K_EXT / res["ext_batched_dp_only"]["kernel_s"] / 1e12
+                                                     / VALU_PEAK_TOPS)
+    if ext_bins:
+        line["ext_bins_serial"] = ext_bins
+        line["ext_serial_kernel_ms"] = res["ext_serial"]["kernel_s"] * 1e3
     if "ext_batched_dp_only" in res:
         line["batched_dp_only_jobs_per_s"] = args.jobs / res["ext_batched_dp_only"]["step_s"]
-        if cells:
-            line["batched_dp_only_roofline_frac"] = cells * K_EXT / res["ext_batched_dp_only"]["kernel_s"] / 1e12 / VALU_PEAK_TOPS
     if "ext_plain" in res:
         line["plain_jobs_per_s"] = args.jobs / res["ext_plain"]["step_s"]
+        # batched form with the ungapped pre-pass (value) and without it, each over the plain form
         line["speedup_vs_plain"] = jobs_s / line["plain_jobs_per_s"]
+        if "batched_dp_only_jobs_per_s" in line:
+            line["speedup_vs_plain_dp_only"] = line["batched_dp_only_jobs_per_s"] / line["plain_jobs_per_s"]
     if e2e:
         line["e2e_jobs_per_s"] = e2e
+        line["e2e_phases_ms"] = e2e_phases
+    if calls:
+        line["call_shapes"] = calls
     if matesw:
         line["matesw_anchors_per_s"] = args.matesw_anchors / matesw["step_s"]
         line["matesw_jobs_per_s"] = matesw["jobs"] / matesw["step_s"]
@@ -389,17 +640,35 @@ def main():
         if "small_" + f in res:
             r = res["small_" + f]
             line["small_%s_jobs_per_s" % f] = args.small_jobs / r["step_s"]
-            if cpu and small_cells:
+            if small_cells:
                 line["small_%s_gcups" % f] = small_cells / r["step_s"] / 1e9
+    nominal = float(np.sum(resc.q_len.astype(np.int64) * resc.t_len))
     for f in ("batched", "plain"):
         if "resc_" + f in res:
             r = res["resc_" + f]
             line["rescue_%s_jobs_per_s" % f] = args.rescue_jobs / r["step_s"]
-            line["rescue_%s_gcell_s" % f] = float(np.sum(resc.q_len.astype(np.int64) * resc.t_len)) / r["step_s"] / 1e9
+            line["rescue_%s_gcell_s" % f] = nominal / r["step_s"] / 1e9
+    if "resc_batched" in res:
+        r = res["resc_batched"]
+        # forward-pass nominal cells x 13 over the whole rescue launch sequence (forward, tracker,
+        # reverse); every forward bin of this workload is the packed 16-bit kernel
+        packed = resc_bins is not None and all("pk<" in b["kernel"] for b in resc_bins if "cells" in b
+                                               and "reverse" not in b["kernel"])
+        peak = VALU_PK16_PEAK_TOPS if packed else VALU_PEAK_TOPS
+        ach = nominal * K_RESC / r["kernel_s"] / 1e12
+        line["rescue_roofline"] = {"bound": "valu", "achieved": ach, "peak": peak, "unit": "Tops/s",
+                                   "frac": ach / peak, "cells": nominal,
+                                   "arith": "packed 16-bit" if packed else "int32"}
+        if resc_exec:
+            line["rescue_roofline"].update(resc_exec)
+            line["rescue_roofline"]["executed_cells"] = resc_exec["forward_padded_cells"] + resc_exec["reverse_cells"]
+        if resc_bins:
+            line["rescue_bins_serial"] = resc_bins
     if serving:
         at_cfg = min(serving, key=lambda r: abs(r["offered_jobs_per_s"] - 2e5))
         line["p50_chunk_latency_ms"] = at_cfg["p50_ms"]
         line["p99_chunk_latency_ms"] = at_cfg["p99_ms"]
+        line["latency_chunks"] = at_cfg["chunks"]
         line["serving"] = [{k: (round(v, 4) if isinstance(v, float) else v) for k, v in r.items()}
                            for r in serving]
     rep = ctx.tune_report()
@@ -407,6 +676,8 @@ def main():
         line["tuning"] = rep.splitlines()
     if cpu:
         line["cpu_baseline"] = cpu
+    if cpu_resc:
+        line["cpu_baseline_rescue"] = cpu_resc
     print(json.dumps(line), flush=True)
 
 

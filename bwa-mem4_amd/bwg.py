@@ -34,6 +34,7 @@ FORM_PLAIN, FORM_BATCHED = 0, 1
 OPT_FORM, OPT_CONCURRENT_BINS, OPT_GRAPHS, OPT_EXT_THREAD_MAX_Q = 1, 2, 3, 4
 OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP, OPT_EXT_SCORE_WIDTH, OPT_UNGAPPED = 5, 6, 7, 8, 9
 OPT_KSW_SCORE_WIDTH, OPT_LANES, OPT_POOL_CACHE_MB, OPT_POOL_PEAK_RESET = 10, 11, 12, 13
+OPT_ITEM_TIMING, OPT_COALESCE = 14, 15
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 STAT_POOL_IN_USE, STAT_POOL_CACHED, STAT_POOL_PEAK = 105, 106, 107
 # host wall time of the phases of the last synchronous extension / rescue call (ns)
@@ -50,7 +51,7 @@ EXPORTED_SYMBOLS = (
     "bwg_batch_ungapped_hits", "bwg_extend_side_batch", "bwg_batch_stage_side", "bwg_batch_side_bands",
     "bwg_set_reference", "bwg_matesw_batch", "bwg_batch_stage_matesw", "bwg_batch_windows",
     "bwg_global_batch", "bwg_batch_stage_global", "bwg_batch_cigar_lengths", "bwg_batch_cigars",
-    "bwg_extend_batch_noalloc",
+    "bwg_extend_batch_noalloc", "bwg_batch_report", "bwg_batch_job_bins",
 )
 
 # bwg_mate_anchor, bwg_rescue_window, bwg_pe_orient and bwg_contig as numpy records
@@ -105,6 +106,10 @@ def _lib():
         L.bwg_strerror.argtypes = [C.c_int]
         L.bwg_extend_batch.restype = C.c_int
         L.bwg_extend_batch.argtypes = [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P]
+        L.bwg_batch_report.restype = C.c_int
+        L.bwg_batch_report.argtypes = [_P, _P, C.c_char_p, C.c_size_t]
+        L.bwg_batch_job_bins.restype = C.c_int
+        L.bwg_batch_job_bins.argtypes = [_P, _P, _P]
         L.bwg_extend_batch_noalloc.restype = C.c_int
         L.bwg_extend_batch_noalloc.argtypes = [_P, _P, C.c_size_t, _P, C.c_size_t, _P, _P, _P, _P, _P, _P]
         L.bwg_extend.restype = C.c_int
@@ -428,6 +433,29 @@ class StagedBatch:
         w = np.zeros(self.n, dtype=np.int32)
         _check(_lib().bwg_batch_side_bands(self.ctx.h, self.h, w.ctypes.data), "bwg_batch_side_bands")
         return w
+
+    def report(self):
+        """The launches of the last batched run: dicts with index, round, stage, kernel, jobs, grid
+        and us (device time; -1 unless OPT_ITEM_TIMING was on)."""
+        n = _lib().bwg_batch_report(self.ctx.h, self.h, None, 0)
+        _check(min(n, 0), "bwg_batch_report")
+        buf = C.create_string_buffer(n + 1)
+        _lib().bwg_batch_report(self.ctx.h, self.h, buf, n + 1)
+        rows = []
+        for line in buf.value.decode().splitlines():
+            f = line.split()
+            d = {"index": int(f[0]), "kernel": f[3]}
+            for kv in f[1:3] + f[4:]:
+                k, v = kv.split("=")
+                d[k] = float(v) if k == "us" else int(v)
+            rows.append(d)
+        return rows
+
+    def job_bins(self):
+        """Each job's launch index (into report()) of the bin it was planned into, -1 for none."""
+        out = np.zeros(self.n, dtype=np.int32)
+        _check(_lib().bwg_batch_job_bins(self.ctx.h, self.h, out.ctypes.data), "bwg_batch_job_bins")
+        return out
 
     def ungapped_hits(self):
         """Jobs of the last batched run resolved by the ungapped-diagonal fast path."""

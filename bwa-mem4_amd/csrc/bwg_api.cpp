@@ -111,6 +111,7 @@ int bwg_create(int device, bwg_ctx **out) {
 
 void bwg_destroy(bwg_ctx *c) {
     if (!c) return;
+    if (c->coalescer) bwg_server_destroy(c->coalescer);
     (void)hipSetDevice(c->device);
     c->ss.destroy();
     for (auto &l : c->lanes) l->destroy();
@@ -184,6 +185,12 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
     case BWG_OPT_POOL_PEAK_RESET:
         c->pool.reset_peak();
         return BWG_OK;
+    case BWG_OPT_ITEM_TIMING:
+        t.item_timing = v != 0;
+        return BWG_OK;
+    case BWG_OPT_COALESCE:
+        t.coalesce = v != 0;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -215,6 +222,8 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_KSW_SCORE_WIDTH: return t.ksw_force_wide ? 32 : 16;
     case BWG_OPT_LANES: return c->max_lanes;
     case BWG_OPT_POOL_CACHE_MB: return (int64_t)(c->pool.cache_cap() >> 20);
+    case BWG_OPT_ITEM_TIMING: return t.item_timing;
+    case BWG_OPT_COALESCE: return t.coalesce;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns.load();
@@ -365,6 +374,37 @@ int check_matesw(bwg_ctx *c, const bwg_ksw_params *p, int32_t min_seed_len, cons
         }
     }
     return BWG_OK;
+}
+
+// Concurrent small synchronous calls.  A call counts itself in c->sync_active for its duration; a
+// call of up to kCoalesceJobs jobs that finds another one running is submitted as a chunk to the
+// context's own server, which merges it with whatever else is waiting into one device batch.
+constexpr size_t kCoalesceJobs = 65536;
+
+struct ActiveCall {
+    bwg_ctx *c;
+    int before;
+    explicit ActiveCall(bwg_ctx *ctx) : c(ctx), before(ctx->sync_active.fetch_add(1)) {}
+    ~ActiveCall() { c->sync_active.fetch_sub(1); }
+};
+
+bool coalesces(bwg_ctx *c, const ActiveCall &a, size_t n) {
+    return n <= kCoalesceJobs && a.before > 0 && c->tuning_snapshot().coalesce;
+}
+
+int run_coalesced(bwg_ctx *c, const bwg_chunk &ch) {
+    bwg_server *s;
+    {
+        std::lock_guard<std::mutex> g(c->coalesce_mu);
+        if (!c->coalescer) {
+            const int rc = bwg_server_create(c, 16ull << 30, 2, &c->coalescer);
+            if (rc) return rc;
+        }
+        s = c->coalescer;
+    }
+    uint64_t ticket = 0;
+    const int rc = bwg_server_submit(s, &ch, &ticket);
+    return rc ? rc : bwg_server_wait(s, ticket);
 }
 
 // Staged-batch calls: stage on the context's own streams under run_mu, then wait, so that the
@@ -520,6 +560,25 @@ int bwg_batch_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
     return batched_ungapped_hits(c, b, hits);
 }
 
+int bwg_batch_report(bwg_ctx *c, bwg_batch *b, char *buf, size_t len) {
+    if (!c || !b) return BWG_EINVAL;
+    if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    const std::string r = batched_report(b);
+    if (buf && len) {
+        const size_t m = std::min(len - 1, r.size());
+        std::memcpy(buf, r.data(), m);
+        buf[m] = 0;
+    }
+    return (int)r.size();
+}
+
+int bwg_batch_job_bins(bwg_ctx *c, bwg_batch *b, int32_t *bin) {
+    if (!c || !b || (!bin && b->n)) return BWG_EINVAL;
+    std::lock_guard<std::mutex> g(c->run_mu);
+    return b->n ? batched_job_bins(b, bin) : BWG_OK;
+}
+
 int bwg_batch_run(bwg_ctx *c, bwg_batch *b) {
     if (!c || !b) return BWG_EINVAL;
     return run_batch(c, b);
@@ -545,6 +604,22 @@ int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_
     if (n == 0) return BWG_OK;
     if (!out) return BWG_EINVAL;
     const Clock::time_point t0 = Clock::now();
+    ActiveCall active(c);
+    if (coalesces(c, active, n)) {
+        bwg_chunk ch{};
+        ch.kind = BWG_CHUNK_EXTEND;
+        ch.ext = p;
+        ch.n = n;
+        ch.arena = arena;
+        ch.arena_len = arena_len;
+        ch.q_off = q_off;
+        ch.q_len = q_len;
+        ch.t_off = t_off;
+        ch.t_len = t_len;
+        ch.h0 = h0;
+        ch.out = out;
+        return run_coalesced(c, ch);  // the server validates the chunk as this call would
+    }
     int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
     if (rc) return rc;
     const ExtScoring sc = ext_scoring(*p);
@@ -637,6 +712,21 @@ int bwg_ksw_align2_batch(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const ui
     if (n == 0) return BWG_OK;
     if (!out) return BWG_EINVAL;
     const Clock::time_point t0 = Clock::now();
+    ActiveCall active(c);
+    if (coalesces(c, active, n)) {
+        bwg_chunk ch{};
+        ch.kind = BWG_CHUNK_RESCUE;
+        ch.ksw = p;
+        ch.n = n;
+        ch.arena = arena;
+        ch.arena_len = arena_len;
+        ch.q_off = q_off;
+        ch.q_len = q_len;
+        ch.t_off = t_off;
+        ch.t_len = t_len;
+        ch.out = out;
+        return run_coalesced(c, ch);
+    }
     int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len);
     if (rc) return rc;
     const KswScoring sc = ksw_scoring(*p);

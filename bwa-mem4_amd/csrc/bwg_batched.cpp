@@ -16,6 +16,7 @@
 // (queue reset, every stage's kernels and the fork/join dependencies) is replayed from a captured
 // hipGraph.
 #include <algorithm>
+#include <cstdio>
 #include <cstring>
 #include <numeric>
 #include <vector>
@@ -43,6 +44,7 @@ struct Plan {
     int retry_round = 0;   // extension: 0, or the band-retry round this plan runs
     bool compact = false;  // group and thread bins drain compacted queues filled by a stage-0 item
     std::vector<Item> items;
+    std::vector<hipEvent_t> ev0, ev1;  // per-item timing events (item_timing), created on first use
     int n_heads = 0;
     uint32_t *d_order = nullptr;
     uint32_t *d_heads = nullptr;
@@ -119,8 +121,9 @@ inline uint32_t work_key(uint64_t w) {
 }
 
 // Group jobs into bins by key, order the bins by stage then descending work and each bin's jobs
-// by descending work; returns the order array, bins get their offsets.  Two stable counting-sort
-// passes (work key descending, then bin), O(n) for any batch size.
+// by descending work; returns the order array, bins get their offsets.  Large batches: two stable
+// counting-sort passes (work key descending, then bin), O(n); small ones: one comparison sort by
+// (bin, work key, index), the same order without the 65537-entry histogram.
 template <class KeyFn, class WorkFn>
 std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, WorkFn work_of) {
     std::vector<int> key(n);
@@ -129,10 +132,8 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
         key[k] = key_of(k);
         const uint64_t w = work_of(k);
         wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
-    }
-    for (size_t k = 0; k < n; ++k) {
         bins[key[k]].count++;
-        bins[key[k]].work += (double)work_of(k);
+        bins[key[k]].work += (double)w;
     }
     std::vector<int> perm(bins.size());
     std::iota(perm.begin(), perm.end(), 0);
@@ -151,13 +152,25 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
         b.offset = off;
         off += b.count;
     }
+    std::vector<uint32_t> order(n);
+    if (n < 16384) {
+        std::iota(order.begin(), order.end(), 0u);
+        std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
+            const int bx = remap[key[x]], by = remap[key[y]];
+            if (bx != by) return bx < by;
+            if (wk[x] != wk[y]) return wk[x] < wk[y];
+            return x < y;
+        });
+        bins.swap(sorted);
+        return order;
+    }
     // pass 1: by descending work
     std::vector<uint32_t> hist(65537, 0), tmp(n);
     for (size_t k = 0; k < n; ++k) hist[(size_t)wk[k] + 1]++;
     for (size_t v = 1; v < hist.size(); ++v) hist[v] += hist[v - 1];
     for (size_t k = 0; k < n; ++k) tmp[hist[wk[k]]++] = (uint32_t)k;
     // pass 2: stable by bin
-    std::vector<uint32_t> order(n), cur(sorted.size());
+    std::vector<uint32_t> cur(sorted.size());
     for (size_t r = 0; r < sorted.size(); ++r) cur[r] = sorted[r].offset;
     for (size_t r = 0; r < n; ++r) {
         const uint32_t k = tmp[r];
@@ -166,6 +179,26 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
     bins.swap(sorted);
     return order;
 }
+
+// The bins of a plan, created on first use and found again through a table indexed by the kernel
+// mapping (kind, K or column capacity, state width), not by a search per job.
+struct BinSet {
+    std::vector<Item> bins;
+    int16_t lut[16][65][2];
+    BinSet() { std::memset(lut, 0xff, sizeof lut); }
+    int find(int kind, int param, bool narrow, int stage = 0) {
+        int16_t &slot = lut[kind & 15][std::min(std::max(param, 0), 64)][narrow ? 1 : 0];
+        if (slot >= 0) return slot;
+        Item it;
+        it.kind = kind;
+        it.param = param;
+        it.narrow = narrow;
+        it.stage = stage;
+        bins.push_back(it);
+        slot = (int16_t)(bins.size() - 1);
+        return slot;
+    }
+};
 
 // Persistent grid of a bin: enough workgroups to fill every CU at the kernel's occupancy, but no
 // more than the bin has work for.
@@ -196,6 +229,37 @@ int ext_class_of(const ExtJob &j, const ExtScoring &sc, const Tuning &tu, int *k
     if (bound >= 65536 || tu.ext_force_wide) return -2;
     return q <= 32 ? (q - 1) / 4 : kExtShortClasses + *kidx;
 }
+
+// ext_class_of for a whole batch: everything but the h0 tests depends on the query length only
+// (the band clamp takes two divisions), so it is computed once per length and looked up.
+struct ExtClassifier {
+    const ExtScoring &sc;
+    const Tuning &tu;
+    std::vector<int16_t> cls_of_q, kidx_of_q;  // class with a narrow-exact bound; -1 plain; -3 unset
+    ExtClassifier(const ExtScoring &s, const Tuning &t) : sc(s), tu(t), cls_of_q(8192, -3), kidx_of_q(8192, -1) {}
+    int classify(const ExtJob &j, int *kidx) {
+        const int q = (int)j.qlen;
+        if (q >= 8192) return ext_class_of(j, sc, tu, kidx);
+        if (cls_of_q[q] == -3) {
+            ExtJob probe = j;
+            probe.h0 = 1;  // a bound below 65536 for every q < 8192 at scores below 8
+            int ki = -1;
+            int c = (int64_t)1 + (int64_t)q * std::max(sc.max_sc, 0) < 65536 ? ext_class_of(probe, sc, tu, &ki)
+                                                                            : -4;
+            if (c == -4) {  // large scores: no memo for this length
+                return ext_class_of(j, sc, tu, kidx);
+            }
+            cls_of_q[q] = (int16_t)c;
+            kidx_of_q[q] = (int16_t)ki;
+        }
+        *kidx = kidx_of_q[q];
+        const int64_t bound = (int64_t)j.h0 + (int64_t)q * std::max(sc.max_sc, 0);
+        const int c = cls_of_q[q];
+        if (bound >= (1 << 18) || c == -1) return -1;
+        if (bound >= 65536 || tu.ext_force_wide) return -2;
+        return c;
+    }
+};
 
 // Candidate mappings of a class, the default first: lanes per job (thread or group kernel), the
 // group kernel's columns per lane (minimal K or the next), and the stored score width.
@@ -290,24 +354,17 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const ExtScoring &sc = p->esc;
     const bool retry = p->retry_round > 0;
     const Tuning &tu = b->tu;
-    std::vector<Item> bins;
-    auto find = [&](int kind, int param, bool narrow) {
-        for (size_t k = 0; k < bins.size(); ++k)
-            if (bins[k].kind == kind && bins[k].param == param && bins[k].narrow == narrow) return (int)k;
-        Item it;
-        it.kind = kind;
-        it.param = param;
-        it.narrow = narrow;
-        bins.push_back(it);
-        return (int)bins.size() - 1;
-    };
+    BinSet bs;
+    std::vector<Item> &bins = bs.bins;
+    auto find = [&](int kind, int param, bool narrow) { return bs.find(kind, param, narrow); };
     // classify, then tune the classes this batch has enough jobs of and the table lacks
     const int n_classes = kExtShortClasses + kNumExtKs;
     std::vector<int8_t> cls(n), kidx(n);
     std::vector<std::vector<uint32_t>> members(n_classes);
+    ExtClassifier classifier(sc, tu);
     for (size_t k = 0; k < n; ++k) {
         int ki = -1;
-        cls[k] = (int8_t)ext_class_of(b->hext[k], sc, tu, &ki);
+        cls[k] = (int8_t)classifier.classify(b->hext[k], &ki);
         kidx[k] = (int8_t)ki;
         if (cls[k] >= 0) members[cls[k]].push_back((uint32_t)k);
     }
@@ -513,18 +570,10 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
     const KswScoring &sc = b->ksc;
     const Tuning &tu = b->tu;
-    std::vector<Item> bins;
-    auto find = [&](int kind, int param, bool packed) {
-        for (size_t k = 0; k < bins.size(); ++k)
-            if (bins[k].kind == kind && bins[k].param == param && bins[k].narrow == packed) return (int)k;
-        Item it;
-        it.kind = kind;
-        it.param = param;
-        it.narrow = packed;
-        it.stage = kind == kKswFwd ? 0 : 1;  // fast bins first in the order array
-        bins.push_back(it);
-        return (int)bins.size() - 1;
-    };
+    BinSet bs;
+    std::vector<Item> &bins = bs.bins;
+    // fast bins first in the order array
+    auto find = [&](int kind, int param, bool packed) { return bs.find(kind, param, packed, kind == kKswFwd ? 0 : 1); };
     // class = the minimal K (index K/2-1), +8 for jobs whose scores may not fit 16 bits; tune the
     // classes with enough jobs
     const bool pk_scoring = ksw_pk_scoring_ok(sc);
@@ -809,6 +858,15 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
         std::stable_sort(ord.begin(), ord.end(),
                          [&](size_t x, size_t y) { return p->items[x].work > p->items[y].work; });
         double load[kAuxStreams + 1] = {};
+        const bool timed = b->tu.item_timing;
+        if (timed && p->ev0.size() != p->items.size()) {
+            p->ev0.assign(p->items.size(), nullptr);
+            p->ev1.assign(p->items.size(), nullptr);
+            for (size_t k = 0; k < p->items.size() && e == hipSuccess; ++k) {
+                e = hipEventCreate(&p->ev0[k]);
+                if (e == hipSuccess) e = hipEventCreate(&p->ev1[k]);
+            }
+        }
         for (size_t r : ord) {
             if (e != hipSuccess) break;
             int slot = 0;
@@ -816,7 +874,9 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
                 if (load[k] < load[slot]) slot = k;
             load[slot] += std::max(p->items[r].work, 1.0);
             hipStream_t st = (!par || slot == 0) ? ss.main : ss.aux[slot - 1];
-            e = launch_item(b, p, p->items[r], st);
+            if (timed) e = hipEventRecord(p->ev0[r], st);
+            if (e == hipSuccess) e = launch_item(b, p, p->items[r], st);
+            if (timed && e == hipSuccess) e = hipEventRecord(p->ev1[r], st);
             ++nl;
         }
         if (par) {
@@ -869,6 +929,10 @@ hipError_t emit_all(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
 }
 
 void free_plan(bwg_ctx *c, Plan *p) {
+    for (size_t k = 0; k < p->ev0.size(); ++k) {
+        if (p->ev0[k]) (void)hipEventDestroy(p->ev0[k]);
+        if (p->ev1[k]) (void)hipEventDestroy(p->ev1[k]);
+    }
     c->pool.free(p->d_order);
     c->pool.free(p->d_heads);
     c->pool.free(p->d_jobs);
@@ -891,7 +955,7 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     hipError_t e = hipSuccess;
     // Capture the launch sequence once the batch has run eagerly; the eager run is also the
     // reference every later replay is compared with by the tests.
-    if (b->tu.use_graphs && !b->graph && b->batched_runs >= 1) {
+    if (b->tu.use_graphs && !b->tu.item_timing && !b->graph && b->batched_runs >= 1) {
         hipGraph_t g = nullptr;
         e = hipStreamBeginCapture(ss.main, hipStreamCaptureModeThreadLocal);
         int64_t nl = 0;
@@ -904,7 +968,7 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     }
     e = hipEventRecord(b->ev_start, ss.main);
     if (e == hipSuccess) {
-        if (b->graph) {
+        if (b->graph && !b->tu.item_timing) {
             e = hipGraphLaunch(b->graph, ss.main);
             *launches = b->graph_launches;
         } else {
@@ -929,13 +993,14 @@ size_t plan_device_bytes(bwg_ctx *c, int kind, size_t n, const uint32_t *q_len, 
     if (kind == 0) {
         size_t n_plain = 0, plain_cols = 0;
         std::vector<uint32_t> per_class(kExtShortClasses + kNumExtKs, 0);
+        ExtClassifier classifier(*esc, tu);
         for (size_t k = 0; k < n; ++k) {
             ExtJob j{};
             j.qlen = q_len[k];
             j.tlen = t_len[k];
             j.h0 = h0[k];
             int ki = -1;
-            const int cl = ext_class_of(j, *esc, tu, &ki);
+            const int cl = classifier.classify(j, &ki);
             if (cl == -1) {
                 ++n_plain;
                 plain_cols += (size_t)q_len[k] + 1;
@@ -1014,6 +1079,49 @@ int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
             declined += cnt[(size_t)kCountStride * it.head];
         }
     *hits = eligible - declined;
+    return BWG_OK;
+}
+
+std::string batched_report(bwg_batch *b) {
+    std::string s;
+    std::vector<Plan *> plans;
+    if (b->plan) plans.push_back((Plan *)b->plan);
+    for (void *q : b->retry_plans) plans.push_back((Plan *)q);
+    char line[256], name[64];
+    int idx = 0;
+    for (size_t pi = 0; pi < plans.size(); ++pi) {
+        Plan *p = plans[pi];
+        for (size_t k = 0; k < p->items.size(); ++k, ++idx) {
+            const Item &it = p->items[k];
+            double us = -1.0;
+            if (k < p->ev0.size() && p->ev0[k] && p->ev1[k]) {
+                float ms = 0.f;
+                if (hipEventElapsedTime(&ms, p->ev0[k], p->ev1[k]) == hipSuccess) us = ms * 1e3;
+            }
+            const int width = (it.kind == kKswFwd || it.kind == kKswRev || it.kind == kGlbGroup ||
+                               it.kind == kExtGroup || it.kind == kExtThread)
+                                  ? (it.narrow ? 16 : 32)
+                                  : 32;
+            std::snprintf(line, sizeof line, "%d round=%d stage=%d %s jobs=%u grid=%d us=%.3f\n", idx,
+                          p->retry_round, it.stage, mapping_name(it.kind, it.param, width, name, sizeof name),
+                          it.count, it.grid, us);
+            s += line;
+        }
+    }
+    return s;
+}
+
+int batched_job_bins(bwg_batch *b, int32_t *bin) {
+    for (size_t k = 0; k < b->n; ++k) bin[k] = -1;
+    Plan *p = (Plan *)b->plan;
+    if (!p) return BWG_EINVAL;
+    for (size_t r = 0; r < p->items.size(); ++r) {
+        const Item &it = p->items[r];
+        const bool is_bin = it.kind == kExtGroup || it.kind == kExtThread || it.kind == kExtPlain ||
+                            it.kind == kKswFwd || it.kind == kKswPlain || it.kind == kGlbGroup;
+        if (!is_bin) continue;
+        for (uint32_t q = 0; q < it.count; ++q) bin[p->order[it.offset + q]] = (int32_t)r;
+    }
     return BWG_OK;
 }
 

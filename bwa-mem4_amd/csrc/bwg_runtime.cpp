@@ -87,13 +87,20 @@ void DevicePool::release_all() {
 // ---------------------------------------------------------------------------------------------
 // StreamSet
 // ---------------------------------------------------------------------------------------------
-int StreamSet::create() {
-    if (hipStreamCreateWithFlags(&main, hipStreamNonBlocking) != hipSuccess) return BWG_EIO;
+int StreamSet::create(int rotation) {
+    hipStream_t s[kAuxStreams + 1] = {};
+    for (int k = 0; k <= kAuxStreams; ++k)
+        if (hipStreamCreateWithFlags(&s[k], hipStreamNonBlocking) != hipSuccess) {
+            for (int r = 0; r < k; ++r) (void)hipStreamDestroy(s[r]);
+            return BWG_EIO;
+        }
+    const int m = ((rotation % (kAuxStreams + 1)) + kAuxStreams + 1) % (kAuxStreams + 1);
+    main = s[m];
+    for (int k = 0, a = 0; k <= kAuxStreams; ++k)
+        if (k != m) aux[a++] = s[k];
     if (hipEventCreateWithFlags(&fork, hipEventDisableTiming) != hipSuccess) return BWG_EIO;
-    for (int k = 0; k < kAuxStreams; ++k) {
-        if (hipStreamCreateWithFlags(&aux[k], hipStreamNonBlocking) != hipSuccess) return BWG_EIO;
+    for (int k = 0; k < kAuxStreams; ++k)
         if (hipEventCreateWithFlags(&join[k], hipEventDisableTiming) != hipSuccess) return BWG_EIO;
-    }
     return BWG_OK;
 }
 
@@ -114,7 +121,7 @@ void StreamSet::destroy() {
     fork = nullptr;
 }
 
-int ExecLane::create() { return ss.create(); }
+int ExecLane::create(int index) { return ss.create(index + 1); }  // the context's own set took rotation 0
 
 void ExecLane::destroy() {
     ss.destroy();
@@ -134,7 +141,7 @@ ExecLane *acquire_lane(bwg_ctx *c) {
         if ((int)c->lanes.size() < c->max_lanes) {
             std::unique_ptr<ExecLane> l(new (std::nothrow) ExecLane());
             if (!l) return nullptr;
-            if (l->create() != BWG_OK) {
+            if (l->create((int)c->lanes.size()) != BWG_OK) {
                 l->destroy();
                 return nullptr;
             }

@@ -9,6 +9,7 @@
 #include <map>
 #include <memory>
 #include <mutex>
+#include <string>
 #include <vector>
 
 #include "bwg_batched_kernels.h"
@@ -92,13 +93,17 @@ class DevicePool {
 };
 
 // A main stream plus helper streams used to run independent kernels of one batch concurrently
-// (fork after a main-stream event, join back before the batch's end event).
+// (fork after a main-stream event, join back before the batch's end event).  The runtime maps
+// streams onto a few hardware queues, and a set's streams created in a row take consecutive
+// queues; so that the main streams of several sets (execution lanes, server slots) do not all
+// share one queue, and so run one after another, set k takes the (k mod 4)-th of its four streams
+// as its main one.
 struct StreamSet {
     hipStream_t main = nullptr;
     hipStream_t aux[kAuxStreams] = {nullptr, nullptr, nullptr};
     hipEvent_t fork = nullptr;
     hipEvent_t join[kAuxStreams] = {nullptr, nullptr, nullptr};
-    int create();
+    int create(int rotation = 0);
     void destroy();
 };
 
@@ -117,6 +122,8 @@ struct Tuning {
     bool concurrent_bins = true;   // run the bins of one batch on the stream set concurrently
     bool use_graphs = true;        // replay a batch's launch sequence as a captured hipGraph
     bool tune_new = true;          // a batch may run the autotuner (false: it only reads the table)
+    bool item_timing = false;      // batched runs time each launch with events (no graph replay)
+    bool coalesce = true;          // concurrent small synchronous calls are merged (bwg_api.cpp)
     int form = BWG_FORM_BATCHED;   // BWG_OPT_FORM
 };
 
@@ -127,7 +134,7 @@ struct ExecLane {
     StreamSet ss;
     void *pin_out = nullptr;
     size_t pin_out_cap = 0;
-    int create();
+    int create(int index);
     void destroy();
 };
 
@@ -162,6 +169,10 @@ struct bwg_ctx {
     int max_lanes = 4;
     // timing events of the allocation-free entry point, created with the context
     hipEvent_t na_ev[2] = {nullptr, nullptr};
+    // synchronous calls in progress, and the server that merges concurrent small ones
+    std::atomic<int> sync_active{0};
+    std::mutex coalesce_mu;
+    bwg_server *coalescer = nullptr;
     // two-strand reference for mate rescue (bwg_set_reference): 2*l_pac bytes and the contigs
     uint8_t *d_ref2 = nullptr;
     int64_t l_pac = 0;
@@ -307,6 +318,10 @@ const void *glb_plan_jobs(const bwg_batch *b);
 void release_batched(bwg_ctx *c, bwg_batch *b);
 // Jobs of the last batched run that the ungapped fast path resolved (0 when it did not run).
 int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits);
+// The launches of a batch's last batched run, one line each (bwg_batch_report), and each job's
+// launch index (bwg_batch_job_bins).
+std::string batched_report(bwg_batch *b);
+int batched_job_bins(bwg_batch *b, int32_t *bin);
 
 }  // namespace bwg
 

@@ -297,8 +297,9 @@ int bwg_server_create(bwg_ctx *ctx, uint64_t hbm_budget_bytes, int n_slots, bwg_
     s->ctx = ctx;
     s->budget = hbm_budget_bytes;
     s->slots.resize((size_t)n_slots);
-    for (auto &sl : s->slots) {
-        if (sl.ss.create() != BWG_OK || hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
+    for (size_t k = 0; k < s->slots.size(); ++k) {
+        Slot &sl = s->slots[k];
+        if (sl.ss.create((int)k + 1) != BWG_OK || hipEventCreateWithFlags(&sl.done, hipEventDisableTiming) != hipSuccess) {
             for (auto &x : s->slots) {
                 x.ss.destroy();
                 if (x.done) (void)hipEventDestroy(x.done);

@@ -34,7 +34,19 @@ const char *mapping_name(int kind, int param, int width, char *buf, size_t len) 
     case kExtThread: std::snprintf(buf, len, "ext_thread<%d,i%d>", param, width); break;
     case kExtGroup: std::snprintf(buf, len, "ext_group<%d,i%d>", param, width); break;
     case kKswFwd: std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16>" : "ksw_group<%d,i32>", param); break;
-    default: std::snprintf(buf, len, "plain"); break;
+    case kKswRev:
+        std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16,reverse>" : "ksw_group<%d,i32,reverse>", param);
+        break;
+    case kKswTracker: std::snprintf(buf, len, "ksw_tracker"); break;
+    case kExtUngapped: std::snprintf(buf, len, "ext_ungapped"); break;
+    case kExtRetryFilter: std::snprintf(buf, len, "ext_retry_filter"); break;
+    case kKswMateRc: std::snprintf(buf, len, "mate_revcomp"); break;
+    case kKswWindows: std::snprintf(buf, len, "matesw_windows"); break;
+    case kGlbGroup: std::snprintf(buf, len, width == 16 ? "glb_group<%d,packed>" : "glb_group<%d>", param); break;
+    case kGlbTrace: std::snprintf(buf, len, "glb_traceback"); break;
+    case kExtPlain: std::snprintf(buf, len, "ext_plain"); break;
+    case kKswPlain: std::snprintf(buf, len, "ksw_plain"); break;
+    default: std::snprintf(buf, len, "?"); break;
     }
     return buf;
 }

@@ -79,7 +79,11 @@ const char *bwg_strerror(int code);
  * bwg_extend_side_batch, bwg_global_batch, bwg_matesw_batch) may be called concurrently from any
  * number of host threads on one context: each call runs on one of the context's execution lanes
  * (BWG_OPT_LANES, default 4; each its own set of HIP streams), so up to that many calls stage,
- * run and copy back at the same time and the rest wait for a lane.  The staged-batch calls
+ * run and copy back at the same time and the rest wait for a lane.  Extension and rescue calls
+ * of up to 65536 jobs that arrive while another synchronous call is running are instead merged
+ * with the other calls waiting at that moment into one device batch (BWG_OPT_COALESCE, default
+ * on; the context's own server, see bwg_server_*): one batch of many small calls costs the device
+ * about what one of them does, and every job's result is the same as in a call of its own.  The staged-batch calls
  * (bwg_batch_*) share the context's own streams and run one at a time.  bwg_set_option may be
  * called at any time; a batch plans with the options in force when it was staged (staged batches)
  * or called (synchronous entry points). */
@@ -142,6 +146,8 @@ enum {
     BWG_OPT_LANES = 11,           /* execution lanes of the synchronous entry points (1..16, default 4) */
     BWG_OPT_POOL_CACHE_MB = 12,   /* device blocks the context keeps cached for reuse, MiB (default 8192) */
     BWG_OPT_POOL_PEAK_RESET = 13, /* any value: restart BWG_STAT_POOL_PEAK at the bytes now in use */
+    BWG_OPT_ITEM_TIMING = 14,     /* batched: time every launch with HIP events (replays no graph; default 0) */
+    BWG_OPT_COALESCE = 15,        /* merge concurrent small synchronous calls into one device batch (default 1) */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
@@ -189,6 +195,14 @@ int bwg_batch_run(bwg_ctx *ctx, bwg_batch *b);
 /* Copy results (bwg_ext_result or bwg_ksw_result, n of them) to host memory. */
 int bwg_batch_results(bwg_ctx *ctx, bwg_batch *b, void *out);
 void bwg_batch_destroy(bwg_ctx *ctx, bwg_batch *b);
+/* Diagnostics, no counterpart in the reference.  bwg_batch_report: the launches of the batch's
+ * last batched run, one text line each ("<index> round=<band-retry round> stage=<s> <kernel> jobs=<n>
+ * grid=<workgroups> us=<device time, -1 unless BWG_OPT_ITEM_TIMING was on>"); writes at most len-1
+ * bytes plus a terminating 0 and returns the full length.  bwg_batch_job_bins: for each job, the
+ * index of the launch whose bin it was planned into (extension DP bins, including jobs the
+ * ungapped pre-pass then resolved; rescue forward bins; -1 for none). */
+int bwg_batch_report(bwg_ctx *ctx, bwg_batch *b, char *buf, size_t len);
+int bwg_batch_job_bins(bwg_ctx *ctx, bwg_batch *b, int32_t *bin);
 /* Extension jobs of the batch's last batched run that the ungapped-diagonal fast path resolved
  * without the banded DP (the reference's batched extension does the same, crates/bwa-neon/src/
  * batched.rs:716-798); 0 for rescue batches, the plain form, or with BWG_OPT_UNGAPPED off. */
