@@ -19,7 +19,8 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libbwg.so")
-WORKLOAD_LIB_PATH = os.path.join(HERE, "libbwg_workload.so")
+# BWG_WORKLOAD_SO selects another build of the host-only workload library (the sanitizer build)
+WORKLOAD_LIB_PATH = os.environ.get("BWG_WORKLOAD_SO") or os.path.join(HERE, "libbwg_workload.so")
 
 ExtendResult = namedtuple("ExtendResult", "score qle tle gtle gscore max_off")
 KswAlignResult = namedtuple("KswAlignResult", "score qb qe tb te score2 te2")

@@ -134,3 +134,46 @@ void orc_brute_enum_local(int32_t qlen, const uint8_t *query, int32_t tlen, cons
     *qe = bj;
     free(c.best);
 }
+
+/* Explicit enumeration of every alignment prefix from the origin under bwa's extension and
+ * global-alignment gap rules (crates/bwa-extend/src/sw.rs:364-414 and :597-636): a deletion (D,
+ * consumes a target base) or insertion (I, consumes a query base) run opens only from an aligned
+ * pair or from the origin, so a D run is never next to an I run; a run costs o + len*e.  Every cell
+ * visited, the origin's boundary cells (-1, j) and (i, -1) included, must lie in the band
+ * |i - j| <= w.  For each cell (i, j) of the tlen x qlen matrix, Mx/Ex/Fx receive the best score of
+ * a prefix path ending there with an aligned pair / a D / an I as its last step (the origin scores
+ * `origin`), or ORC_ENUM_NONE when no such path exists.  No recurrence is evaluated: every path is
+ * walked.  Tiny inputs only (the number of paths grows like the Delannoy numbers). */
+typedef struct {
+    int32_t qlen, tlen, w;
+    const uint8_t *q, *t;
+    const int8_t *mat;
+    int32_t o_del, e_del, o_ins, e_ins;
+    int32_t *M, *E, *F;
+} prefix_ctx;
+
+static int in_band(const prefix_ctx *c, int32_t i, int32_t j) {
+    const int32_t d = i - j;
+    return d <= c->w && -d <= c->w;
+}
+
+static void prefix_dfs(prefix_ctx *c, int32_t i, int32_t j, int32_t score, int last /*0 M/origin,1 D,2 I*/) {
+    if (i >= 0 && j >= 0) {
+        int32_t *slot = (last == 0 ? c->M : last == 1 ? c->E : c->F) + i * c->qlen + j;
+        if (*slot == ORC_ENUM_NONE || score > *slot) *slot = score;
+    }
+    if (i + 1 < c->tlen && j + 1 < c->qlen && in_band(c, i + 1, j + 1))
+        prefix_dfs(c, i + 1, j + 1, score + c->mat[c->t[i + 1] * 5 + c->q[j + 1]], 0);
+    if (last != 2 && i + 1 < c->tlen && in_band(c, i + 1, j))
+        prefix_dfs(c, i + 1, j, score - (last == 1 ? c->e_del : c->o_del + c->e_del), 1);
+    if (last != 1 && j + 1 < c->qlen && in_band(c, i, j + 1))
+        prefix_dfs(c, i, j + 1, score - (last == 2 ? c->e_ins : c->o_ins + c->e_ins), 2);
+}
+
+void orc_enum_prefix(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target, const int8_t *mat,
+                     int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t w, int32_t origin,
+                     int32_t *Mx, int32_t *Ex, int32_t *Fx) {
+    prefix_ctx c = {qlen, tlen, w, query, target, mat, o_del, e_del, o_ins, e_ins, Mx, Ex, Fx};
+    for (int32_t k = 0; k < qlen * tlen; ++k) Mx[k] = Ex[k] = Fx[k] = ORC_ENUM_NONE;
+    prefix_dfs(&c, -1, -1, origin, 0);
+}

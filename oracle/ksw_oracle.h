@@ -114,6 +114,10 @@ void orc_lockstep_local_fwd(size_t n, const uint8_t *const *queries, const int32
                             int32_t e_ins, int32_t max_sc, int32_t lanes, orc_fwd_result *out);
 
 /* Brute-force checkers (ksw_brute.c). */
+#define ORC_ENUM_NONE (-(1 << 30))
+void orc_enum_prefix(int32_t qlen, const uint8_t *query, int32_t tlen, const uint8_t *target, const int8_t *mat,
+                     int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t w, int32_t origin,
+                     int32_t *Mx, int32_t *Ex, int32_t *Fx);
 /* Unbanded extension recurrence without band, z-drop or zero-row break: best score only
  * (crates/bwa-extend/src/sw.rs:1255-1305). */
 int32_t orc_brute_extend_unbanded(int32_t qlen, const uint8_t *query, int32_t tlen,

@@ -9,7 +9,8 @@ import os
 import numpy as np
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
-ORACLE_SO = os.path.join(ROOT, "oracle", "liboracle_ksw.so")
+# BWG_ORACLE_SO selects another build of the same sources (the sanitizer build, oracle/Makefile)
+ORACLE_SO = os.environ.get("BWG_ORACLE_SO") or os.path.join(ROOT, "oracle", "liboracle_ksw.so")
 
 
 class ExtResult(C.Structure):
@@ -56,6 +57,8 @@ def lib():
         L.orc_ksw_extend2.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P]
         L.orc_ksw_local_fwd.restype = FwdResult
         L.orc_ksw_local_fwd.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _I, _P]
+        L.orc_enum_prefix.restype = None
+        L.orc_enum_prefix.argtypes = [_I, _P, _I, _P, _P, _I, _I, _I, _I, _I, _I, _P, _P, _P]
         L.orc_ksw_align2.restype = KswResult
         L.orc_ksw_align2.argtypes = [_I, _P, _I, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I, _P]
         L.orc_batched_align2_job.restype = KswResult
@@ -258,6 +261,22 @@ def global_batch(batch, w, mat, o_del, e_del, o_ins, e_ins, nthreads=8):
                            e_ins, score.ctypes.data, off.ctypes.data, buf.ctypes.data, ncig.ctypes.data, nthreads)
     parts = [buf[int(off[k]):int(off[k]) + int(ncig[k])] for k in range(n)]
     return score, ncig, (np.concatenate(parts) if parts else np.zeros(0, np.uint32))
+
+
+ENUM_NONE = -(1 << 30)
+
+
+def enum_prefix(q, t, mat, o_del, e_del, o_ins, e_ins, w, origin):
+    """Best score of every alignment prefix from the origin ending at each cell with an aligned
+    pair / a deletion / an insertion (orc_enum_prefix: explicit path enumeration, no recurrence).
+    Returns three (tlen, qlen) int arrays with ENUM_NONE where no such prefix exists."""
+    qa, qp = _buf(q)
+    ta, tp = _buf(t)
+    ma, mp = _mat(mat)
+    out = [np.zeros((len(ta), len(qa)), dtype=np.int32) for _ in range(3)]
+    lib().orc_enum_prefix(len(qa), qp, len(ta), tp, mp, o_del, e_del, o_ins, e_ins, w, origin,
+                          out[0].ctypes.data, out[1].ctypes.data, out[2].ctypes.data)
+    return out
 
 
 def brute_global(q, t, mat, o_del, e_del, o_ins, e_ins):

@@ -103,8 +103,9 @@ def test_rescue_workload_sample(gpu_ctx, form):
     _say("rescue_workload form=%d" % form)
     batch = bwg.generate(bwg.WORKLOAD_RESCUE, 4000, seed=42)
     mat = bwg.default_mat()
-    got = _ksw(gpu_ctx, form, batch, mat, 6, 1, 6, 1, 19, 1)
-    want = ob.align2_batch(batch, mat, 6, 1, 6, 1, 19, 1)
+    # the rescue workload's own minsc (min_seed_len 20 * a)
+    got = _ksw(gpu_ctx, form, batch, mat, 6, 1, 6, 1, 20, 1)
+    want = ob.align2_batch(batch, mat, 6, 1, 6, 1, 20, 1)
     bad = np.nonzero(got != want)[0]
     assert bad.size == 0, "first mismatch job %d: got %s want %s" % (bad[0], got[bad[0]], want[bad[0]])
     assert np.mean(got["qb"] >= 0) > 0.5  # the reverse pass ran for the placed mates
