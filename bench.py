@@ -73,6 +73,21 @@ def host_info():
     return {"cpu_model": model, "cpus_online": online, "cpus_available": avail, "threads": min(16, avail)}
 
 
+SEGMENTS = []  # labels of the trace segments the marker copies delimit, in order
+
+
+def mark(ctx, label, **info):
+    """A 1 MiB device copy (hbm_copy_kernel) as a marker in a kernel trace: the dispatches between
+    two markers belong to the segment `label` names (--segments-out, --summarize-trace); `info`
+    goes into that segment's entry."""
+    ctx.hbm_copy(1 << 20, reps=1)
+    SEGMENTS.append(dict(label=label, **info))
+
+
+def segment(label):
+    return next(s for s in SEGMENTS if s["label"] == label)
+
+
 def time_steps(ctx, staged, steps, warmup):
     for _ in range(warmup):
         staged.run()
@@ -121,6 +136,7 @@ def ext_bin_table(rep, bins, job_cells, dp_mask):
             dsel = sel & dp_mask
             cells = int(job_cells[dsel].sum())
             row.update(dp_jobs=int(dsel.sum()), dp_cells=cells)
+            row.update(ops=cells * K_EXT, peak_tops=VALU_PEAK_TOPS)
             if r["us"] > 0:
                 ops = cells * K_EXT / (r["us"] * 1e-6) / 1e12
                 row["frac"] = round(ops / VALU_PEAK_TOPS, 4)
@@ -152,12 +168,14 @@ def resc_bin_table(rep, bins, batch, res, max_sc=1):
             fwd_of[r["kernel"].split(">")[0]] = sel
             row["cells"] = int((q[sel] * t[sel]).sum())
             row["padded_cells"] = int((qpad[sel] * t[sel]).sum())
+            row.update(ops=row["padded_cells"] * K_RESC, peak_tops=peak)
             if r["us"] > 0:
                 row["frac"] = round(row["padded_cells"] * K_RESC / (r["us"] * 1e-6) / 1e12 / peak, 4)
                 row["peak_tops"] = peak
         elif "reverse" in r["kernel"]:
             sel = fwd_of.get(r["kernel"].split(",reverse")[0].split(">")[0], np.zeros(len(q), bool)) & ran_rev
             row["cells"] = int((rpad[sel] * rrows[sel]).sum())
+            row.update(ops=row["cells"] * K_RESC, peak_tops=peak)
             if r["us"] > 0:
                 row["frac"] = round(row["cells"] * K_RESC / (r["us"] * 1e-6) / 1e12 / peak, 4)
                 row["peak_tops"] = peak
@@ -179,6 +197,268 @@ def traffic_from_profile(jobs):
     if t.get("jobs") != jobs:
         return None
     return t.get("bytes_per_batch")
+
+
+def _short_kernel(name):
+    """A trace's demangled kernel name without return type, namespaces and argument list."""
+    name = name.replace("(anonymous namespace)::", "").split("(", 1)[0]
+    if name.startswith("void "):
+        name = name[5:]
+    head, sep, tmpl = name.partition("<")
+    return head.rsplit("::", 1)[-1] + sep + tmpl
+
+
+def _trace_name(bin_name):
+    """The trace name of the kernel a bench bin label (the library's mapping name) launches."""
+    import re
+    m = re.match(r"(ext_group|ext_thread|ksw_pk|ksw_group)<(\d+),i(\d+)(,reverse)?>$", bin_name)
+    if not m:
+        return bin_name + "_kernel"
+    fam, k, width, rev = m.group(1), int(m.group(2)), int(m.group(3)), bool(m.group(4))
+    if fam in ("ext_group", "ext_thread"):
+        return "%s_kernel<%d, %s>" % (fam, k, "true" if width == 16 else "false")
+    return "%s_kernel<%d, %s>" % (fam, k // 2 if fam == "ksw_pk" else k, "true" if rev else "false")
+
+
+def _split_at_markers(rows, what, with_markers=False):
+    """Dispatch rows (kernel name at [2], grid at [3], in dispatch order) split at the marker copies:
+    the rows before the first marker, then those after each marker (and, with_markers, each
+    marker's own rows)."""
+    copies = [r for r in rows if r[2] == "hbm_copy_kernel"]
+    if not copies:
+        raise SystemExit("no marker copies in %s" % what)
+    grid = copies[-1][3]  # the run ends with a marker: its grid is the markers' grid
+    # a marker is the fill of the copy's source buffer and then the copies (a warm-up and the timed one)
+    is_copy = [r[2] == "hbm_copy_kernel" and r[3] == grid for r in rows]
+    starts = {i for i in range(len(rows) - 1) if rows[i][2].startswith("__amd_rocclr_fillBuffer") and is_copy[i + 1]}
+    in_marker = starts | {i for i, c in enumerate(is_copy) if c}
+    pieces, cur, marks = [], [], []
+    for i, r in enumerate(rows):
+        if i in starts:
+            pieces.append(cur)
+            cur = []
+            marks.append([])
+        if i not in in_marker:
+            cur.append(r)
+        elif marks:
+            marks[-1].append(r)
+    pieces.append(cur)
+    return (pieces, marks) if with_markers else pieces
+
+
+def _launch_sequences(piece):
+    """A segment's dispatches split into launch sequences: each starts with the zero_u32_kernel
+    that clears the queue heads."""
+    runs, cur = [], None
+    for r in piece:
+        if r[2].startswith("zero_u32_kernel"):
+            cur = []
+            runs.append(cur)
+        if cur is not None:
+            cur.append(r)
+    return runs
+
+
+def summarize_trace(trace_csv, segments_json):
+    """Markdown summary of a rocprofv3 kernel trace (--kernel-trace --output-format csv) of a bench
+    run made with --segments-out: the dispatches split at the marker copies into the labelled
+    segments, per-kernel device times per segment, the device span of every timed step (the first
+    dispatch of a step is its counter-clearing zero_u32_kernel) with the roofline fraction that span
+    gives, and the serial per-bin device times with each bin's fraction recomputed from the trace
+    beside the figure bench.py took with HIP events."""
+    import csv
+    with open(segments_json) as f:
+        meta = json.load(f)
+    rows = []
+    with open(trace_csv) as f:
+        for r in csv.DictReader(f):
+            rows.append((int(r["Start_Timestamp"]), int(r["End_Timestamp"]), _short_kernel(r["Kernel_Name"]),
+                         int(r["Grid_Size_X"])))
+    rows.sort()
+    pieces = _split_at_markers(rows, trace_csv)
+    segs = meta["segments"]
+    if len(pieces) != len(segs) + 1:
+        raise SystemExit("%d marker runs in the trace, %d segments in %s" % (len(pieces) - 1, len(segs),
+                                                                              segments_json))
+    out = ["# Kernel trace by bench segment", "",
+           "%d dispatches; %d before the first marker (staging, the first run that plans, tunes and "
+           "captures, the CPU-side set-up)." % (len(rows), len(pieces[0])), ""]
+
+    for s, piece in zip(segs, pieces[1:]):
+        label = s["label"]
+        out.append("## %s" % label)
+        out.append("")
+        if not piece:
+            out += ["(no dispatches)", ""]
+            continue
+        by = {}
+        for r in piece:
+            by.setdefault(r[2], []).append((r[1] - r[0]) / 1e3)
+        out.append("| kernel | calls | total us | avg us | min us | max us |")
+        out.append("|---|---:|---:|---:|---:|---:|")
+        for k, v in sorted(by.items(), key=lambda kv: -sum(kv[1])):
+            out.append("| `%s` | %d | %.1f | %.2f | %.2f | %.2f |" % (k, len(v), sum(v), np.mean(v), min(v), max(v)))
+        out.append("")
+        runs = _launch_sequences(piece)
+        if label.endswith("_timed") and runs:
+            w = s.get("warmup", 0)
+            spans = [(max(r[1] for r in run) - run[0][0]) / 1e3 for run in runs[w:]]
+            busy = [sum(r[1] - r[0] for r in run) / 1e3 for run in runs[w:]]
+            out.append("%d launch sequences (%d warm-up, %d timed). Timed steps: device span %.1f us mean "
+                       "(min %.1f, max %.1f); summed kernel time %.1f us mean (the bins overlap on "
+                       "several streams)." % (len(runs), w, len(spans), np.mean(spans), min(spans), max(spans),
+                                              np.mean(busy)))
+            rl = s.get("roofline")
+            if rl:
+                ach = rl["ops"] / (np.mean(spans) * 1e-6) / 1e12
+                out.append("")
+                out.append("Roofline from the trace: %.4g ops per step / %.1f us = %.3f Tops/s = %.4f of "
+                           "%.1f Tops/s (%s); bench.py's HIP-event figure: %.4f." %
+                           (rl["ops"], np.mean(spans), ach, ach / rl["peak"], rl["peak"], rl["what"],
+                            rl["bench_frac"]))
+            out.append("")
+        if label.endswith("_serial_bins") and runs:
+            table = meta.get("ext_bins" if label.startswith("ext") else "rescue_bins") or []
+            out.append("%d serial runs, one launch at a time on one stream; per bin the trace's mean "
+                       "duration over the runs, and the roofline fraction it gives:" % len(runs))
+            out.append("")
+            out.append("| bin | trace kernel | jobs | trace avg us | bench.py event us | ops | frac (trace) | "
+                       "frac (bench.py) |")
+            out.append("|---|---|---:|---:|---:|---:|---:|---:|")
+            for b in table:
+                tk = _trace_name(b["kernel"])
+                d = [(r[1] - r[0]) / 1e3 for run in runs for r in run if r[2] == tk]
+                if not d:
+                    out.append("| %s | `%s` | %d | (not in trace) | %.1f | | | |" % (b["kernel"], tk, b["jobs"], b["us"]))
+                    continue
+                t = float(np.mean(d))
+                ops, fr = b.get("ops"), ""
+                if ops and b.get("peak_tops"):
+                    fr = "%.4f" % (ops / (t * 1e-6) / 1e12 / b["peak_tops"])
+                out.append("| %s | `%s` | %d | %.1f | %.1f | %s | %s | %s |" %
+                           (b["kernel"], tk, b["jobs"], t, b["us"], "%.4g" % ops if ops else "", fr,
+                            b.get("frac", "")))
+            out.append("")
+    print("\n".join(out))
+
+
+def summarize_counters(paths, segments_json, traffic_out=""):
+    """Markdown summary of rocprofv3 --pmc runs (--output-format csv) of one bench command made with
+    --segments-out and --calib-copy-mb, one counter set per run: FETCH_SIZE and WRITE_SIZE (KiB per
+    dispatch) scaled by what they read for the calibration copy, which moves calib_copy_mb MiB each
+    way per launch; per-kernel means over the timed and serial segments (tuning and staging
+    excluded); and the extension batch's HBM bytes per launch sequence, written as JSON to
+    `traffic_out` when given."""
+    import csv
+    with open(segments_json) as f:
+        meta = json.load(f)
+    calib_mb = meta.get("calib_copy_mb") or 0
+    runs = []
+    for path in paths:
+        per = {}
+        with open(path) as f:
+            for r in csv.DictReader(f):
+                did = int(r["Dispatch_Id"])
+                grid = int(r.get("Grid_Size") or r.get("Grid_Size_X") or 0)
+                e = per.setdefault(did, [_short_kernel(r["Kernel_Name"]), grid, {}])
+                e[2][r["Counter_Name"]] = e[2].get(r["Counter_Name"], 0.0) + float(r["Counter_Value"])
+        runs.append([(did, did, v[0], v[1], v[2]) for did, v in sorted(per.items())])
+    # each run on its own (the autotuner may choose other kernels in another run): its segments, the
+    # calibration copies before the first marker, and per kernel the counters of every launch in the
+    # timed and serial segments
+    segs = meta["segments"]
+    labels = [s["label"] for s in segs]
+    by = {}
+    calib = {}
+    split = []
+    for path, rows in zip(paths, runs):
+        pieces, marks = _split_at_markers(rows, path, with_markers=True)
+        if len(pieces) != len(segs) + 1:
+            raise SystemExit("%d marker runs in %s, %d segments in %s" % (len(pieces) - 1, path, len(segs),
+                                                                          segments_json))
+        split.append(pieces)
+        if segs and segs[0]["label"] == "setup":  # the first copy is the calibration copy
+            for r in marks[0]:
+                if r[2] == "hbm_copy_kernel":
+                    for name, v in r[4].items():
+                        calib.setdefault(name, []).append(v)
+        for s, piece in zip(segs, pieces[1:]):
+            if s["label"].endswith(("_timed", "_serial_bins")):
+                for r in piece:
+                    for name, v in r[4].items():
+                        by.setdefault(r[2], {}).setdefault(name, []).append(v)
+    out = ["# Hardware counters by kernel", ""]
+    fscale = wscale = None
+    if calib_mb and ("FETCH_SIZE" in calib or "WRITE_SIZE" in calib):
+        kib = calib_mb * 1024.0
+        f, w = calib.get("FETCH_SIZE"), calib.get("WRITE_SIZE")
+        fscale = kib / np.mean(f) if f else None
+        wscale = kib / np.mean(w) if w else None
+        out.append("**Calibration.** `hbm_copy_kernel` moves %d MiB each way per launch; it reads FETCH_SIZE %s KiB "
+                   "and WRITE_SIZE %s KiB against %d KiB, so FETCH_SIZE is scaled by %s and WRITE_SIZE by %s "
+                   "below." % (calib_mb, "%.0f" % np.mean(f) if f else "-", "%.0f" % np.mean(w) if w else "-",
+                               kib, "%.4f" % fscale if fscale else "-", "%.4f" % wscale if wscale else "-"))
+        out.append("")
+
+    def mean(cs, key, scale=1.0):
+        v = cs.get(key)
+        return float(np.mean(v)) * scale if v else None
+
+    def fmt(v, f):
+        return f % v if v is not None else ""
+
+    out.append("Per launch, mean over the launches in the timed and serial segments.  fetch / write: scaled "
+               "HBM MiB.  VALUBusy and LDSBankConflict: rocprofv3's derived metrics (percent of GPU time "
+               "the VALUs issue / LDS is stalled by bank conflicts); busy (raw): 100 x SQ_ACTIVE_INST_VALU / "
+               "(256 CUs x GRBM_GUI_ACTIVE) from the raw counters as the CSV reports them.  VALU util: "
+               "SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), active lanes per VALU instruction "
+               "cycle.  LDS conflict / active: SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE.")
+    out.append("")
+    out.append("| kernel | launches | fetch MiB | write MiB | VALU insts | VALUBusy % | busy (raw) % | VALU util % | "
+               "LDSBankConflict % | LDS conflict / active % |")
+    out.append("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
+    for k, lst in sorted(by.items(), key=lambda kv: kv[0]):
+        n_launch = max(len(v) for v in lst.values())
+        fe = mean(lst, "FETCH_SIZE", (fscale or 0) / 1024.0) if fscale else None
+        wr = mean(lst, "WRITE_SIZE", (wscale or 0) / 1024.0) if wscale else None
+        ai, tc = mean(lst, "SQ_ACTIVE_INST_VALU"), mean(lst, "SQ_THREAD_CYCLES_VALU")
+        util = 100.0 * tc / (64.0 * ai) if ai and tc is not None else None
+        gui = mean(lst, "GRBM_GUI_ACTIVE")
+        raw = 100.0 * ai / (256.0 * gui) if ai is not None and gui else None
+        lb, la = mean(lst, "SQ_LDS_BANK_CONFLICT"), mean(lst, "SQ_LDS_IDX_ACTIVE")
+        lr = 100.0 * lb / la if la and lb is not None else None
+        out.append("| `%s` | %d | %s | %s | %s | %s | %s | %s | %s | %s |" % (
+            k, n_launch, fmt(fe, "%.2f"), fmt(wr, "%.2f"), fmt(mean(lst, "SQ_INSTS_VALU"), "%.3g"),
+            fmt(mean(lst, "VALUBusy"), "%.1f"), fmt(raw, "%.1f"), fmt(util, "%.1f"), fmt(mean(lst, "LDSBankConflict"), "%.1f"),
+            fmt(lr, "%.1f")))
+    out.append("")
+    traffic = None
+    if fscale and wscale and "ext_batched_timed" in labels:
+        seg_i = labels.index("ext_batched_timed")
+        warm = segs[seg_i].get("warmup", 0)
+
+        def per_batch(key, scale):
+            v = []
+            for pieces in split:
+                for q in _launch_sequences(pieces[1 + seg_i])[warm:]:
+                    if any(key in r[4] for r in q):
+                        v.append(sum(r[4].get(key, 0.0) for r in q) * scale * 1024.0)
+            return v
+
+        fb, wb = per_batch("FETCH_SIZE", fscale), per_batch("WRITE_SIZE", wscale)
+        if fb and wb:
+            traffic = {"jobs": meta.get("jobs"), "bytes_per_batch": float(np.mean(fb) + np.mean(wb)),
+                       "fetch_bytes": float(np.mean(fb)), "write_bytes": float(np.mean(wb)),
+                       "fetch_scale": fscale, "write_scale": wscale, "launch_sequences": [len(fb), len(wb)]}
+            out.append("Extension batch traffic (every kernel of the batch's launch sequence, mean of the timed "
+                       "runs): %.1f MiB fetched + %.1f MiB written = %.1f MiB per batch of %s jobs." %
+                       (np.mean(fb) / 2**20, np.mean(wb) / 2**20, (np.mean(fb) + np.mean(wb)) / 2**20,
+                        meta.get("jobs")))
+            out.append("")
+    print("\n".join(out))
+    if traffic_out and traffic:
+        with open(traffic_out, "w") as f:
+            json.dump(traffic, f, indent=1)
 
 
 def pct(v, f):
@@ -294,7 +574,24 @@ def main():
                     help="run a batch's bins one after another on one stream (for counter runs)")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="also run one device-to-device copy of this size (HBM counter calibration)")
+    ap.add_argument("--segments-out", default="",
+                    help="write the trace segment labels and the per-bin cell tables here (JSON)")
+    ap.add_argument("--summarize-trace", nargs=2, metavar=("KERNEL_TRACE_CSV", "SEGMENTS_JSON"),
+                    help="summarise a rocprofv3 kernel trace of a bench run (no GPU needed) and exit")
+    ap.add_argument("--summarize-counters", nargs="+", metavar="CSV",
+                    help="summarise rocprofv3 --pmc runs of one bench command (counter_collection CSVs, then "
+                         "its --segments-out JSON) and exit")
+    ap.add_argument("--traffic-out", default="",
+                    help="with --summarize-counters: write the extension batch's HBM bytes here as JSON")
     args = ap.parse_args()
+    if args.summarize_trace:
+        summarize_trace(*args.summarize_trace)
+        return
+    if args.summarize_counters:
+        if len(args.summarize_counters) < 2:
+            raise SystemExit("--summarize-counters takes counter CSVs and then a segments JSON")
+        summarize_counters(args.summarize_counters[:-1], args.summarize_counters[-1], args.traffic_out)
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: this path has no multi-GPU step (jobs shard across GPUs "
               "with no exchange); run with --gpus 1", file=sys.stderr)
@@ -340,6 +637,7 @@ def main():
     copy_gbs = None
     if args.calib_copy_mb:
         copy_gbs = ctx.hbm_copy(args.calib_copy_mb << 20, reps=1)
+        SEGMENTS.append(dict(label="setup"))  # the calibration copy's dispatches look like a marker's
         log("calibration copy: hbm_copy_kernel, %d MiB read + %d MiB written per launch, %.0f GB/s"
             % (args.calib_copy_mb, args.calib_copy_mb, copy_gbs))
     res, outs = {}, {}
@@ -352,17 +650,19 @@ def main():
             log("extension %s: staging" % fname)
             st = ctx.stage(ext, ext_p)
             st.run()  # first run builds the form's plan and buffers (and tunes)
-            if form == bwg.FORM_BATCHED:
-                ctx.hbm_copy(1 << 20, reps=1)  # trace marker: the timed steps follow this copy kernel
+            mark(ctx, "ext_%s_timed" % fname, warmup=args.warmup)
             per, kern = time_steps(ctx, st, args.steps if form == bwg.FORM_BATCHED else max(1, args.steps // 3),
                                    args.warmup)
+            mark(ctx, "ext_%s_after" % fname)
             outs["ext_" + fname] = st.results()
             hits = st.ungapped_hits()
             res["ext_" + fname] = dict(step_s=per, kernel_s=kern, ungapped_hits=hits)
             log("extension %s: %.3f ms/step (kernels %.3f ms), %d jobs resolved by the ungapped fast path"
                 % (fname, per * 1e3, kern * 1e3, hits))
             if form == bwg.FORM_BATCHED and "bins" in only and job_cells is not None:
+                mark(ctx, "ext_serial_bins")
                 rep, tot = serial_bin_times(ctx, st)
+                mark(ctx, "ext_serial_after")
                 ext_bins = ext_bin_table(rep, st.job_bins(), job_cells, ~hit)
                 res["ext_serial"] = dict(kernel_s=tot)
                 for row in ext_bins:
@@ -392,12 +692,16 @@ def main():
             log("rescue %s: staging" % fname)
             st = ctx.stage(resc, ksw_p)
             st.run()
+            mark(ctx, "resc_%s_timed" % fname, warmup=1)
             per, kern = time_steps(ctx, st, max(1, args.steps // 2) if form == bwg.FORM_BATCHED else 2, 1)
+            mark(ctx, "resc_%s_after" % fname)
             outs["resc_" + fname] = st.results()
             res["resc_" + fname] = dict(step_s=per, kernel_s=kern)
             log("rescue %s: %.3f ms/step (kernels %.3f ms)" % (fname, per * 1e3, kern * 1e3))
             if form == bwg.FORM_BATCHED and "bins" in only:
+                mark(ctx, "resc_serial_bins")
                 rep, tot = serial_bin_times(ctx, st)
+                mark(ctx, "resc_serial_after")
                 resc_bins, fwd_pad, rev_cells = resc_bin_table(rep, st.job_bins(), resc, outs["resc_batched"])
                 resc_exec = dict(forward_padded_cells=fwd_pad, reverse_cells=rev_cells, serial_kernel_s=tot)
                 for row in resc_bins:
@@ -607,6 +911,11 @@ def main():
                             "hbm": {"achieved": alg_bytes / h["kernel_s"] / 1e9, "peak": HBM_PEAK_GBPS,
                                     "unit": "GB/s", "frac": alg_bytes / h["kernel_s"] / 1e9 / HBM_PEAK_GBPS,
                                     "alg_bytes": alg_bytes}}
+        if "ext_batched_timed" in [s["label"] for s in SEGMENTS] and head == "ext_batched":
+            segment("ext_batched_timed")["roofline"] = dict(
+                ops=(dp_cells if h.get("ungapped_hits") else cells) * K_EXT, peak=VALU_PEAK_TOPS,
+                what="int32 VALU; 16 ops per DP cell of the jobs the ungapped pre-pass declines",
+                bench_frac=ops / VALU_PEAK_TOPS)
         if "ext_batched_dp_only" in res:
             line["batched_dp_only_roofline_frac"] = (cells * K_EXT / res["ext_batched_dp_only"]["kernel_s"] / 1e12
                                                      / VALU_PEAK_TOPS)
@@ -664,6 +973,10 @@ def main():
             line["rescue_roofline"]["executed_cells"] = resc_exec["forward_padded_cells"] + resc_exec["reverse_cells"]
         if resc_bins:
             line["rescue_bins_serial"] = resc_bins
+        if "resc_batched_timed" in [s["label"] for s in SEGMENTS]:
+            segment("resc_batched_timed")["roofline"] = dict(
+                ops=nominal * K_RESC, peak=peak, bench_frac=ach / peak,
+                what="%s VALU; 13 ops per nominal qlen x tlen cell" % line["rescue_roofline"]["arith"])
     if serving:
         at_cfg = min(serving, key=lambda r: abs(r["offered_jobs_per_s"] - 2e5))
         line["p50_chunk_latency_ms"] = at_cfg["p50_ms"]
@@ -674,6 +987,11 @@ def main():
     rep = ctx.tune_report()
     if rep:
         line["tuning"] = rep.splitlines()
+    if args.segments_out:
+        with open(args.segments_out, "w") as f:
+            json.dump({"segments": SEGMENTS, "ext_bins": ext_bins, "rescue_bins": resc_bins,
+                       "steps": args.steps, "jobs": args.jobs, "rescue_jobs": args.rescue_jobs,
+                       "calib_copy_mb": args.calib_copy_mb}, f, indent=1)
     if cpu:
         line["cpu_baseline"] = cpu
     if cpu_resc:

@@ -59,6 +59,7 @@ struct Plan {
     bool ungapped = false;
     int32_t ungapped_xmax = -1;
     uint32_t *d_corder = nullptr;
+    uint8_t *d_declined = nullptr;  // the pre-pass verdict per job
     int n_bins = 0;
     std::vector<uint32_t> meta;
     // host copies kept alive until the uploads complete
@@ -460,6 +461,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             p->ungapped_xmax = xmax;
             u.kind = kExtUngapped;
             u.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)c->num_cus * 8, ((int64_t)n + 15) / 16));
+            p->d_declined = (uint8_t *)batch_alloc(c, b, std::max<size_t>(1, n));
+            if (!p->d_declined) return BWG_ENOMEM;
         } else {
             u.kind = kExtRetryFilter;
         }
@@ -797,7 +800,7 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
     case kExtUngapped:
         return launch_ext_ungapped(st, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order, it.count,
                                    p->d_heads + seg_word(p), p->d_heads + seg_word(p) + p->n_bins, p->n_bins,
-                                   p->d_corder, p->d_heads + p->n_bins, p->esc, p->ungapped_xmax,
+                                   p->d_declined, p->d_corder, p->d_heads + p->n_bins, p->esc, p->ungapped_xmax,
                                    (bwg_ext_result *)b->d_out, it.grid);
     case kExtPlain:
         return launch_ext_plain_scatter(st, b->d_arena, b->d_ref, (const ExtJob *)p->d_plain_jobs,
@@ -940,6 +943,7 @@ void free_plan(bwg_ctx *c, Plan *p) {
     c->pool.free(p->d_plain_jobs);
     c->pool.free(p->d_plain_scratch);
     c->pool.free(p->d_corder);
+    c->pool.free(p->d_declined);
     c->pool.free(p->d_plain_slot);
     delete p;
 }
@@ -991,6 +995,7 @@ size_t plan_device_bytes(bwg_ctx *c, int kind, size_t n, const uint32_t *q_len, 
     // order array, the compacted queues of the ungapped pre-pass, queue heads and bin metadata
     size_t bytes = 2 * W(n * 4) + W(64 * (kCountStride + 3) * 4);
     if (kind == 0) {
+        bytes += W(n);  // the ungapped pre-pass's verdicts
         size_t n_plain = 0, plain_cols = 0;
         std::vector<uint32_t> per_class(kExtShortClasses + kNumExtKs, 0);
         ExtClassifier classifier(*esc, tu);

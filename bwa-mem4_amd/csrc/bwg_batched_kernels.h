@@ -22,14 +22,17 @@ int ext_thread_blocks_per_cu(int qcap, bool narrow);
 hipError_t launch_ext_thread(hipStream_t st, int qcap, bool narrow, const uint8_t *arena, const uint8_t *ref,
                              const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
                              uint32_t *head, const ExtScoring &sc, bwg_ext_result *out, int grid);
-// Ungapped-diagonal fast path over the order array's bins (uniform DNA matrices with a > 0, b > 0
-// and xmax >= 0 only; at most 255 bins): accepted jobs get their result, declined ones are appended
-// to corder[seg_off[b] + ccount[kCountStride*b]++], in slot order within each block's chunk.
+// Ungapped-diagonal fast path (uniform DNA matrices with a > 0, b > 0 and xmax >= 0 only; at most
+// 255 bins): a walk over all n_slots jobs in index order gives accepted jobs their result and every
+// job its verdict in declined[job] (n_slots bytes), then a compaction over the order array appends
+// the declined jobs of the bins without skip set to corder[seg_off[b] + ccount[kCountStride*b]++], in
+// slot order within each block's chunk.  The order array is a permutation of the n_slots jobs.
 constexpr int kCountStride = 32;  // compacted-queue counters one 128-byte line apart
 hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
-                               const uint32_t *skip, int n_bins, uint32_t *corder, uint32_t *ccount,
-                               const ExtScoring &sc, int32_t xmax, bwg_ext_result *out, int min_blocks);
+                               const uint32_t *skip, int n_bins, uint8_t *declined, uint32_t *corder,
+                               uint32_t *ccount, const ExtScoring &sc, int32_t xmax, bwg_ext_result *out,
+                               int min_blocks);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n);
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, uint32_t n, int32_t v);
 // Band-retry acceptance between two rounds of a one-sided extension batch (the reference's

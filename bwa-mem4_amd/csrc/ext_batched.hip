@@ -517,18 +517,22 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 // ---- ungapped-diagonal fast path -------------------------------------------------------------
 // The reference's batched extension first tries a closed-form walk along the main diagonal
 // (crates/bwa-neon/src/batched.rs:602-697, ungapped_hit) and runs the banded DP only for the jobs it
-// declines.  One job per 16-lane group; block b takes the contiguous order-array slots
-// [b*chunk, (b+1)*chunk).  The block first stages the chunk's order entries and then its job
-// descriptors in LDS (two gathers with every load in flight at once, where a group walking its jobs
-// would wait on each job's two dependent loads), and its 16 groups then walk the slots
-// independently (group q the slots q, q+16, ...).  Lane g reads columns
-// g, g+16, ... of the query and target (16 consecutive bytes per step, the first 128 columns loaded
-// up front), the group counts mismatches and N bases with ballots and keeps the mismatch columns
-// (< 128) as a 128-bit mask.  An accepted job's result is written directly; a declined one is
-// flagged in LDS and counted for its bin.  At the end of the chunk the declined jobs are appended,
-// in slot order, to their bins' compacted queues with one atomic per bin (the counters sit
-// kCountStride words apart, on separate cache lines).  Slot ranges [seg_off[b], seg_off[b+1])
-// belong to bin b; bins with skip[b] set (the plain bin) are left alone.
+// declines.  Two kernels:
+//
+//  * ext_ungapped_kernel walks the jobs in INDEX order, one job per 16-lane group, so that one
+//    wavefront's four groups read four consecutive 32-byte descriptors and, for a batch staged in
+//    job order, neighbouring stretches of the arena: every fetched line is used, where a walk in bin
+//    order touches a fresh line for every descriptor, query and target it reads.  Lane g reads
+//    columns g, g+16, ... of the query and of the target's diagonal only (16 consecutive bytes per
+//    step, the first 128 columns loaded up front); the group counts mismatches and N bases with
+//    ballots and keeps the mismatch columns (< 128) as a 128-bit mask.  An accepted job's result is
+//    written directly; every job's verdict goes to one byte of `declined`.
+//  * ext_compact_kernel walks the order array's contiguous slot chunks and appends the declined
+//    jobs, in slot order within each chunk, to their bins' compacted queues with one atomic per bin
+//    (the counters sit kCountStride words apart, on separate cache lines).  Slot ranges
+//    [seg_off[b], seg_off[b+1]) belong to bin b; bins with skip[b] set (the plain bin) are left
+//    alone: that bin runs every one of its jobs from the order array and writes each result again,
+//    the same value the walk wrote for the ones it accepted.
 constexpr int kUngappedChunkMax = 512;
 constexpr int kUngappedBinsMax = 256;
 
@@ -540,139 +544,151 @@ __device__ __forceinline__ int bin_of_slot(uint32_t slot, int bin, const uint32_
 
 __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__restrict__ arena,
                                                            const uint8_t *__restrict__ ref,
-                                                           const ExtJob *__restrict__ jobs,
-                                                           const uint32_t *__restrict__ order, uint32_t n_slots,
-                                                           uint32_t chunk, const uint32_t *__restrict__ seg_off,
-                                                           const uint32_t *__restrict__ skip, int n_bins,
-                                                           uint32_t *__restrict__ corder,
-                                                           uint32_t *__restrict__ ccount, int32_t a, int32_t b,
-                                                           int32_t xmax, int32_t zdrop,
-                                                           bwg_ext_result *__restrict__ out) {
-    __shared__ __align__(16) ExtJob s_jobs[kUngappedChunkMax];  // the chunk's descriptors, in slot order
-    __shared__ uint32_t s_idx[kUngappedChunkMax];    // the chunk's order entries
-    __shared__ uint8_t s_flag[kUngappedChunkMax];    // 1: the slot's job was declined
+                                                           const ExtJob *__restrict__ jobs, uint32_t n_jobs,
+                                                           int32_t a, int32_t b, int32_t xmax, int32_t zdrop,
+                                                           bwg_ext_result *__restrict__ out,
+                                                           uint8_t *__restrict__ declined) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane & 15;
+    const uint32_t stride = gridDim.x * 16u;
+    static_assert(sizeof(ExtJob) == 32, "a descriptor is two 16-byte words");
+    uint32_t idx = blockIdx.x * 16u + (threadIdx.x >> 4);
+    ExtJob next{};
+    if (idx < n_jobs) next = jobs[idx];
+    // (the steps compare n_jobs - idx with the stride: no 32-bit index wraps)
+    for (bool more = idx < n_jobs; more; more = n_jobs - idx > stride, idx += stride) {
+        // the next job's descriptor is in flight while this job's bytes load
+        const ExtJob jb = next;
+        if (n_jobs - idx > stride) next = jobs[idx + stride];
+        const uint8_t *q = arena + jb.q_off;
+        const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
+        const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
+        bool bail = !(qlen > 0 && h0 > 0 && tlen >= qlen);
+        int32_t nmis = 0;
+        uint32_t bits[4] = {0u, 0u, 0u, 0u};
+        uint32_t qv[8], tv[8];
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {  // independent loads of the first 128 columns
+            const int32_t c = 16 * st + g;
+            const bool in = !bail && c < qlen;
+            qv[st] = in ? q[c] : 0u;
+            tv[st] = in ? t[c] : 0u;
+        }
+#pragma unroll
+        for (int st = 0; st < 8; ++st) {
+            const int32_t c = 16 * st + g;
+            const bool in = !bail && c < qlen;
+            const uint64_t bn = __ballot(in && (qv[st] >= 4 || tv[st] >= 4));
+            const uint64_t bm = __ballot(in && qv[st] != tv[st]);
+            const uint32_t gn = (uint32_t)(bn >> (lane & ~15)) & 0xffffu;
+            const uint32_t gm = (uint32_t)(bm >> (lane & ~15)) & 0xffffu;
+            nmis += __popc(gm);
+            if (gn || nmis > xmax) bail = true;
+            else bits[st >> 1] |= gm << (16 * (st & 1));
+        }
+        // past column 128 only a mismatch-free walk can be accepted
+        for (int32_t c0 = 128; c0 < qlen && !bail; c0 += 16) {
+            const int32_t c = c0 + g;
+            const bool in = c < qlen;
+            const uint32_t qb = in ? q[c] : 0u, tb = in ? t[c] : 0u;
+            const uint64_t bx = __ballot(in && (qb >= 4 || tb >= 4 || qb != tb));
+            if ((uint32_t)(bx >> (lane & ~15)) & 0xffffu) bail = true;
+        }
+        bwg_ext_result r;
+        if (!bail && nmis == 0) {
+            // a mismatch-free walk only climbs: local and to-end alignments both take the query
+            r.score = h0 + qlen * a;
+            r.qle = r.tle = r.gtle = qlen;
+            r.gscore = r.score;
+            r.max_off = 0;
+        } else if (!bail) {
+            // the walk in runs between mismatches: a run of matches only climbs, so its last
+            // column is its best and z-drop can only fire at a mismatch; a dead diagonal declines.
+            // With mismatches the walk covers at most 128 columns.
+            bail = qlen > 128;
+            int32_t running = h0, best = h0, best_end = 0, prev = 0;
+            for (int w = 0; w < 4 && !bail; ++w) {
+                uint32_t m = bits[w];
+                while (m && !bail) {
+                    const int32_t p = 32 * w + __ffs(m) - 1;
+                    m &= m - 1;
+                    running += (p - prev) * a;
+                    if (p > prev && running > best) {
+                        best = running;
+                        best_end = p;
+                    }
+                    running -= b;
+                    if (running <= 0) bail = true;
+                    else if (zdrop > 0 && best - running > zdrop) bail = true;
+                    prev = p + 1;
+                }
+            }
+            if (!bail) {
+                running += (qlen - prev) * a;
+                if (qlen > prev && running > best) {
+                    best = running;
+                    best_end = qlen;
+                }
+                r.score = best;
+                r.qle = r.tle = best_end;
+                r.gtle = qlen;
+                r.gscore = running;
+                r.max_off = 0;
+            }
+        }
+        if (g == 0) {
+            if (!bail) out[idx] = r;
+            declined[idx] = bail ? 1 : 0;
+        }
+    }
+}
+
+__global__ __launch_bounds__(256) void ext_compact_kernel(const uint32_t *__restrict__ order, uint32_t n_slots,
+                                                          uint32_t chunk, const uint32_t *__restrict__ seg_off,
+                                                          const uint32_t *__restrict__ skip, int n_bins,
+                                                          const uint8_t *__restrict__ declined,
+                                                          uint32_t *__restrict__ corder,
+                                                          uint32_t *__restrict__ ccount) {
     __shared__ uint32_t s_bincnt[kUngappedBinsMax];  // declined jobs per bin (bins from the chunk's first)
     __shared__ uint32_t s_binpos[kUngappedBinsMax];  // their first compacted-queue position
     __shared__ uint32_t s_wave[4];
     const int lane = threadIdx.x & 63;
-    const int g = lane & 15;
-    const int gid = threadIdx.x >> 4;
     const uint32_t lo = blockIdx.x * chunk;
     const uint32_t hi = min(n_slots, lo + chunk);
-    const int bin0 = bin_of_slot(lo, 0, seg_off, n_bins);
     const uint32_t n = hi > lo ? hi - lo : 0;
+    const int bin0 = bin_of_slot(lo, 0, seg_off, n_bins);
     for (int k = threadIdx.x; k < kUngappedBinsMax; k += blockDim.x) s_bincnt[k] = 0;
-    for (uint32_t k = threadIdx.x; k < n; k += blockDim.x) s_idx[k] = order[lo + k];
     __syncthreads();
-    static_assert(sizeof(ExtJob) == 32, "a descriptor is two 16-byte words");
-    for (uint32_t k = threadIdx.x; k < 2 * n; k += blockDim.x)
-        ((uint4 *)s_jobs)[k] = ((const uint4 *)(jobs + s_idx[k >> 1]))[k & 1];
-    __syncthreads();
-
-    int bin = bin0;
-    for (uint32_t slot = lo + gid; slot < hi; slot += 16u) {
-        bin = bin_of_slot(slot, bin, seg_off, n_bins);
-        bool declined = false;
-        if (!skip[bin]) {
-            const uint32_t idx = s_idx[slot - lo];
-            const ExtJob jb = s_jobs[slot - lo];
-            const uint8_t *q = arena + jb.q_off;
-            const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
-            const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
-            bool bail = !(qlen > 0 && h0 > 0 && tlen >= qlen);
-            int32_t nmis = 0;
-            uint32_t bits[4] = {0u, 0u, 0u, 0u};
-            uint32_t qv[8], tv[8];
+    // thread t owns the slots [t*per, t*per + per) of the chunk: at most kPer of them
+    constexpr uint32_t kPer = (kUngappedChunkMax + 255) / 256;
+    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
+    const uint32_t t0 = min(n, threadIdx.x * per), t1 = min(n, t0 + per);
+    uint32_t idx[kPer];
+    uint32_t flags = 0, cnt = 0;
+    int bb = bin_of_slot(lo + t0, bin0, seg_off, n_bins);
+    const int first_bb = bb;
 #pragma unroll
-            for (int st = 0; st < 8; ++st) {  // independent loads of the first 128 columns
-                const int32_t c = 16 * st + g;
-                const bool in = !bail && c < qlen;
-                qv[st] = in ? q[c] : 0u;
-                tv[st] = in ? t[c] : 0u;
+    for (uint32_t u = 0; u < kPer; ++u) {
+        const uint32_t k = t0 + u;
+        idx[u] = 0;
+        if (k < t1) {
+            bb = bin_of_slot(lo + k, bb, seg_off, n_bins);
+            idx[u] = order[lo + k];
+            if (!skip[bb] && declined[idx[u]]) {
+                flags |= 1u << u;
+                ++cnt;
+                atomicAdd(&s_bincnt[bb - bin0], 1u);
             }
-#pragma unroll
-            for (int st = 0; st < 8; ++st) {
-                const int32_t c = 16 * st + g;
-                const bool in = !bail && c < qlen;
-                const uint64_t bn = __ballot(in && (qv[st] >= 4 || tv[st] >= 4));
-                const uint64_t bm = __ballot(in && qv[st] != tv[st]);
-                const uint32_t gn = (uint32_t)(bn >> (lane & ~15)) & 0xffffu;
-                const uint32_t gm = (uint32_t)(bm >> (lane & ~15)) & 0xffffu;
-                nmis += __popc(gm);
-                if (gn || nmis > xmax) bail = true;
-                else bits[st >> 1] |= gm << (16 * (st & 1));
-            }
-            // past column 128 only a mismatch-free walk can be accepted
-            for (int32_t c0 = 128; c0 < qlen && !bail; c0 += 16) {
-                const int32_t c = c0 + g;
-                const bool in = c < qlen;
-                const uint32_t qb = in ? q[c] : 0u, tb = in ? t[c] : 0u;
-                const uint64_t bx = __ballot(in && (qb >= 4 || tb >= 4 || qb != tb));
-                if ((uint32_t)(bx >> (lane & ~15)) & 0xffffu) bail = true;
-            }
-            bwg_ext_result r;
-            if (!bail && nmis == 0) {
-                // a mismatch-free walk only climbs: local and to-end alignments both take the query
-                r.score = h0 + qlen * a;
-                r.qle = r.tle = r.gtle = qlen;
-                r.gscore = r.score;
-                r.max_off = 0;
-            } else if (!bail) {
-                // the walk in runs between mismatches: a run of matches only climbs, so its last
-                // column is its best and z-drop can only fire at a mismatch; a dead diagonal declines.
-                // With mismatches the walk covers at most 128 columns.
-                bail = qlen > 128;
-                int32_t running = h0, best = h0, best_end = 0, prev = 0;
-                for (int w = 0; w < 4 && !bail; ++w) {
-                    uint32_t m = bits[w];
-                    while (m && !bail) {
-                        const int32_t p = 32 * w + __ffs(m) - 1;
-                        m &= m - 1;
-                        running += (p - prev) * a;
-                        if (p > prev && running > best) {
-                            best = running;
-                            best_end = p;
-                        }
-                        running -= b;
-                        if (running <= 0) bail = true;
-                        else if (zdrop > 0 && best - running > zdrop) bail = true;
-                        prev = p + 1;
-                    }
-                }
-                if (!bail) {
-                    running += (qlen - prev) * a;
-                    if (qlen > prev && running > best) {
-                        best = running;
-                        best_end = qlen;
-                    }
-                    r.score = best;
-                    r.qle = r.tle = best_end;
-                    r.gtle = qlen;
-                    r.gscore = running;
-                    r.max_off = 0;
-                }
-            }
-            if (!bail && g == 0) out[idx] = r;
-            declined = bail;
-        }
-        if (g == 0) {
-            s_flag[slot - lo] = declined ? 1 : 0;
-            if (declined) atomicAdd(&s_bincnt[bin - bin0], 1u);
         }
     }
     __syncthreads();
-
     // one reservation per bin of the chunk
     const int nb = n ? bin_of_slot(hi - 1, bin0, seg_off, n_bins) - bin0 + 1 : 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         const uint32_t c = s_bincnt[k];
         s_binpos[k] = c ? seg_off[bin0 + k] + atomicAdd(&ccount[kCountStride * (bin0 + k)], c) : 0u;
     }
-    // exclusive scan of the declined flags in slot order: thread t owns slots [t*per, t*per + per)
-    const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
-    const uint32_t t0 = min(n, threadIdx.x * per), t1 = min(n, t0 + per);
-    uint32_t cnt = 0;
-    for (uint32_t k = t0; k < t1; ++k) cnt += s_flag[k];
+    // exclusive scan of the declined counts in slot order
     uint32_t incl = cnt;  // inclusive scan over the wavefront
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
@@ -685,15 +701,19 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
     for (int w = 0; w < (int)(threadIdx.x >> 6); ++w) before += s_wave[w];
     // rank within its bin = declined slots before it in the chunk minus those of the earlier bins
     if (cnt) {
-        int bb = bin_of_slot(lo + t0, bin0, seg_off, n_bins);
+        bb = first_bb;
         uint32_t bin_first = 0;  // declined slots of the chunk's bins before bb
         for (int k = 0; k < bb - bin0; ++k) bin_first += s_bincnt[k];
-        for (uint32_t k = t0; k < t1; ++k) {
-            const int nbb = bin_of_slot(lo + k, bb, seg_off, n_bins);
-            for (; bb < nbb; ++bb) bin_first += s_bincnt[bb - bin0];
-            if (s_flag[k]) {
-                corder[s_binpos[bb - bin0] + (before - bin_first)] = s_idx[k];
-                ++before;
+#pragma unroll
+        for (uint32_t u = 0; u < kPer; ++u) {
+            const uint32_t k = t0 + u;
+            if (k < t1) {
+                const int nbb = bin_of_slot(lo + k, bb, seg_off, n_bins);
+                for (; bb < nbb; ++bb) bin_first += s_bincnt[bb - bin0];
+                if (flags >> u & 1u) {
+                    corder[s_binpos[bb - bin0] + (before - bin_first)] = idx[u];
+                    ++before;
+                }
             }
         }
     }
@@ -829,18 +849,24 @@ hipError_t launch_ext_retry_filter(hipStream_t st, const uint32_t *order, uint32
 
 hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
-                               const uint32_t *skip, int n_bins, uint32_t *corder, uint32_t *ccount,
-                               const ExtScoring &sc, int32_t xmax, bwg_ext_result *out, int min_blocks) {
+                               const uint32_t *skip, int n_bins, uint8_t *declined, uint32_t *corder,
+                               uint32_t *ccount, const ExtScoring &sc, int32_t xmax, bwg_ext_result *out,
+                               int min_blocks) {
     if (n_slots == 0) return hipSuccess;
     if (n_bins > 255) return hipErrorInvalidValue;
-    // contiguous chunks of at most kUngappedChunkMax slots, a multiple of 16, at least min_blocks
+    // the walk: a grid-stride loop of 16 jobs per block and step
+    const uint32_t walk_blocks = std::min<uint32_t>((uint32_t)std::max(1, min_blocks), (n_slots + 15) / 16);
+    hipLaunchKernelGGL(ext_ungapped_kernel, dim3(walk_blocks), dim3(256), 0, st, arena, ref, jobs, n_slots,
+                       (int32_t)sc.mat[0], -(int32_t)sc.mat[1], xmax, sc.zdrop, out, declined);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // the compaction: contiguous chunks of at most kUngappedChunkMax slots, at least min_blocks of them
     uint32_t blocks = (uint32_t)std::max(1, min_blocks);
     uint32_t chunk = (n_slots + blocks - 1) / blocks;
     chunk = std::min<uint32_t>((chunk + 15) / 16 * 16, kUngappedChunkMax);
     blocks = (n_slots + chunk - 1) / chunk;
-    hipLaunchKernelGGL(ext_ungapped_kernel, dim3(blocks), dim3(256), 0, st, arena, ref, jobs, order, n_slots, chunk,
-                       seg_off, skip, n_bins, corder, ccount, (int32_t)sc.mat[0], -(int32_t)sc.mat[1], xmax,
-                       sc.zdrop, out);
+    hipLaunchKernelGGL(ext_compact_kernel, dim3(blocks), dim3(256), 0, st, order, n_slots, chunk, seg_off, skip,
+                       n_bins, declined, corder, ccount);
     return hipGetLastError();
 }
 
