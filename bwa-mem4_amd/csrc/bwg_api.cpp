@@ -48,18 +48,53 @@ bool codes_ok(const uint8_t *p, size_t len) {
 
 // Shared validation of the arena description.  The base codes are checked over the whole arena
 // first, one pass at memory speed; only an arena holding some code above 4 is checked job by job
-// (such a byte may lie outside every job).
+// (such a byte may lie outside every job).  Large calls split each pass over `parts` threads.
 int check_jobs(size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
-               const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len) {
+               const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, int parts) {
     if (n == 0) return BWG_OK;
     if (!arena || !q_off || !q_len || !t_off || !t_len) return BWG_EINVAL;
-    for (size_t k = 0; k < n; ++k) {
-        if (q_off[k] > arena_len || q_len[k] > arena_len - q_off[k]) return BWG_EINVAL;
-        if (t_off[k] > arena_len || t_len[k] > arena_len - t_off[k]) return BWG_EINVAL;
-    }
-    if (codes_ok(arena, arena_len)) return BWG_OK;
-    for (size_t k = 0; k < n; ++k)
-        if (!codes_ok(arena + q_off[k], q_len[k]) || !codes_ok(arena + t_off[k], t_len[k])) return BWG_EINVAL;
+    std::atomic<bool> bad{false};
+    parallel_for(n, parts, [&](size_t b, size_t e, int) {
+        for (size_t k = b; k < e; ++k)
+            if (q_off[k] > arena_len || q_len[k] > arena_len - q_off[k] || t_off[k] > arena_len ||
+                t_len[k] > arena_len - t_off[k]) {
+                bad = true;
+                return;
+            }
+    });
+    if (bad) return BWG_EINVAL;
+    const int arena_parts = (int)std::max<size_t>(1, std::min<size_t>((size_t)parts, arena_len >> 20));
+    parallel_for(arena_len, arena_parts, [&](size_t b, size_t e, int) {
+        if (!codes_ok(arena + b, e - b)) bad = true;
+    });
+    if (!bad) return BWG_OK;
+    bad = false;
+    parallel_for(n, parts, [&](size_t b, size_t e, int) {
+        for (size_t k = b; k < e; ++k)
+            if (!codes_ok(arena + q_off[k], q_len[k]) || !codes_ok(arena + t_off[k], t_len[k])) {
+                bad = true;
+                return;
+            }
+    });
+    return bad ? BWG_EINVAL : BWG_OK;
+}
+
+// The first job (in index order) for which test(k) returns an error code, tested over `parts`
+// contiguous ranges: the lowest range with a failure holds it.
+template <class Test>
+int first_failure(size_t n, int parts, Test test) {
+    std::vector<int> code((size_t)std::max(parts, 1), BWG_OK);
+    parallel_for(n, parts, [&](size_t b, size_t e, int part) {
+        for (size_t k = b; k < e; ++k) {
+            const int r = test(k);
+            if (r) {
+                code[(size_t)part] = r;
+                return;
+            }
+        }
+    });
+    for (int r : code)
+        if (r) return r;
     return BWG_OK;
 }
 
@@ -117,6 +152,7 @@ void bwg_destroy(bwg_ctx *c) {
     for (auto &l : c->lanes) l->destroy();
     c->lanes.clear();
     c->free_lanes.clear();
+    destroy_uploads(c);
     for (hipEvent_t ev : c->na_ev)
         if (ev) (void)hipEventDestroy(ev);
     c->pool.release_all();
@@ -191,6 +227,10 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
     case BWG_OPT_COALESCE:
         t.coalesce = v != 0;
         return BWG_OK;
+    case BWG_OPT_HOST_THREADS:
+        if (v < 0 || v > 16) return BWG_EINVAL;
+        t.host_threads = (int)v;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -224,6 +264,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_POOL_CACHE_MB: return (int64_t)(c->pool.cache_cap() >> 20);
     case BWG_OPT_ITEM_TIMING: return t.item_timing;
     case BWG_OPT_COALESCE: return t.coalesce;
+    case BWG_OPT_HOST_THREADS: return t.host_threads;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns.load();
@@ -283,34 +324,34 @@ inline int64_t ns_between(Clock::time_point a, Clock::time_point b) {
 // Preconditions of ksw_extend2 (crates/bwa-extend/src/lib.rs:181-195, sw.rs:203-230) and the size
 // limits of include/bwg.h.
 int check_ext(const bwg_ext_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
-              const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0) {
+              const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0,
+              int parts = 1) {
     int rc = check_mat(p->m, p->mat);
     if (rc) return rc;
     if (p->e_del < 1 || p->e_ins < 1) return BWG_EINVAL;
-    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len, parts);
     if (rc) return rc;
     if (n && !h0) return BWG_EINVAL;
     const ExtScoring sc = ext_scoring(*p);
-    for (size_t k = 0; k < n; ++k) {
-        if (h0[k] <= 0 || q_len[k] < 1) return BWG_EINVAL;
-        if (!ext_job_fits(sc, q_len[k], t_len[k], h0[k])) return BWG_E2BIG;
-    }
-    return BWG_OK;
+    return first_failure(n, parts, [&](size_t k) {
+        if (h0[k] <= 0 || q_len[k] < 1) return (int)BWG_EINVAL;
+        return ext_job_fits(sc, q_len[k], t_len[k], h0[k]) ? (int)BWG_OK : (int)BWG_E2BIG;
+    });
 }
 
 // Preconditions of ksw_align2 / batched_ksw_align2 (crates/bwa-extend/src/sw.rs:1108-1120,
 // crates/bwa-neon/src/matesw.rs:299-309) and the size limits.
 int check_ksw(const bwg_ksw_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
-              const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len) {
+              const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len, int parts = 1) {
     int rc = check_mat(p->m, p->mat);
     if (rc) return rc;
     if (p->e_del < 1 || p->e_ins < 1 || p->max_sc < 1) return BWG_EINVAL;
-    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len, parts);
     if (rc) return rc;
     const KswScoring sc = ksw_scoring(*p);
-    for (size_t k = 0; k < n; ++k)
-        if (!ksw_job_fits(sc, q_len[k], t_len[k])) return BWG_E2BIG;
-    return BWG_OK;
+    return first_failure(n, parts, [&](size_t k) {
+        return ksw_job_fits(sc, q_len[k], t_len[k]) ? (int)BWG_OK : (int)BWG_E2BIG;
+    });
 }
 
 int check_global(const bwg_global_params *p, size_t n, const uint8_t *arena, size_t arena_len, const uint64_t *q_off,
@@ -319,7 +360,7 @@ int check_global(const bwg_global_params *p, size_t n, const uint8_t *arena, siz
     int rc = check_mat(p->m, p->mat);
     if (rc) return rc;
     if (p->e_del < 1 || p->e_ins < 1 || p->o_del < 0 || p->o_ins < 0) return BWG_EINVAL;
-    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len);
+    rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len, 1);
     if (rc) return rc;
     for (size_t k = 0; k < n; ++k) {
         if (q_len[k] > 512) return BWG_E2BIG;
@@ -431,6 +472,8 @@ int run_on_lane(bwg_ctx *c, Clock::time_point t_begin, Clock::time_point t_valid
     const Clock::time_point t_staged = Clock::now();
     if (!rc) rc = prepare_batch(c, b, b->tu.form, L->ss);
     const Clock::time_point t_planned = Clock::now();
+    if (!rc) rc = wait_arena(b);  // the rest of an upload the planning did not hide
+    const Clock::time_point t_uploaded = Clock::now();
     if (!rc) rc = enqueue_batch(c, b, b->tu.form, L->ss);
     if (!rc) rc = bwg_map_hip(hipStreamSynchronize(L->ss.main));
     const Clock::time_point t_run = Clock::now();
@@ -444,9 +487,9 @@ int run_on_lane(bwg_ctx *c, Clock::time_point t_begin, Clock::time_point t_valid
     release_lane(c, L);
     if (!rc) {
         c->phase_ns[kPhValidate] = ns_between(t_begin, t_valid);
-        c->phase_ns[kPhStage] = ns_between(t_valid, t_staged);
+        c->phase_ns[kPhStage] = ns_between(t_valid, t_staged) + ns_between(t_planned, t_uploaded);
         c->phase_ns[kPhPlan] = ns_between(t_staged, t_planned);
-        c->phase_ns[kPhKernels] = ns_between(t_planned, t_run);
+        c->phase_ns[kPhKernels] = ns_between(t_uploaded, t_run);
         c->phase_ns[kPhFetch] = ns_between(t_run, t_end);
         c->phase_ns[kPhTotal] = ns_between(t_begin, t_end);
     }
@@ -465,7 +508,7 @@ int bwg_batch_stage_ext(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uin
                         bwg_batch **out) {
     if (!c || !p || !out) return BWG_EINVAL;
     *out = nullptr;
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     const ExtScoring sc = ext_scoring(*p);
     return stage_on_context(c, [&](hipStream_t st) {
@@ -480,7 +523,7 @@ int bwg_batch_stage_side(bwg_ctx *c, const bwg_ext_params *p, int max_band_try, 
     *out = nullptr;
     if (max_band_try < 1 || max_band_try > 8 || p->w < 0) return BWG_EINVAL;
     if (p->w > (INT32_MAX >> (max_band_try - 1))) return BWG_E2BIG;
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     if (n && !prev) return BWG_EINVAL;
     const ExtScoring sc = ext_scoring(*p);
@@ -545,7 +588,7 @@ int bwg_batch_stage_ksw(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const uin
                         const uint64_t *t_off, const uint32_t *t_len, bwg_batch **out) {
     if (!c || !p || !out) return BWG_EINVAL;
     *out = nullptr;
-    int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len);
+    int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     const KswScoring sc = ksw_scoring(*p);
     return stage_on_context(c, [&](hipStream_t st) {
@@ -620,13 +663,13 @@ int bwg_extend_batch(bwg_ctx *c, const bwg_ext_params *p, size_t n, const uint8_
         ch.out = out;
         return run_coalesced(c, ch);  // the server validates the chunk as this call would
     }
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     const ExtScoring sc = ext_scoring(*p);
     return run_on_lane(
         c, t0, Clock::now(),
         [&](hipStream_t st, bwg_batch **b) {
-            return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, b);
+            return stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, b, true);
         },
         [&](bwg_batch *b, hipStream_t st) { return enqueue_results(c, b, out, st); });
 }
@@ -638,7 +681,7 @@ int bwg_extend_batch_noalloc(bwg_ctx *c, const bwg_ext_params *p, size_t n, cons
     if (!c || !p) return BWG_EINVAL;
     if (n == 0) return BWG_OK;
     if (!out) return BWG_EINVAL;
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     const ExtScoring sc = ext_scoring(*p);
     uint8_t *base = noalloc_scratch(c->device);
@@ -681,13 +724,14 @@ int bwg_extend_side_batch(bwg_ctx *c, const bwg_ext_params *p, int max_band_try,
     const Clock::time_point t0 = Clock::now();
     if (max_band_try > 8 || p->w < 0) return BWG_EINVAL;
     if (p->w > (INT32_MAX >> (max_band_try - 1))) return BWG_E2BIG;
-    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0);
+    int rc = check_ext(p, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     const ExtScoring sc = ext_scoring(*p);
     return run_on_lane(
         c, t0, Clock::now(),
         [&](hipStream_t st, bwg_batch **b) {
-            return stage_side(c, sc, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev, st, b);
+            return stage_side(c, sc, max_band_try, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, prev, st, b,
+                              true);
         },
         [&](bwg_batch *b, hipStream_t st) {
             int r = enqueue_results(c, b, out, st);
@@ -727,13 +771,13 @@ int bwg_ksw_align2_batch(bwg_ctx *c, const bwg_ksw_params *p, size_t n, const ui
         ch.out = out;
         return run_coalesced(c, ch);
     }
-    int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len);
+    int rc = check_ksw(p, n, arena, arena_len, q_off, q_len, t_off, t_len, host_parts(c->tuning_snapshot(), n));
     if (rc) return rc;
     const KswScoring sc = ksw_scoring(*p);
     return run_on_lane(
         c, t0, Clock::now(),
         [&](hipStream_t st, bwg_batch **b) {
-            return stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, st, b);
+            return stage_ksw(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, nullptr, st, b, true);
         },
         [&](bwg_batch *b, hipStream_t st) { return enqueue_results(c, b, out, st); });
 }

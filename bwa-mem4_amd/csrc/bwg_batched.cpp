@@ -63,7 +63,7 @@ struct Plan {
     int n_bins = 0;
     std::vector<uint32_t> meta;
     // host copies kept alive until the uploads complete
-    std::vector<uint32_t> order;
+    HostVec<uint32_t> order;
     std::vector<KswJob> ksw_jobs;
     std::vector<ExtJob> ext_plain_jobs;
     std::vector<KswJob> ksw_plain_jobs;
@@ -121,21 +121,15 @@ inline uint32_t work_key(uint64_t w) {
     return e * 2048 + m + 1;
 }
 
-// Group jobs into bins by key, order the bins by stage then descending work and each bin's jobs
-// by descending work; returns the order array, bins get their offsets.  Large batches: two stable
-// counting-sort passes (work key descending, then bin), O(n); small ones: one comparison sort by
-// (bin, work key, index), the same order without the 65537-entry histogram.
-template <class KeyFn, class WorkFn>
-std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, WorkFn work_of) {
-    std::vector<int> key(n);
-    std::vector<uint16_t> wk(n);
-    for (size_t k = 0; k < n; ++k) {
-        key[k] = key_of(k);
-        const uint64_t w = work_of(k);
-        wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
-        bins[key[k]].count++;
-        bins[key[k]].work += (double)w;
-    }
+// The second half of binning: order the bins by stage then descending work, give them their
+// offsets, and order the jobs by (bin, descending work, index) from each job's bin (its index in
+// `bins` as created) and work key wk (65535 - work_key, so ascending wk is descending work).
+// Small batches: one comparison sort; large ones: a stable LSD radix sort over the 24-bit (32-bit
+// past 256 bins) key (bin rank << 16 | wk), 8 bits a pass, each pass split over `parts` threads
+// (per-part digit histograms, offsets in (digit, part) order, a stable scatter), which gives the
+// same order.
+template <class BinOf>
+HostVec<uint32_t> order_jobs(size_t n, std::vector<Item> &bins, BinOf bin_of, const uint16_t *wk, int parts) {
     std::vector<int> perm(bins.size());
     std::iota(perm.begin(), perm.end(), 0);
     std::stable_sort(perm.begin(), perm.end(), [&](int a, int b) {
@@ -143,9 +137,9 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
         return bins[a].work > bins[b].work;
     });
     std::vector<Item> sorted;
-    std::vector<int> remap(bins.size());
+    std::vector<uint32_t> remap(bins.size());
     for (size_t r = 0; r < perm.size(); ++r) {
-        remap[perm[r]] = (int)r;
+        remap[perm[r]] = (uint32_t)r;
         sorted.push_back(bins[perm[r]]);
     }
     uint32_t off = 0;
@@ -153,32 +147,65 @@ std::vector<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, 
         b.offset = off;
         off += b.count;
     }
-    std::vector<uint32_t> order(n);
+    bins.swap(sorted);
+    HostVec<uint32_t> order(n);
     if (n < 16384) {
         std::iota(order.begin(), order.end(), 0u);
         std::sort(order.begin(), order.end(), [&](uint32_t x, uint32_t y) {
-            const int bx = remap[key[x]], by = remap[key[y]];
+            const uint32_t bx = remap[bin_of(x)], by = remap[bin_of(y)];
             if (bx != by) return bx < by;
             if (wk[x] != wk[y]) return wk[x] < wk[y];
             return x < y;
         });
-        bins.swap(sorted);
         return order;
     }
-    // pass 1: by descending work
-    std::vector<uint32_t> hist(65537, 0), tmp(n);
-    for (size_t k = 0; k < n; ++k) hist[(size_t)wk[k] + 1]++;
-    for (size_t v = 1; v < hist.size(); ++v) hist[v] += hist[v - 1];
-    for (size_t k = 0; k < n; ++k) tmp[hist[wk[k]]++] = (uint32_t)k;
-    // pass 2: stable by bin
-    std::vector<uint32_t> cur(sorted.size());
-    for (size_t r = 0; r < sorted.size(); ++r) cur[r] = sorted[r].offset;
-    for (size_t r = 0; r < n; ++r) {
-        const uint32_t k = tmp[r];
-        order[cur[remap[key[k]]]++] = k;
+    HostVec<uint64_t> a(n), t(n);
+    parallel_for(n, parts, [&](size_t lo, size_t hi, int) {
+        for (size_t k = lo; k < hi; ++k)
+            a[k] = ((uint64_t)((remap[bin_of(k)] << 16) | wk[k]) << 32) | (uint64_t)k;
+    });
+    const int passes = bins.size() <= 256 ? 3 : 4;
+    std::vector<uint32_t> hist((size_t)parts * 256);
+    for (int pass = 0; pass < passes; ++pass) {
+        const int sh = 32 + 8 * pass;
+        std::fill(hist.begin(), hist.end(), 0u);
+        parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
+            uint32_t *h = &hist[(size_t)q * 256];
+            for (size_t k = lo; k < hi; ++k) h[(a[k] >> sh) & 255]++;
+        });
+        uint32_t run = 0;
+        for (int d = 0; d < 256; ++d)
+            for (int q = 0; q < parts; ++q) {
+                const uint32_t cnt = hist[(size_t)q * 256 + d];
+                hist[(size_t)q * 256 + d] = run;
+                run += cnt;
+            }
+        parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
+            uint32_t *h = &hist[(size_t)q * 256];
+            for (size_t k = lo; k < hi; ++k) t[h[(a[k] >> sh) & 255]++] = a[k];
+        });
+        a.swap(t);
     }
-    bins.swap(sorted);
+    parallel_for(n, parts, [&](size_t lo, size_t hi, int) {
+        for (size_t k = lo; k < hi; ++k) order[k] = (uint32_t)a[k];
+    });
     return order;
+}
+
+// Group jobs into bins by key (which may create bins), accumulate each bin's count and work, then
+// order_jobs.
+template <class KeyFn, class WorkFn>
+HostVec<uint32_t> bin_jobs(size_t n, std::vector<Item> &bins, KeyFn key_of, WorkFn work_of) {
+    HostVec<int> key(n);
+    HostVec<uint16_t> wk(n);
+    for (size_t k = 0; k < n; ++k) {
+        key[k] = key_of(k);
+        const uint64_t w = work_of(k);
+        wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
+        bins[key[k]].count++;
+        bins[key[k]].work += (double)w;
+    }
+    return order_jobs(n, bins, [&](size_t k) { return key[k]; }, wk.data(), 1);
 }
 
 // The bins of a plan, created on first use and found again through a table indexed by the kernel
@@ -358,23 +385,36 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     BinSet bs;
     std::vector<Item> &bins = bs.bins;
     auto find = [&](int kind, int param, bool narrow) { return bs.find(kind, param, narrow); };
-    // classify, then tune the classes this batch has enough jobs of and the table lacks
+    // classify (over the host threads, each with its own memo), then tune the classes this batch
+    // has enough jobs of and the table lacks
+    const int parts = host_parts(tu, n);
     const int n_classes = kExtShortClasses + kNumExtKs;
-    std::vector<int8_t> cls(n), kidx(n);
-    std::vector<std::vector<uint32_t>> members(n_classes);
-    ExtClassifier classifier(sc, tu);
-    for (size_t k = 0; k < n; ++k) {
-        int ki = -1;
-        cls[k] = (int8_t)classifier.classify(b->hext[k], &ki);
-        kidx[k] = (int8_t)ki;
-        if (cls[k] >= 0) members[cls[k]].push_back((uint32_t)k);
-    }
+    HostVec<int8_t> cls(n), kidx(n);
+    std::vector<uint32_t> cls_count((size_t)parts * n_classes, 0u);
+    parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
+        ExtClassifier classifier(sc, tu);
+        uint32_t *cc = &cls_count[(size_t)q * n_classes];
+        for (size_t k = lo; k < hi; ++k) {
+            int ki = -1;
+            const int cl = classifier.classify(b->hext[k], &ki);
+            cls[k] = (int8_t)cl;
+            kidx[k] = (int8_t)ki;
+            if (cl >= 0) cc[cl]++;
+        }
+    });
     const uint64_t sig = scoring_sig(sc);
     std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0), choice_width(n_classes, 16);
     for (int cl = 0; cl < n_classes; ++cl) {
         if (!tu.autotune) break;
-        if (tu.tune_new && !retry && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
-            const int rc = tune_ext_class(c, b, sc, sig, cl, members[cl], st);
+        size_t n_members = 0;
+        for (int q = 0; q < parts; ++q) n_members += cls_count[(size_t)q * n_classes + cl];
+        if (tu.tune_new && !retry && n_members >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
+            std::vector<uint32_t> members;
+            members.reserve(n_members);
+            for (size_t k = 0; k < n; ++k)
+                if (cls[k] == cl) members.push_back((uint32_t)k);
+            int rc = wait_arena(b);  // the sample kernels read the arena
+            if (!rc) rc = tune_ext_class(c, b, sc, sig, cl, members, st);
             if (rc) return rc;
         }
         TuneEntry ent;
@@ -384,14 +424,17 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             choice_width[cl] = ent.cands[ent.choice].width;
         }
     }
-    auto key_of = [&](size_t k) {
-        const ExtJob &j = b->hext[k];
-        const int q = (int)j.qlen;
+    // Each job's kernel mapping as a code (kind << 8 | param << 1 | narrow), a pure function of its
+    // class and length, so that the parts accumulate per-mapping counts, work and column capacity
+    // on their own; the bins are then made in code order.
+    auto code_of = [&](size_t k) -> uint32_t {
+        const int q = (int)b->hext[k].qlen;
         const int cl = cls[k];
-        if (cl == -1) return find(kExtPlain, 0, false);
         int kind, param;
         bool narrow = cl >= 0;
-        if (cl >= 0 && choice_kind[cl] >= 0) {
+        if (cl == -1) {
+            kind = kExtPlain, param = 0, narrow = false;
+        } else if (cl >= 0 && choice_kind[cl] >= 0) {
             kind = choice_kind[cl];
             param = choice_param[cl];
             narrow = choice_width[cl] == 16;
@@ -403,12 +446,47 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             const int r = std::min(kidx[k] + (tu.ext_k_step ? 1 : 0), kNumExtKs - 1);
             kind = kExtGroup, param = kExtKs[r];
         }
-        const int bi = find(kind, param, narrow);
-        if (kind == kExtGroup) bins[bi].qcap = std::max(bins[bi].qcap, qcap_class(q));
-        return bi;
+        return ((uint32_t)kind << 8) | ((uint32_t)param << 1) | (narrow ? 1u : 0u);
     };
-    auto work_of = [&](size_t k) { return (uint64_t)b->hext[k].qlen * (b->hext[k].tlen + 1); };
-    p->order = bin_jobs(n, bins, key_of, work_of);
+    constexpr int kCodes = 16 << 8;
+    struct CodeAcc {
+        uint32_t count = 0;
+        int32_t qcap = 0;
+        double work = 0.0;
+    };
+    std::vector<CodeAcc> acc((size_t)parts * kCodes);
+    HostVec<uint16_t> code(n), wk(n);
+    parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
+        CodeAcc *a = &acc[(size_t)q * kCodes];
+        for (size_t k = lo; k < hi; ++k) {
+            const uint32_t cd = code_of(k);
+            const ExtJob &j = b->hext[k];
+            const uint64_t w = (uint64_t)j.qlen * (j.tlen + 1);
+            code[k] = (uint16_t)cd;
+            wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
+            a[cd].count++;
+            a[cd].work += (double)w;
+            if ((int)(cd >> 8) == kExtGroup) a[cd].qcap = std::max(a[cd].qcap, qcap_class((int)j.qlen));
+        }
+    });
+    std::vector<int16_t> bin_of_code(kCodes, -1);
+    for (int cd = 0; cd < kCodes; ++cd) {
+        CodeAcc t;
+        for (int q = 0; q < parts; ++q) {
+            const CodeAcc &a = acc[(size_t)q * kCodes + cd];
+            t.count += a.count;
+            t.work += a.work;
+            t.qcap = std::max(t.qcap, a.qcap);
+        }
+        if (!t.count) continue;
+        const int kind = cd >> 8;
+        const int bi = find(kind, (cd >> 1) & 127, (cd & 1) != 0);
+        bins[bi].count += t.count;
+        bins[bi].work += t.work;
+        if (kind == kExtGroup) bins[bi].qcap = std::max(bins[bi].qcap, t.qcap);
+        bin_of_code[cd] = (int16_t)bi;
+    }
+    p->order = order_jobs(n, bins, [&](size_t k) { return (uint32_t)bin_of_code[code[k]]; }, wk.data(), parts);
     for (size_t r = 0; r < bins.size(); ++r) {
         Item &it = bins[r];
         it.head = (uint32_t)r;
@@ -600,7 +678,8 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         if (!tu.autotune) continue;
         // a rescue-window batch's descriptors are written on the device, so it only reads the table
         if (tu.tune_new && !b->matesw && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {
-            const int rc = tune_ksw_class(c, b, sig, cl, members[cl], st);
+            int rc = wait_arena(b);  // the sample kernels read the arena
+            if (!rc) rc = tune_ksw_class(c, b, sig, cl, members[cl], st);
             if (rc) return rc;
         }
         TuneEntry ent;
@@ -618,7 +697,7 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     // Row-maximum slots for the fast-path jobs: tiles of 64 consecutive jobs of the order array,
     // row-major within a tile ([row][64]), so the tracker kernel's 64 lanes read consecutive words.
     // A job's slot for row i is row_slot(job) + 64*i.
-    p->ksw_jobs = b->hksw;
+    p->ksw_jobs.assign(b->hksw.begin(), b->hksw.end());
     for (auto &it : bins)
         if (it.kind == kKswFwd) p->n_fast += it.count;
     uint64_t rows = 0;

@@ -2,10 +2,13 @@
 // copies.  The batched form's planning and launches are in bwg_batched.cpp.
 #include "bwg_runtime.h"
 
+#include <sched.h>
+
 #include <algorithm>
 #include <cstring>
 #include <iterator>
 #include <new>
+#include <system_error>
 
 namespace bwg {
 
@@ -87,6 +90,20 @@ void DevicePool::release_all() {
 // ---------------------------------------------------------------------------------------------
 // StreamSet
 // ---------------------------------------------------------------------------------------------
+int host_parts(const Tuning &tu, size_t n) {
+    static const int avail = [] {
+        cpu_set_t set;
+        CPU_ZERO(&set);
+        int k = sched_getaffinity(0, sizeof set, &set) == 0 ? CPU_COUNT(&set) : 0;
+        if (k <= 0) k = (int)std::thread::hardware_concurrency();
+        return std::max(1, k);
+    }();
+    // an explicit thread count also splits smaller calls (down to 1024 jobs a thread)
+    const int cap = tu.host_threads > 0 ? tu.host_threads : std::min(avail, 16);
+    const size_t grain = tu.host_threads > 0 ? 1024 : kHostGrain;
+    return (int)std::max<size_t>(1, std::min<size_t>((size_t)cap, n / grain));
+}
+
 int StreamSet::create(int rotation) {
     hipStream_t s[kAuxStreams + 1] = {};
     for (int k = 0; k <= kAuxStreams; ++k)
@@ -186,17 +203,43 @@ bwg_batch *new_batch(const bwg_ctx *c, int kind, size_t n, const hipEvent_t *ev 
     return b;
 }
 
+// Arenas from this size on are uploaded from a helper thread when the caller defers them.
+constexpr size_t kDeferArenaBytes = 8u << 20;
+
 int stage_buffers(bwg_ctx *c, bwg_batch *b, const uint8_t *arena, size_t arena_len, const void *jobs,
-                  size_t job_bytes, size_t out_bytes, hipStream_t st) {
+                  size_t job_bytes, size_t out_bytes, hipStream_t st, bool defer_arena = false) {
     b->arena_len = arena_len;
     b->d_arena = (uint8_t *)batch_alloc(c, b, arena_len + 64);
     b->d_jobs = batch_alloc(c, b, std::max<size_t>(job_bytes, 64));
     b->d_out = batch_alloc(c, b, std::max<size_t>(out_bytes, 64));
     if (!b->d_arena || !b->d_jobs || !b->d_out) return BWG_ENOMEM;
     hipError_t e = hipSuccess;
-    if (arena_len) e = hipMemcpyAsync(b->d_arena, arena, arena_len, hipMemcpyHostToDevice, st);
-    if (e == hipSuccess && job_bytes) e = hipMemcpyAsync(b->d_jobs, jobs, job_bytes, hipMemcpyHostToDevice, st);
-    return bwg_map_hip(e);
+    if (job_bytes) e = hipMemcpyAsync(b->d_jobs, jobs, job_bytes, hipMemcpyHostToDevice, st);
+    if (e != hipSuccess || !arena_len) return bwg_map_hip(e);
+    if (defer_arena && arena_len >= kDeferArenaBytes && !b->bump) {
+        // an upload from pageable memory keeps its host threads busy for the whole transfer: give
+        // it threads of its own, in stream order on st, while the caller plans; it goes through
+        // the context's pinned chunks (several copying threads) when a set is free
+        const int dev = c->device;
+        uint8_t *dst = b->d_arena;
+        try {
+            b->arena_copy = std::thread([c, b, dev, dst, arena, arena_len, st] {
+                hipError_t r = hipSetDevice(dev);
+                UploadStaging *u = r == hipSuccess ? acquire_upload(c) : nullptr;
+                if (u) {
+                    r = upload_pinned(*u, dst, arena, arena_len, st, dev);
+                    release_upload(c, u);
+                } else if (r == hipSuccess) {
+                    r = hipMemcpyAsync(dst, arena, arena_len, hipMemcpyHostToDevice, st);
+                }
+                b->arena_err = r;
+            });
+            return BWG_OK;
+        } catch (const std::system_error &) {
+            // no thread to spare: upload here
+        }
+    }
+    return bwg_map_hip(hipMemcpyAsync(b->d_arena, arena, arena_len, hipMemcpyHostToDevice, st));
 }
 
 }  // namespace
@@ -205,12 +248,109 @@ void *batch_alloc(bwg_ctx *c, bwg_batch *b, size_t bytes) {
     return b->bump ? b->bump->take(bytes) : c->pool.alloc(bytes);
 }
 
+int wait_arena(bwg_batch *b) {
+    if (!b->arena_copy.joinable()) return BWG_OK;
+    b->arena_copy.join();
+    return bwg_map_hip(b->arena_err);
+}
+
+int UploadStaging::create() {
+    for (int k = 0; k < kBufs; ++k) {
+        if (hipHostMalloc(&buf[k], kChunk, hipHostMallocDefault) != hipSuccess) {
+            buf[k] = nullptr;
+            return BWG_ENOMEM;
+        }
+        if (hipEventCreateWithFlags(&ev[k], hipEventDisableTiming) != hipSuccess) {
+            ev[k] = nullptr;
+            return BWG_EIO;
+        }
+    }
+    return BWG_OK;
+}
+
+void UploadStaging::destroy() {
+    for (int k = 0; k < kBufs; ++k) {
+        if (ev[k]) {
+            if (pending[k]) (void)hipEventSynchronize(ev[k]);
+            (void)hipEventDestroy(ev[k]);
+        }
+        if (buf[k]) (void)hipHostFree(buf[k]);
+        ev[k] = nullptr;
+        buf[k] = nullptr;
+        pending[k] = false;
+    }
+}
+
+UploadStaging *acquire_upload(bwg_ctx *c) {
+    std::lock_guard<std::mutex> g(c->up_mu);
+    if (!c->up_free.empty()) {
+        UploadStaging *u = c->up_free.back();
+        c->up_free.pop_back();
+        return u;
+    }
+    if ((int)c->up_all.size() >= bwg_ctx::kMaxUploadSets) return nullptr;
+    std::unique_ptr<UploadStaging> u(new (std::nothrow) UploadStaging());
+    if (!u) return nullptr;
+    if (u->create() != BWG_OK) {
+        u->destroy();
+        return nullptr;
+    }
+    c->up_all.push_back(std::move(u));
+    return c->up_all.back().get();
+}
+
+void release_upload(bwg_ctx *c, UploadStaging *u) {
+    std::lock_guard<std::mutex> g(c->up_mu);
+    c->up_free.push_back(u);
+}
+
+void destroy_uploads(bwg_ctx *c) {
+    std::lock_guard<std::mutex> g(c->up_mu);
+    for (auto &u : c->up_all) u->destroy();
+    c->up_all.clear();
+    c->up_free.clear();
+}
+
+hipError_t upload_pinned(UploadStaging &u, uint8_t *dst, const uint8_t *src, size_t len, hipStream_t st,
+                         int device) {
+    const size_t chunks = (len + UploadStaging::kChunk - 1) / UploadStaging::kChunk;
+    const int T = (int)std::max<size_t>(1, std::min<size_t>(UploadStaging::kThreads, chunks));
+    std::atomic<int> err{(int)hipSuccess};
+    // thread t copies chunks t, t + T, ... through its own buffers 2t and 2t + 1 in turn
+    auto worker = [&](int t) {
+        hipError_t e = hipSetDevice(device);
+        int flip = 0;
+        for (size_t ch = (size_t)t; ch < chunks && e == hipSuccess && err.load() == (int)hipSuccess; ch += (size_t)T) {
+            const int bi = 2 * t + flip;
+            flip ^= 1;
+            if (u.pending[bi]) e = hipEventSynchronize(u.ev[bi]);  // its last DMA has read it
+            if (e != hipSuccess) break;
+            const size_t off = ch * UploadStaging::kChunk;
+            const size_t sz = std::min(UploadStaging::kChunk, len - off);
+            std::memcpy(u.buf[bi], src + off, sz);
+            e = hipMemcpyAsync(dst + off, u.buf[bi], sz, hipMemcpyHostToDevice, st);
+            if (e == hipSuccess) e = hipEventRecord(u.ev[bi], st);
+            if (e == hipSuccess) u.pending[bi] = true;
+        }
+        if (e != hipSuccess) err.store((int)e);
+    };
+    std::vector<std::thread> th;
+    try {
+        for (int t = 1; t < T; ++t) th.emplace_back(worker, t);
+    } catch (const std::system_error &) {
+        err.store((int)hipErrorOutOfMemory);
+    }
+    worker(0);
+    for (auto &t : th) t.join();
+    return (hipError_t)err.load();
+}
+
 namespace {
 
 int stage_ext_impl(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
                    const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                    const int32_t *h0, const uint8_t *ref, Bump *bump, const hipEvent_t *ev, hipStream_t st,
-                   bwg_batch **out) {
+                   bwg_batch **out, bool defer_arena) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     bwg_batch *b = new_batch(c, 0, n, ev);
     if (!b) return BWG_ENOMEM;
@@ -218,17 +358,19 @@ int stage_ext_impl(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *ar
     b->esc = sc;
     b->d_ref = ref;
     b->hext.resize(n);
-    for (size_t k = 0; k < n; ++k) {
-        ExtJob &j = b->hext[k];
-        j.q_off = q_off[k];
-        j.t_off = t_off[k];
-        j.qlen = q_len[k];
-        j.tlen = t_len[k];
-        j.h0 = h0[k];
-        j.aux = (uint32_t)k;
-    }
+    parallel_for(n, host_parts(b->tu, n), [&](size_t lo, size_t hi, int) {
+        for (size_t k = lo; k < hi; ++k) {
+            ExtJob &j = b->hext[k];
+            j.q_off = q_off[k];
+            j.t_off = t_off[k];
+            j.qlen = q_len[k];
+            j.tlen = t_len[k];
+            j.h0 = h0[k];
+            j.aux = (uint32_t)k;
+        }
+    });
     int rc = stage_buffers(c, b, arena, arena_len, b->hext.data(), n * sizeof(ExtJob),
-                           n * sizeof(bwg_ext_result), st);
+                           n * sizeof(bwg_ext_result), st, defer_arena);
     if (rc) {
         destroy_batch(c, b);
         return rc;
@@ -242,22 +384,23 @@ int stage_ext_impl(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *ar
 int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
-              bwg_batch **out) {
+              bwg_batch **out, bool defer_arena) {
     return stage_ext_impl(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, ref, nullptr, nullptr, st,
-                          out);
+                          out, defer_arena);
 }
 
 int stage_ext_bump(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
                    const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
                    const int32_t *h0, Bump *bump, const hipEvent_t ev[2], hipStream_t st, bwg_batch **out) {
-    return stage_ext_impl(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, bump, ev, st, out);
+    return stage_ext_impl(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, bump, ev, st, out,
+                          false);
 }
 
 int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint8_t *arena, size_t arena_len,
                const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
-               const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out) {
+               const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out, bool defer_arena) {
     bwg_batch *b = nullptr;
-    int rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, &b);
+    int rc = stage_ext(c, sc, n, arena, arena_len, q_off, q_len, t_off, t_len, h0, nullptr, st, &b, defer_arena);
     if (rc) return rc;
     b->side_tries = tries;
     const size_t words = std::max<size_t>(n, 1) * 4;
@@ -431,24 +574,26 @@ int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out, hipStream_t st) {
 
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out) {
+              const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out, bool defer_arena) {
     if (hipSetDevice(c->device) != hipSuccess) return BWG_EIO;
     bwg_batch *b = new_batch(c, 1, n);
     if (!b) return BWG_ENOMEM;
     b->ksc = sc;
     b->d_ref = ref;
     b->hksw.resize(n);
-    for (size_t k = 0; k < n; ++k) {
-        KswJob &j = b->hksw[k];
-        j.q_off = q_off[k];
-        j.t_off = t_off[k];
-        j.qlen = q_len[k];
-        j.tlen = t_len[k];
-        j.aux0 = (uint32_t)k;
-        j.aux1 = 0;
-    }
+    parallel_for(n, host_parts(b->tu, n), [&](size_t lo, size_t hi, int) {
+        for (size_t k = lo; k < hi; ++k) {
+            KswJob &j = b->hksw[k];
+            j.q_off = q_off[k];
+            j.t_off = t_off[k];
+            j.qlen = q_len[k];
+            j.tlen = t_len[k];
+            j.aux0 = (uint32_t)k;
+            j.aux1 = 0;
+        }
+    });
     int rc = stage_buffers(c, b, arena, arena_len, b->hksw.data(), n * sizeof(KswJob),
-                           n * sizeof(bwg_ksw_result), st);
+                           n * sizeof(bwg_ksw_result), st, defer_arena);
     if (rc) {
         destroy_batch(c, b);
         return rc;
@@ -480,7 +625,7 @@ int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
     hipError_t e;
     if (b->kind == 0) {
         // {H,E} pairs over absolute columns 0..qlen per job
-        std::vector<ExtJob> jobs = b->hext;
+        std::vector<ExtJob> jobs(b->hext.begin(), b->hext.end());
         for (size_t k = 0; k < n; ++k) {
             jobs[k].aux = (uint32_t)total;
             total += (size_t)jobs[k].qlen + 1;
@@ -495,7 +640,7 @@ int prepare_plain(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
     }
     // rescue: {H,E} pairs for the padded query (+16 for the reverse pass's own padding) and one
     // int32 row maximum per target row; slots are 16-byte aligned
-    std::vector<KswJob> jobs = b->hksw;
+    std::vector<KswJob> jobs(b->hksw.begin(), b->hksw.end());
     for (size_t k = 0; k < n; ++k) {
         const size_t bytes = ((size_t)ksw_padded((int32_t)jobs[k].qlen, b->ksc.max_sc) + 16) * 8 +
                              (size_t)jobs[k].tlen * 4;
@@ -526,6 +671,8 @@ int prepare_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
 
 int enqueue_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss) {
     if (b->n == 0) return BWG_OK;
+    const int arc = wait_arena(b);
+    if (arc) return arc;
     if (runs_plain(b, form)) {
         int rc = prepare_plain(c, b, ss.main);
         if (rc) return rc;
@@ -591,6 +738,7 @@ int fetch_results(bwg_ctx *c, bwg_batch *b, void *out) {
 }
 
 void destroy_batch(bwg_ctx *c, bwg_batch *b) {
+    (void)wait_arena(b);  // a deferred upload still writing into the arena
     (void)hipSetDevice(c->device);
     c->pool.free(b->d_arena);
     c->pool.free(b->d_jobs);

@@ -10,6 +10,7 @@
 #include <memory>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "bwg_batched_kernels.h"
@@ -124,8 +125,51 @@ struct Tuning {
     bool tune_new = true;          // a batch may run the autotuner (false: it only reads the table)
     bool item_timing = false;      // batched runs time each launch with events (no graph replay)
     bool coalesce = true;          // concurrent small synchronous calls are merged (bwg_api.cpp)
+    int host_threads = 0;          // BWG_OPT_HOST_THREADS: 0 = the CPUs this process may use, at most 16
     int form = BWG_FORM_BATCHED;   // BWG_OPT_FORM
 };
+
+// Host passes over the jobs of a large call (validation, descriptors, classification, binning)
+// split into contiguous ranges over worker threads: parts(tu, n) ranges of at least kHostGrain
+// jobs, fn(begin, end, part) on part-1 new threads and the caller.
+constexpr size_t kHostGrain = 65536;
+
+// An allocator whose resize leaves new elements default-initialised (no zeroing pass): the host
+// arrays of a large call are written in full, in parallel, right after they are sized.
+template <class T>
+struct NoInitAlloc : std::allocator<T> {
+    template <class U>
+    struct rebind {
+        using other = NoInitAlloc<U>;
+    };
+    NoInitAlloc() = default;
+    template <class U>
+    NoInitAlloc(const NoInitAlloc<U> &) {}
+    template <class U>
+    void construct(U *p) {
+        ::new ((void *)p) U;
+    }
+    template <class U, class... A>
+    void construct(U *p, A &&...a) {
+        ::new ((void *)p) U(std::forward<A>(a)...);
+    }
+};
+template <class T>
+using HostVec = std::vector<T, NoInitAlloc<T>>;
+int host_parts(const Tuning &tu, size_t n);
+template <class Fn>
+void parallel_for(size_t n, int parts, Fn &&fn) {
+    if (parts <= 1) {
+        fn((size_t)0, n, 0);
+        return;
+    }
+    std::vector<std::thread> th;
+    th.reserve((size_t)parts - 1);
+    for (int q = 1; q < parts; ++q)
+        th.emplace_back([&fn, n, parts, q] { fn(n * q / parts, n * (q + 1) / parts, q); });
+    fn((size_t)0, n / parts, 0);
+    for (auto &t : th) t.join();
+}
 
 // An execution lane of the synchronous entry points: its own stream set, so that calls from
 // several host threads stage, run and copy back concurrently instead of queueing behind one
@@ -135,6 +179,20 @@ struct ExecLane {
     void *pin_out = nullptr;
     size_t pin_out_cap = 0;
     int create(int index);
+    void destroy();
+};
+
+// Pinned chunks through which a large pageable arena goes to the device (upload_pinned): two
+// buffers for each of up to four copying threads.  A context keeps a few such sets (acquire_upload)
+// and reuses them; a buffer is refilled only after its last DMA has completed.
+struct UploadStaging {
+    static constexpr int kThreads = 4;
+    static constexpr int kBufs = 2 * kThreads;
+    static constexpr size_t kChunk = 8u << 20;
+    void *buf[kBufs] = {};
+    hipEvent_t ev[kBufs] = {};
+    bool pending[kBufs] = {};
+    int create();
     void destroy();
 };
 
@@ -169,6 +227,11 @@ struct bwg_ctx {
     int max_lanes = 4;
     // timing events of the allocation-free entry point, created with the context
     hipEvent_t na_ev[2] = {nullptr, nullptr};
+    // pinned upload staging of the synchronous calls (at most kMaxUploadSets, made on demand)
+    static constexpr int kMaxUploadSets = 4;
+    std::mutex up_mu;
+    std::vector<std::unique_ptr<bwg::UploadStaging>> up_all;
+    std::vector<bwg::UploadStaging *> up_free;
     // synchronous calls in progress, and the server that merges concurrent small ones
     std::atomic<int> sync_active{0};
     std::mutex coalesce_mu;
@@ -185,11 +248,15 @@ struct bwg_ctx {
 struct bwg_batch {
     int kind = 0;  // 0 extension, 1 rescue, 2 global alignment with traceback
     size_t n = 0;
+    // a large arena's upload, issued from a helper thread while the calling thread plans the batch
+    // (synchronous entry points only); wait_arena joins it before anything reads the arena
+    std::thread arena_copy;
+    hipError_t arena_err = hipSuccess;
     bwg::ExtScoring esc{};
     bwg::KswScoring ksc{};
     // host copies of the job descriptors, used to plan the launches
-    std::vector<bwg::ExtJob> hext;
-    std::vector<bwg::KswJob> hksw;
+    bwg::HostVec<bwg::ExtJob> hext;
+    bwg::HostVec<bwg::KswJob> hksw;
     std::vector<bwg::GlobalJob> hglb;
     bwg::GlobalScoring gsc{};
     uint32_t *d_z = nullptr;      // global: traceback codes
@@ -244,12 +311,12 @@ namespace bwg {
 int stage_ext(bwg_ctx *c, const ExtScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
               const uint32_t *t_len, const int32_t *h0, const uint8_t *ref, hipStream_t st,
-              bwg_batch **out);
+              bwg_batch **out, bool defer_arena = false);
 // A one-sided extension batch with the reference's band doubling: up to `tries` rounds at bands
 // sc.w << r; prev holds each job's score before the first round.
 int stage_side(bwg_ctx *c, const ExtScoring &sc, int tries, size_t n, const uint8_t *arena, size_t arena_len,
                const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off, const uint32_t *t_len,
-               const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out);
+               const int32_t *h0, const int32_t *prev, hipStream_t st, bwg_batch **out, bool defer_arena = false);
 int fetch_side_bands(bwg_ctx *c, bwg_batch *b, int32_t *w_out, hipStream_t st);
 // A mate-rescue round: n anchors, 4n jobs whose windows the device computes against the context's
 // two-strand reference.
@@ -266,7 +333,7 @@ int fetch_cigar_lengths(bwg_ctx *c, bwg_batch *b, uint32_t *n_ops, hipStream_t s
 int fetch_cigars(bwg_ctx *c, bwg_batch *b, uint32_t *ops, size_t cap, hipStream_t st);
 int stage_ksw(bwg_ctx *c, const KswScoring &sc, size_t n, const uint8_t *arena, size_t arena_len,
               const uint64_t *q_off, const uint32_t *q_len, const uint64_t *t_off,
-              const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out);
+              const uint32_t *t_len, const uint8_t *ref, hipStream_t st, bwg_batch **out, bool defer_arena = false);
 // Host-side preparation of a batch for a form: the batched form's plan (binning, autotuning, plan
 // uploads), or the plain form's per-job scratch.  Done once; enqueue_batch does it if needed.
 int prepare_batch(bwg_ctx *c, bwg_batch *b, int form, StreamSet &ss);
@@ -287,6 +354,16 @@ ExecLane *acquire_lane(bwg_ctx *c);
 void release_lane(bwg_ctx *c, ExecLane *l);
 // Device memory of a batch: from its bump region when it has one, else from the context's pool.
 void *batch_alloc(bwg_ctx *c, bwg_batch *b, size_t bytes);
+// Joins a deferred arena upload (a no-op otherwise) and returns its status.
+int wait_arena(bwg_batch *b);
+// A free pinned upload set of the context (nullptr when all are in use), and its return.
+UploadStaging *acquire_upload(bwg_ctx *c);
+void release_upload(bwg_ctx *c, UploadStaging *u);
+void destroy_uploads(bwg_ctx *c);
+// len bytes of pageable host memory to dst through u's pinned chunks, on up to four threads, each
+// chunk's DMA enqueued on st; returns once every chunk is enqueued (they complete in stream order).
+hipError_t upload_pinned(UploadStaging &u, uint8_t *dst, const uint8_t *src, size_t len, hipStream_t st,
+                         int device);
 // The allocation-free entry point's scratch (noalloc.hip): its device address is looked up once
 // per device (noalloc_setup, with that device current); calls on one device are serialised by
 // noalloc_call_mutex, since they share the array.

@@ -148,6 +148,7 @@ enum {
     BWG_OPT_POOL_PEAK_RESET = 13, /* any value: restart BWG_STAT_POOL_PEAK at the bytes now in use */
     BWG_OPT_ITEM_TIMING = 14,     /* batched: time every launch with HIP events (replays no graph; default 0) */
     BWG_OPT_COALESCE = 15,        /* merge concurrent small synchronous calls into one device batch (default 1) */
+    BWG_OPT_HOST_THREADS = 16,    /* host threads for a call's validation and planning (0: auto, calls of 64k+ jobs; 1..16: that many, 1024+ jobs a thread) */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
@@ -157,8 +158,11 @@ enum {
     BWG_STAT_POOL_CACHED = 106,    /* device bytes freed by batches and cached for reuse */
     BWG_STAT_POOL_PEAK = 107,      /* largest BWG_STAT_POOL_IN_USE since creation or the last reset */
     /* host wall time of the phases of the last synchronous extension or rescue call, ns: host-side
-     * validation; descriptor build and host-to-device copies; planning (binning, autotuning, plan
-     * uploads); device kernels (HIP events); result copy to the host; the whole call */
+     * validation; descriptor build and host-to-device copies (a large arena uploads from a helper
+     * thread while the call plans: only the part of that upload planning did not hide counts
+     * here); planning (binning, autotuning, plan uploads); enqueueing the kernels and waiting for
+     * them; result copy to the host; the whole call.  Calls of 64k jobs or more split validation,
+     * descriptor build and binning over host threads (BWG_OPT_HOST_THREADS). */
     BWG_STAT_PHASE_VALIDATE_NS = 110,
     BWG_STAT_PHASE_STAGE_NS = 111,
     BWG_STAT_PHASE_PLAN_NS = 112,
