@@ -211,12 +211,15 @@ def _short_kernel(name):
 def _trace_name(bin_name):
     """The trace name of the kernel a bench bin label (the library's mapping name) launches."""
     import re
-    m = re.match(r"(ext_group|ext_thread|ksw_pk|ksw_group)<(\d+),i(\d+)(,reverse)?>$", bin_name)
+    m = re.match(r"(ext_group8|ext_group|ext_thread|ksw_pk|ksw_group)<(\d+),i(\d+)(,reverse)?>$", bin_name)
     if not m:
         return bin_name + "_kernel"
     fam, k, width, rev = m.group(1), int(m.group(2)), int(m.group(3)), bool(m.group(4))
-    if fam in ("ext_group", "ext_thread"):
-        return "%s_kernel<%d, %s>" % (fam, k, "true" if width == 16 else "false")
+    narrow = "true" if width == 16 else "false"
+    if fam in ("ext_group", "ext_group8"):
+        return "ext_group_kernel<%d, %s, %d>" % (k, narrow, 8 if fam == "ext_group8" else 16)
+    if fam == "ext_thread":
+        return "ext_thread_kernel<%d, %s>" % (k, narrow)
     return "%s_kernel<%d, %s>" % (fam, k // 2 if fam == "ksw_pk" else k, "true" if rev else "false")
 
 

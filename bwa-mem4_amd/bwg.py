@@ -37,6 +37,7 @@ OPT_AUTOTUNE, OPT_EXT_K_STEP, OPT_KSW_K_STEP, OPT_EXT_SCORE_WIDTH, OPT_UNGAPPED 
 OPT_KSW_SCORE_WIDTH, OPT_LANES, OPT_POOL_CACHE_MB, OPT_POOL_PEAK_RESET = 10, 11, 12, 13
 OPT_ITEM_TIMING, OPT_COALESCE = 14, 15
 OPT_HOST_THREADS = 16
+OPT_EXT_GROUP_LANES = 17
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 STAT_POOL_IN_USE, STAT_POOL_CACHED, STAT_POOL_PEAK = 105, 106, 107
 # host wall time of the phases of the last synchronous extension / rescue call (ns)

@@ -231,6 +231,10 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
         if (v < 0 || v > 16) return BWG_EINVAL;
         t.host_threads = (int)v;
         return BWG_OK;
+    case BWG_OPT_EXT_GROUP_LANES:
+        if (v != 8 && v != 16) return BWG_EINVAL;
+        t.ext_group_lanes = (int)v;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -265,6 +269,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_ITEM_TIMING: return t.item_timing;
     case BWG_OPT_COALESCE: return t.coalesce;
     case BWG_OPT_HOST_THREADS: return t.host_threads;
+    case BWG_OPT_EXT_GROUP_LANES: return t.ext_group_lanes;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns.load();

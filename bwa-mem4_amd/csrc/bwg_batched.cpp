@@ -236,6 +236,11 @@ int ext_grid(const bwg_ctx *c, const Tuning &tu, int kind, int param, bool narro
         return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ext_group_oversub,
                                                            ((int64_t)count + 15) / 16));
     }
+    if (kind == kExtGroup8) {  // 32 groups a block
+        const int per_cu = ext_group8_blocks_per_cu(param, qcap, narrow);
+        return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ext_group_oversub,
+                                                           ((int64_t)count + 31) / 32));
+    }
     const int per_cu = ext_thread_blocks_per_cu(param, narrow);
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)count + 63) / 64));
 }
@@ -311,6 +316,8 @@ std::vector<TuneCandidate> ext_candidates(int cls) {
         add(kExtGroup, kExtKs[r], 16);
         add(kExtGroup, kExtKs[r], 32);
         if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1], 16);
+        // eight jobs a wavefront, 2K + 1 columns per lane cover the same window
+        if (kExtKs[r] <= 15) add(kExtGroup8, 2 * kExtKs[r] + 1, 16);  // 16-bit state: fits LDS at qcap 512
     }
     return v;
 }
@@ -350,6 +357,9 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, const ExtScoring &esc, uint64_t sig
                 if (cd.kind == kExtGroup)
                     e = launch_ext_group(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1, S,
                                          nullptr, d, esc, qcap, (bwg_ext_result *)b->d_out, grid);
+                else if (cd.kind == kExtGroup8)
+                    e = launch_ext_group8(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1,
+                                          S, nullptr, d, esc, qcap, (bwg_ext_result *)b->d_out, grid);
                 else
                     e = launch_ext_thread(st, cd.param, narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, d + 1,
                                           S, nullptr, d, esc, (bwg_ext_result *)b->d_out, grid);
@@ -445,6 +455,9 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         } else {
             const int r = std::min(kidx[k] + (tu.ext_k_step ? 1 : 0), kNumExtKs - 1);
             kind = kExtGroup, param = kExtKs[r];
+            if (tu.ext_group_lanes == 8 && param <= 15 &&
+                ext_group8_lds_bytes(2 * param + 1, qcap_class(q), narrow) <= kMaxBlockLds)
+                kind = kExtGroup8, param = 2 * param + 1;
         }
         return ((uint32_t)kind << 8) | ((uint32_t)param << 1) | (narrow ? 1u : 0u);
     };
@@ -466,7 +479,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
             a[cd].count++;
             a[cd].work += (double)w;
-            if ((int)(cd >> 8) == kExtGroup) a[cd].qcap = std::max(a[cd].qcap, qcap_class((int)j.qlen));
+            if (is_ext_group((int)(cd >> 8))) a[cd].qcap = std::max(a[cd].qcap, qcap_class((int)j.qlen));
         }
     });
     std::vector<int16_t> bin_of_code(kCodes, -1);
@@ -483,14 +496,14 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         const int bi = find(kind, (cd >> 1) & 127, (cd & 1) != 0);
         bins[bi].count += t.count;
         bins[bi].work += t.work;
-        if (kind == kExtGroup) bins[bi].qcap = std::max(bins[bi].qcap, t.qcap);
+        if (is_ext_group(kind)) bins[bi].qcap = std::max(bins[bi].qcap, t.qcap);
         bin_of_code[cd] = (int16_t)bi;
     }
     p->order = order_jobs(n, bins, [&](size_t k) { return (uint32_t)bin_of_code[code[k]]; }, wk.data(), parts);
     for (size_t r = 0; r < bins.size(); ++r) {
         Item &it = bins[r];
         it.head = (uint32_t)r;
-        if (it.kind == kExtGroup || it.kind == kExtThread) {
+        if (is_ext_dp(it.kind)) {
             it.grid = ext_grid(c, tu, it.kind, it.param, it.narrow, it.qcap, it.count);
         } else {
             // plain fallback: compact descriptors with per-job scratch offsets
@@ -523,7 +536,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         bool any = false;
         for (int r = 0; r < p->n_bins; ++r) {
             p->meta[seg_word(p) + r] = bins[r].offset;
-            const bool dp = bins[r].kind == kExtGroup || bins[r].kind == kExtThread;
+            const bool dp = is_ext_dp(bins[r].kind);
             p->meta[seg_word(p) + p->n_bins + r] = dp ? 0u : 1u;
             any |= dp;
         }
@@ -857,6 +870,7 @@ int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
 hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
     switch (it.kind) {
     case kExtGroup:
+    case kExtGroup8:
     case kExtThread: {
         // after the ungapped pre-pass or a retry filter the bin drains its compacted queue
         const uint32_t *order = p->compact ? p->d_corder + it.offset : p->d_order + it.offset;
@@ -865,6 +879,10 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
             return launch_ext_group(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order, it.count,
                                     dcount, p->d_heads + it.head, p->esc, it.qcap, (bwg_ext_result *)b->d_out,
                                     it.grid);
+        if (it.kind == kExtGroup8)
+            return launch_ext_group8(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order,
+                                     it.count, dcount, p->d_heads + it.head, p->esc, it.qcap,
+                                     (bwg_ext_result *)b->d_out, it.grid);
         return launch_ext_thread(st, it.param, it.narrow, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, order,
                                  it.count, dcount, p->d_heads + it.head, p->esc, (bwg_ext_result *)b->d_out, it.grid);
     }
@@ -1158,7 +1176,7 @@ int batched_ungapped_hits(bwg_ctx *c, bwg_batch *b, uint64_t *hits) {
     if (e != hipSuccess) return bwg_map_hip(e);
     uint64_t eligible = 0, declined = 0;
     for (const Item &it : p->items)
-        if (it.kind == kExtGroup || it.kind == kExtThread) {
+        if (is_ext_dp(it.kind)) {
             eligible += it.count;
             declined += cnt[(size_t)kCountStride * it.head];
         }
@@ -1183,7 +1201,7 @@ std::string batched_report(bwg_batch *b) {
                 if (hipEventElapsedTime(&ms, p->ev0[k], p->ev1[k]) == hipSuccess) us = ms * 1e3;
             }
             const int width = (it.kind == kKswFwd || it.kind == kKswRev || it.kind == kGlbGroup ||
-                               it.kind == kExtGroup || it.kind == kExtThread)
+                               is_ext_dp(it.kind))
                                   ? (it.narrow ? 16 : 32)
                                   : 32;
             std::snprintf(line, sizeof line, "%d round=%d stage=%d %s jobs=%u grid=%d us=%.3f\n", idx,
@@ -1201,7 +1219,7 @@ int batched_job_bins(bwg_batch *b, int32_t *bin) {
     if (!p) return BWG_EINVAL;
     for (size_t r = 0; r < p->items.size(); ++r) {
         const Item &it = p->items[r];
-        const bool is_bin = it.kind == kExtGroup || it.kind == kExtThread || it.kind == kExtPlain ||
+        const bool is_bin = is_ext_dp(it.kind) || it.kind == kExtPlain ||
                             it.kind == kKswFwd || it.kind == kKswPlain || it.kind == kGlbGroup;
         if (!is_bin) continue;
         for (uint32_t q = 0; q < it.count; ++q) bin[p->order[it.offset + q]] = (int32_t)r;

@@ -6,18 +6,28 @@
 // Even K values (padded query columns per lane) the rescue group kernel is instantiated for.
 #define BWG_KSW_GROUP_KS BWG_K(2) BWG_K(4) BWG_K(6) BWG_K(8) BWG_K(10) BWG_K(12) BWG_K(14) BWG_K(16)
 #define BWG_EXT_GROUP_KS BWG_K(3) BWG_K(5) BWG_K(7) BWG_K(9) BWG_K(11) BWG_K(13) BWG_K(15) BWG_K(17) BWG_K(21) BWG_K(25) BWG_K(33)
+// Columns per lane of the 8-lane group kernel: 2K + 1 for the 16-lane K up to 15.
+#define BWG_EXT_GROUP8_KS BWG_K(7) BWG_K(11) BWG_K(15) BWG_K(19) BWG_K(23) BWG_K(27) BWG_K(31)
 
 namespace bwg {
 
 // Extension kernels.  The group kernel sizes its per-row block width itself (at most K columns per
 // lane); `narrow` selects 16-bit column state in both kernels (valid when h0 + qlen*max_sc <
 // 65536).
+constexpr size_t kMaxBlockLds = 160u << 10;  // LDS a gfx950 workgroup may allocate
 size_t ext_group_lds_bytes(int K, int qcap, bool narrow);
 int ext_group_blocks_per_cu(int K, int qcap, bool narrow);
 // `count` jobs of `order` (or *dcount of them when dcount is not null: a compacted queue).
 hipError_t launch_ext_group(hipStream_t st, int K, bool narrow, const uint8_t *arena, const uint8_t *ref,
                             const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
                             uint32_t *head, const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid);
+// The same kernel with 8-lane groups, eight jobs per wavefront (K up to 31 columns per lane): half
+// the per-row bookkeeping instructions per job row, and twice the block's LDS.
+size_t ext_group8_lds_bytes(int K, int qcap, bool narrow);  // at most kMaxBlockLds to launch
+int ext_group8_blocks_per_cu(int K, int qcap, bool narrow);
+hipError_t launch_ext_group8(hipStream_t st, int K, bool narrow, const uint8_t *arena, const uint8_t *ref,
+                             const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
+                             uint32_t *head, const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid);
 int ext_thread_blocks_per_cu(int qcap, bool narrow);
 hipError_t launch_ext_thread(hipStream_t st, int qcap, bool narrow, const uint8_t *arena, const uint8_t *ref,
                              const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,

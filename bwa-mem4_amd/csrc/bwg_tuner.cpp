@@ -33,6 +33,7 @@ const char *mapping_name(int kind, int param, int width, char *buf, size_t len) 
     switch (kind) {
     case kExtThread: std::snprintf(buf, len, "ext_thread<%d,i%d>", param, width); break;
     case kExtGroup: std::snprintf(buf, len, "ext_group<%d,i%d>", param, width); break;
+    case kExtGroup8: std::snprintf(buf, len, "ext_group8<%d,i%d>", param, width); break;
     case kKswFwd: std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16>" : "ksw_group<%d,i32>", param); break;
     case kKswRev:
         std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16,reverse>" : "ksw_group<%d,i32,reverse>", param);

@@ -15,7 +15,12 @@ namespace bwg {
 
 // Kernel mappings a bin can be run with.
 enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain, kExtUngapped,
-                kExtRetryFilter, kKswMateRc, kKswWindows, kGlbGroup, kGlbTrace };
+                kExtRetryFilter, kKswMateRc, kKswWindows, kGlbGroup, kGlbTrace, kExtGroup8 };
+
+// The extension's 16- and 8-lane group mappings.
+inline bool is_ext_group(int kind) { return kind == kExtGroup || kind == kExtGroup8; }
+// Bins drained by the extension DP kernels (group or thread mappings).
+inline bool is_ext_dp(int kind) { return is_ext_group(kind) || kind == kExtThread; }
 
 struct TuneCandidate {
     int kind = 0;

@@ -15,6 +15,9 @@
 //    (F(l) = max_{m<l} c_m - (l-1-m)*kr*e_ins), and a second pass applies each lane's incoming F.
 //    Row maximum with the later-column tie rule is a max over keys (h << 13 | j).  The row
 //    epilogue (to-end score, z-drop, band tightening) is evaluated redundantly by the 16 lanes.
+//    The template's GL = 8 runs the same row with 8-lane groups, two per DPP row interleaved by
+//    lane parity (GroupLanes<8>), so eight jobs share a wavefront and its per-row bookkeeping; its
+//    K (up to 31) is the columns per lane for the same windows.  It is an autotuner candidate.
 //
 //  * ext_thread_kernel<QCAP>: one job per lane for short queries (qlen <= QCAP), the scalar
 //    recurrence with the eh arrays in LDS laid out [column][lane].
@@ -132,8 +135,8 @@ template <> struct EhView<true> {
 // the last group's loads inside the allocation.
 __host__ __device__ __forceinline__ int eh_bytes(int qcap, bool narrow) { return (narrow ? 4 : 8) * (qcap + 2); }
 __host__ __device__ __forceinline__ int plane_bytes(int qcap) { return 5 * qcap; }
-__host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap, int K, bool narrow) {
-    return (size_t)groups * (eh_bytes(qcap, narrow) + plane_bytes(qcap)) + 16 * K * 8 + 64;
+__host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int K, bool narrow) {
+    return (size_t)(256 / GL) * (eh_bytes(qcap, narrow) + plane_bytes(qcap)) + (size_t)GL * K * 8 + 64;
 }
 
 // The cells of one DP row for one group, with the row's window spread over the 16 lanes in blocks
@@ -141,9 +144,10 @@ __host__ __device__ __forceinline__ size_t block_lds_bytes(int groups, int qcap,
 // the lane's row-maximum key (h << 13 | absolute column), 0 when the lane has no window column.
 // Only window columns store: the columns past the window keep their stale eh values, which a
 // later row reads when the band widens again (by up to two columns a row), as the reference's do.
-template <int KR, class EH>
+template <int KR, int GL, class EH>
 __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t beg, int32_t end, int g,
                                              int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
+    using L = GroupLanes<GL>;
     const int32_t base = beg + g * KR;  // first window column of this lane
     const int32_t nval = min(max(end - base, 0), KR);
     const int8_t *sb = srow + base;
@@ -176,8 +180,8 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
     // F entering this lane's block: exclusive max-scan of U_l = c_l + l*KR*e_ins over the row
     const int32_t Ke = KR * e_ins;
     const int32_t U = grun + g * Ke;
-    const int32_t incl = row_scan_max_nonneg(U);
-    const int32_t excl = row_shift_in(incl, -Ke);
+    const int32_t incl = L::scan_max_nonneg(U);
+    const int32_t excl = L::shift_in(incl, -Ke);
     const int32_t f_in = excl - (g - 1) * Ke;
 
     int32_t key = 0;
@@ -199,23 +203,27 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
 
 // row_cells at the odd block width kr (lo <= kr <= hi, both odd), chosen by a binary search
 // over the wavefront-uniform kr so that each row runs one straight-line body.
-template <int LO, int HI, class EH>
+template <int LO, int HI, int GL, class EH>
 __device__ __forceinline__ int32_t row_cells_at(int kr, const EH &eh, const int8_t *srow, int32_t beg, int32_t end,
                                                 int g, int32_t e_ins, int32_t oe_ins, int32_t e_del,
                                                 int32_t oe_del) {
     if constexpr (LO >= HI) {
-        return row_cells<HI>(eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells<HI, GL>(eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
     } else {
         constexpr int MID = LO + ((HI - LO) / 4) * 2;  // odd, LO <= MID < HI
         if (kr <= MID)
-            return row_cells_at<LO, MID>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
-        return row_cells_at<MID + 2, HI>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+            return row_cells_at<LO, MID, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells_at<MID + 2, HI, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
     }
 }
 
-constexpr int kGroupsPerBlock = 16;
+// Groups of a 256-thread block: 16 of 16 lanes, or 32 of 8 lanes.
+template <int GL>
+constexpr int groups_per_block() {
+    return 256 / GL;
+}
 
-template <int K, bool NARROW>
+template <int K, bool NARROW, int GL>
 __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restrict__ arena,
                                                         const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
@@ -227,18 +235,18 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
     const uint32_t count = dcount ? *dcount : count_static;  // a compacted queue's length is on the device
     // an empty queue (a retry round that retries nothing) ends the grid before any queue atomic
     if (count == 0) return;
-    const int lane = threadIdx.x & 63;
-    const int g = lane & 15;             // lane within the group
-    const int gid = threadIdx.x >> 4;    // group within the block
+    using L = GroupLanes<GL>;
+    const int g = L::g();      // lane within the group
+    const int gid = L::gid();  // group within the block
     const EhView<NARROW> eh((char *)lds + gid * eh_bytes(qcap, NARROW), qcap);
-    int8_t *planes = (int8_t *)lds + kGroupsPerBlock * eh_bytes(qcap, NARROW) + gid * plane_bytes(qcap);
+    int8_t *planes = (int8_t *)lds + groups_per_block<GL>() * eh_bytes(qcap, NARROW) + gid * plane_bytes(qcap);
 
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
 
     GroupJob J = {};
     bool active = false, exhausted = false;
-    GroupSlots slots(count, true);
+    GroupSlotsT<GL> slots(count, true);
     for (;;) {
         // ---- fetch and initialise a job for idle groups ------------------------------------
         if (!active && !exhausted) {
@@ -255,11 +263,11 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 J.out_idx = order[slot];
                 J.w = clamp_band(sc.w, J.qlen, sc);
                 // row -1 (sw.rs:263-279): H[0] = h0, H[j] = max(h0 - oe_ins - (j-1)*e_ins, 0)
-                for (int j = g; j <= J.qlen + 1; j += 16) {
+                for (int j = g; j <= J.qlen + 1; j += GL) {
                     const int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
                     eh.init(j, j <= J.qlen ? h : 0);
                 }
-                for (int j = g; j < J.qlen; j += 16) {
+                for (int j = g; j < J.qlen; j += GL) {
                     const int qc = q[j];
 #pragma unroll
                     for (int c = 0; c < 5; ++c) planes[c * qcap + j] = sc.mat[c * 5 + qc];
@@ -293,8 +301,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         // possible window).  The band narrows as rows go by, and the wavefront's time per row
         // follows its widest group, so the row's columns are spread over every lane again.
         const int32_t wwin = active ? max(end - beg, 0) : 0;
-        const int32_t wmax = wave_max_of_rows(wwin);
-        const int32_t kr = min(((wmax + 15) >> 4) | 1, K);
+        const int32_t wmax = L::wave_max(wwin);
+        const int32_t kr = min(((wmax + GL - 1) / GL) | 1, K);
         if (!active) continue;
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
         const int8_t *srow = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap);
@@ -303,13 +311,13 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         // would wait for it too, a full memory round trip every row.
         asm volatile("" ::"v"(J.tcur) : "memory");
         const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
-        int32_t key = row_cells_at<1, K>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        int32_t key = row_cells_at<1, K, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
         if (g == 0) {
             eh.set_h(beg, h1_init);  // H(i, beg-1)
             eh.set_e(end, 0);        // E at the sentinel column
         }
         wave_lds_sync();
-        key = row_allmax(key);  // h << 13 | absolute column
+        key = L::allmax(key);  // h << 13 | absolute column
         const int32_t row_max = key >> kKeyShift;
         const int32_t mj = key & kKeyMask;
         // the edge slots: end holds H(i, end-1) (h1_init if the window is empty) and the E just
@@ -349,9 +357,9 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                         const int2 x = eh.load(j);
                         live = (x.x | x.y) != 0;
                     }
-                    const int32_t cand = row_allmin(live ? j : 0x7fffffff);
+                    const int32_t cand = L::allmin(live ? j : 0x7fffffff);
                     if (cand != 0x7fffffff) { nb = cand; break; }
-                    nb += 16;
+                    nb += GL;
                     if (nb >= end) { nb = end; break; }
                 }
             }
@@ -364,9 +372,9 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                         const int2 x = eh.load(j);
                         live = (x.x | x.y) != 0;
                     }
-                    const int32_t cand = row_allmax(live ? j : -1);
+                    const int32_t cand = L::allmax(live ? j : -1);
                     if (cand >= 0) { ne = cand; break; }
-                    ne -= 16;
+                    ne -= GL;
                     if (ne < nb) { ne = nb - 1; break; }
                 }
             }
@@ -767,26 +775,26 @@ __global__ __launch_bounds__(256) void ext_retry_filter_kernel(
     }
 }
 
-template <int K, bool NARROW>
+template <int K, bool NARROW, int GL>
 hipError_t launch_group(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs, const uint32_t *order,
                         uint32_t count, const uint32_t *dcount, uint32_t *head, const ExtScoring &sc, int qcap,
                         bwg_ext_result *out, int grid) {
-    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K, NARROW);
-    hipLaunchKernelGGL((ext_group_kernel<K, NARROW>), dim3(grid), dim3(256), lds, st, arena, ref, jobs, order, count,
-                       dcount, head, sc, qcap, out);
+    const size_t lds = block_lds_bytes(GL, qcap, K, NARROW);
+    hipLaunchKernelGGL((ext_group_kernel<K, NARROW, GL>), dim3(grid), dim3(256), lds, st, arena, ref, jobs, order,
+                       count, dcount, head, sc, qcap, out);
     return hipGetLastError();
 }
 
 // Occupancy queries are cached: they depend only on the kernel and its LDS size.
-template <int K, bool NARROW>
+template <int K, bool NARROW, int GL>
 int group_occupancy(int qcap) {
     static std::atomic<int> cache[130];
     const int slot = std::min(qcap / 32, 129);
     int v = cache[slot].load(std::memory_order_relaxed);
     if (v > 0) return v;
     int blocks = 0;
-    const size_t lds = block_lds_bytes(kGroupsPerBlock, qcap, K, NARROW);
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K, NARROW>, 256, lds) != hipSuccess)
+    const size_t lds = block_lds_bytes(GL, qcap, K, NARROW);
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ext_group_kernel<K, NARROW, GL>, 256, lds) != hipSuccess)
         blocks = 1;
     v = blocks > 0 ? blocks : 1;
     cache[slot].store(v, std::memory_order_relaxed);
@@ -808,16 +816,38 @@ int thread_occupancy() {
 
 }  // namespace
 
-size_t ext_group_lds_bytes(int K, int qcap, bool narrow) {
-    return block_lds_bytes(kGroupsPerBlock, qcap, K, narrow);
-}
+size_t ext_group_lds_bytes(int K, int qcap, bool narrow) { return block_lds_bytes(16, qcap, K, narrow); }
+size_t ext_group8_lds_bytes(int K, int qcap, bool narrow) { return block_lds_bytes(8, qcap, K, narrow); }
 
 int ext_group_blocks_per_cu(int K, int qcap, bool narrow) {
     switch (K) {
-#define BWG_K(k) case k: return narrow ? group_occupancy<k, true>(qcap) : group_occupancy<k, false>(qcap);
+#define BWG_K(k) case k: return narrow ? group_occupancy<k, true, 16>(qcap) : group_occupancy<k, false, 16>(qcap);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return 1;
+    }
+}
+
+int ext_group8_blocks_per_cu(int K, int qcap, bool narrow) {
+    switch (K) {
+#define BWG_K(k) case k: return narrow ? group_occupancy<k, true, 8>(qcap) : group_occupancy<k, false, 8>(qcap);
+        BWG_EXT_GROUP8_KS
+#undef BWG_K
+    default: return 1;
+    }
+}
+
+hipError_t launch_ext_group8(hipStream_t st, int K, bool narrow, const uint8_t *arena, const uint8_t *ref,
+                             const ExtJob *jobs, const uint32_t *order, uint32_t count, const uint32_t *dcount,
+                             uint32_t *head, const ExtScoring &sc, int qcap, bwg_ext_result *out, int grid) {
+    switch (K) {
+#define BWG_K(k)                                                                                                   \
+    case k:                                                                                                        \
+        return narrow ? launch_group<k, true, 8>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid) \
+                      : launch_group<k, false, 8>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid);
+        BWG_EXT_GROUP8_KS
+#undef BWG_K
+    default: return hipErrorInvalidValue;
     }
 }
 
@@ -827,8 +857,8 @@ hipError_t launch_ext_group(hipStream_t st, int K, bool narrow, const uint8_t *a
     switch (K) {
 #define BWG_K(k)                                                                                              \
     case k:                                                                                                   \
-        return narrow ? launch_group<k, true>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid) \
-                      : launch_group<k, false>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid);
+        return narrow ? launch_group<k, true, 16>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid) \
+                      : launch_group<k, false, 16>(st, arena, ref, jobs, order, count, dcount, head, sc, qcap, out, grid);
         BWG_EXT_GROUP_KS
 #undef BWG_K
     default: return hipErrorInvalidValue;

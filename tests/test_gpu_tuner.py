@@ -30,20 +30,28 @@ def test_forced_extension_mappings_agree(gpu_ctx):
     gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
     try:
         # thread kernel up to q=32 / up to q=16 / never; group kernels at the minimal K or one
-        # above; 16- and 32-bit column state
-        for tmax, kstep, width in [(32, 0, 16), (16, 0, 16), (0, 0, 16), (32, 1, 16), (0, 1, 16),
-                                   (32, 0, 32), (0, 1, 32)]:
-            _say("forced mapping thread_max_q=%d k_step=%d width=%d" % (tmax, kstep, width))
+        # above, with 16 or 8 lanes per job; 16- and 32-bit column state
+        for tmax, kstep, width, lanes in [(32, 0, 16, 16), (16, 0, 16, 16), (0, 0, 16, 16), (32, 1, 16, 16),
+                                          (0, 1, 16, 16), (32, 0, 32, 16), (0, 1, 32, 16), (32, 0, 16, 8),
+                                          (0, 0, 16, 8), (0, 1, 16, 8), (0, 0, 32, 8), (32, 1, 32, 8)]:
+            _say("forced mapping thread_max_q=%d k_step=%d width=%d lanes=%d" % (tmax, kstep, width, lanes))
             gpu_ctx.set_option(bwg.OPT_EXT_THREAD_MAX_Q, tmax)
             gpu_ctx.set_option(bwg.OPT_EXT_K_STEP, kstep)
             gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, width)
+            gpu_ctx.set_option(bwg.OPT_EXT_GROUP_LANES, lanes)
             gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
             got = gpu_ctx.extend_arrays(batch, params)
-            assert _first_bad(got, want) is None, (tmax, kstep, width, _first_bad(got, want))
+            assert _first_bad(got, want) is None, (tmax, kstep, width, lanes, _first_bad(got, want))
+            if lanes == 8 and tmax == 0:
+                st = gpu_ctx.stage(batch, params)
+                st.run()
+                assert any("ext_group8<" in r["kernel"] for r in st.report())
+                st.close()
     finally:
         gpu_ctx.set_option(bwg.OPT_EXT_THREAD_MAX_Q, 32)
         gpu_ctx.set_option(bwg.OPT_EXT_K_STEP, 0)
         gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, 16)
+        gpu_ctx.set_option(bwg.OPT_EXT_GROUP_LANES, 16)
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
@@ -75,8 +83,8 @@ def _long_query_batch(n, rng, qlo, qhi, h0_of):
     return bwg.JobBatch.from_pairs(qs, ts, np.asarray(h0s, dtype=np.int32))
 
 
-@pytest.mark.parametrize("width", [16, 32])
-def test_long_queries_and_state_width_boundary(gpu_ctx, width):
+@pytest.mark.parametrize("width,lanes", [(16, 16), (32, 16), (16, 8), (32, 8)])
+def test_long_queries_and_state_width_boundary(gpu_ctx, width, lanes):
     """Queries up to 500 columns in the group kernels (windows up to 33 columns per lane at w=250),
     with h0 + qlen*max_sc around 65536, where 16-bit column state stops being exact and the
     planner switches those jobs to 32-bit state."""
@@ -91,13 +99,15 @@ def test_long_queries_and_state_width_boundary(gpu_ctx, width):
     gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
     try:
         gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, width)
+        gpu_ctx.set_option(bwg.OPT_EXT_GROUP_LANES, lanes)
         for bi, (batch, w) in enumerate(batches):
-            _say("long queries batch %d w=%d width=%d" % (bi, w, width))
+            _say("long queries batch %d w=%d width=%d lanes=%d" % (bi, w, width, lanes))
             got = gpu_ctx.extend_arrays(batch, bwg.ext_params(5, mat, 6, 1, 6, 1, w, 5, 100))
             want = ob.extend_batch(batch, mat, 6, 1, 6, 1, w, 5, 100)
             assert _first_bad(got, want) is None, (bi, _first_bad(got, want))
     finally:
         gpu_ctx.set_option(bwg.OPT_EXT_SCORE_WIDTH, 16)
+        gpu_ctx.set_option(bwg.OPT_EXT_GROUP_LANES, 16)
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
