@@ -412,12 +412,15 @@ def summarize_counters(paths, segments_json, traffic_out=""):
 
     out.append("Per launch, mean over the launches in the timed and serial segments.  fetch / write: scaled "
                "HBM MiB.  VALUBusy and LDSBankConflict: rocprofv3's derived metrics (percent of GPU time "
-               "the VALUs issue / LDS is stalled by bank conflicts); busy (raw): 100 x SQ_ACTIVE_INST_VALU / "
-               "(256 CUs x GRBM_GUI_ACTIVE) from the raw counters as the CSV reports them.  VALU util: "
-               "SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), active lanes per VALU instruction "
-               "cycle.  LDS conflict / active: SQ_LDS_BANK_CONFLICT / SQ_LDS_IDX_ACTIVE.")
+               "the VALUs issue / LDS is stalled by bank conflicts).  LDS busy: 100 x SQ_LDS_IDX_ACTIVE / (256 "
+               "CUs x GPU cycles), the share of cycles the LDS serves indexed accesses, conflicts included; "
+               "the CSV's raw GRBM_GUI_ACTIVE is the sum over the 8 XCDs, so GPU cycles are a 1/8 of it (the "
+               "derived VALUBusy takes the per-XCD maximum, and equals 100 x SQ_ACTIVE_INST_VALU / (256 x "
+               "GRBM_GUI_ACTIVE / 8) here).  VALU util: SQ_THREAD_CYCLES_VALU / (64 x SQ_ACTIVE_INST_VALU), "
+               "active lanes per VALU instruction cycle.  LDS conflict / active: SQ_LDS_BANK_CONFLICT / "
+               "SQ_LDS_IDX_ACTIVE.")
     out.append("")
-    out.append("| kernel | launches | fetch MiB | write MiB | VALU insts | VALUBusy % | busy (raw) % | VALU util % | "
+    out.append("| kernel | launches | fetch MiB | write MiB | VALU insts | VALUBusy % | LDS busy % | VALU util % | "
                "LDSBankConflict % | LDS conflict / active % |")
     out.append("|---|---:|---:|---:|---:|---:|---:|---:|---:|---:|")
     for k, lst in sorted(by.items(), key=lambda kv: kv[0]):
@@ -426,9 +429,9 @@ def summarize_counters(paths, segments_json, traffic_out=""):
         wr = mean(lst, "WRITE_SIZE", (wscale or 0) / 1024.0) if wscale else None
         ai, tc = mean(lst, "SQ_ACTIVE_INST_VALU"), mean(lst, "SQ_THREAD_CYCLES_VALU")
         util = 100.0 * tc / (64.0 * ai) if ai and tc is not None else None
-        gui = mean(lst, "GRBM_GUI_ACTIVE")
-        raw = 100.0 * ai / (256.0 * gui) if ai is not None and gui else None
         lb, la = mean(lst, "SQ_LDS_BANK_CONFLICT"), mean(lst, "SQ_LDS_IDX_ACTIVE")
+        gui = mean(lst, "GRBM_GUI_ACTIVE")
+        raw = 100.0 * la / (256.0 * gui / 8.0) if la is not None and gui else None
         lr = 100.0 * lb / la if la and lb is not None else None
         out.append("| `%s` | %d | %s | %s | %s | %s | %s | %s | %s | %s |" % (
             k, n_launch, fmt(fe, "%.2f"), fmt(wr, "%.2f"), fmt(mean(lst, "SQ_INSTS_VALU"), "%.3g"),
