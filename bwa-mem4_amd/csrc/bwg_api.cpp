@@ -368,7 +368,11 @@ int check_global(const bwg_global_params *p, size_t n, const uint8_t *arena, siz
     rc = check_jobs(n, arena, arena_len, q_off, q_len, t_off, t_len, 1);
     if (rc) return rc;
     for (size_t k = 0; k < n; ++k) {
-        if (q_len[k] > 512) return BWG_E2BIG;
+        if (q_len[k] > kMaxJobLen || t_len[k] > kMaxJobLen) return BWG_E2BIG;
+        // a query past the group kernel's columns runs one job per thread with a code byte per band
+        // cell: at most 1 GiB of them a job
+        const uint64_t ncol = std::min<uint64_t>(q_len[k], 2 * (uint64_t)std::max(w[k], 0) + 1);
+        if (q_len[k] > kGlbGroupMaxQ && (uint64_t)t_len[k] * ncol > (1ull << 30)) return BWG_E2BIG;
         // the band must reach the last cell (cigar.rs:441-443 guarantees it)
         if ((int64_t)w[k] < std::abs((int64_t)q_len[k] - (int64_t)t_len[k])) return BWG_EINVAL;
         // gap costs of a whole side stay far from MINUS_INF

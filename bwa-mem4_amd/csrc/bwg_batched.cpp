@@ -810,19 +810,23 @@ int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     const size_t n = b->n;
     std::vector<Item> bins;
     std::vector<GlobalJob> jobs = b->hglb;
+    // queries beyond the group kernel's LDS columns, or bands wider than its widest K, run one job
+    // per thread (k = 0 marks them for the traceback kernel, which skips them)
     auto key_of = [&](size_t k) {
         GlobalJob &j = jobs[k];
         const int64_t ncol = std::min<int64_t>(j.qlen, 2 * (int64_t)std::max(j.w, 0) + 1);
-        const int K = odd_k_for((int)std::max<int64_t>(ncol, 1));
-        j.k = (uint32_t)K;
+        const int K = j.qlen > kGlbGroupMaxQ ? -1 : odd_k_for((int)std::max<int64_t>(ncol, 1));
+        j.k = K < 0 ? 0u : (uint32_t)K;
+        const int kind = K < 0 ? kGlbPlain : kGlbGroup;
+        const int param = K < 0 ? 0 : K;
         for (size_t r = 0; r < bins.size(); ++r)
-            if (bins[r].param == K) {
+            if (bins[r].kind == kind && bins[r].param == param) {
                 bins[r].qcap = std::max(bins[r].qcap, qcap_class((int)j.qlen));
                 return (int)r;
             }
         Item it;
-        it.kind = kGlbGroup;
-        it.param = K;
+        it.kind = kind;
+        it.param = param;
         it.qcap = qcap_class((int)j.qlen);
         bins.push_back(it);
         return (int)bins.size() - 1;
@@ -832,9 +836,15 @@ int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     uint64_t zw = 0, ow = 0;
     for (size_t k = 0; k < n; ++k) {
         GlobalJob &j = jobs[k];
+        if (j.k == 0) zw = (zw + 1) & ~1ull;  // its {H, E} pairs 8-byte aligned
         j.z_off = zw;
         j.ops_off = ow;
-        zw += (uint64_t)((j.tlen + 7) & ~7u) * 16 * ((j.k + 7) / 8);  // rows in tiles of 8
+        if (j.k == 0) {  // one job per thread: {H, E} columns, then a code byte per band cell
+            const uint64_t ncol = std::min<uint64_t>(j.qlen, 2 * (uint64_t)std::max(j.w, 0) + 1);
+            zw += 2 * ((uint64_t)j.qlen + 2) + ((uint64_t)j.tlen * ncol + 3) / 4;
+        } else {
+            zw += (uint64_t)((j.tlen + 7) & ~7u) * 16 * ((j.k + 7) / 8);  // rows in tiles of 8
+        }
         ow += (uint64_t)j.qlen + j.tlen + 2;
     }
     std::vector<Item> items;
@@ -842,9 +852,14 @@ int plan_glb(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         Item it = bins[r];
         it.head = (uint32_t)r;
         it.stage = 0;
-        it.narrow = glb_scores_packable(b->gsc);  // the packed-score kernel form
-        const int per_cu = glb_group_blocks_per_cu(it.param, it.qcap, it.narrow);
-        it.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus, ((int64_t)it.count + 15) / 16));
+        if (it.kind == kGlbPlain) {
+            it.grid = (int)std::max<int64_t>(1, ((int64_t)it.count + 63) / 64);
+        } else {
+            it.narrow = glb_scores_packable(b->gsc);  // the packed-score kernel form
+            const int per_cu = glb_group_blocks_per_cu(it.param, it.qcap, it.narrow);
+            it.grid = (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus,
+                                                               ((int64_t)it.count + 15) / 16));
+        }
         items.push_back(it);
     }
     Item tr;
@@ -916,6 +931,9 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
         return launch_glb_group(st, it.param, it.narrow, b->d_arena, b->d_ref, (const GlobalJob *)p->d_jobs,
                                 p->d_order + it.offset, it.count, p->d_heads + it.head, b->gsc, it.qcap, b->d_z,
                                 (int32_t *)b->d_out, it.grid);
+    case kGlbPlain:
+        return launch_glb_plain(st, b->d_arena, b->d_ref, (const GlobalJob *)p->d_jobs, p->d_order + it.offset,
+                                it.count, b->gsc, b->d_z, b->d_ops, b->d_nops, (int32_t *)b->d_out);
     case kGlbTrace:
         return launch_glb_traceback(st, (const GlobalJob *)p->d_jobs, it.count, b->d_z, b->d_ops, b->d_nops);
     case kKswMateRc:
@@ -1220,7 +1238,8 @@ int batched_job_bins(bwg_batch *b, int32_t *bin) {
     for (size_t r = 0; r < p->items.size(); ++r) {
         const Item &it = p->items[r];
         const bool is_bin = is_ext_dp(it.kind) || it.kind == kExtPlain ||
-                            it.kind == kKswFwd || it.kind == kKswPlain || it.kind == kGlbGroup;
+                            it.kind == kKswFwd || it.kind == kKswPlain || it.kind == kGlbGroup ||
+                            it.kind == kGlbPlain;
         if (!is_bin) continue;
         for (uint32_t q = 0; q < it.count; ++q) bin[p->order[it.offset + q]] = (int32_t)r;
     }

@@ -106,6 +106,13 @@ hipError_t launch_glb_traceback(hipStream_t st, const GlobalJob *jobs, uint32_t 
                                 uint32_t *n_ops);
 hipError_t launch_glb_cigar_gather(hipStream_t st, const GlobalJob *jobs, uint32_t n, const uint32_t *ops,
                                    const uint32_t *n_ops, const uint64_t *dst_off, uint32_t *dst);
+// Long queries (k == 0 in their GlobalJob): one job per thread over `count` jobs of `order`, DP and
+// traceback both, in a z region of 2*(qlen + 2) words plus tlen*min(qlen, 2w + 1) code bytes.
+hipError_t launch_glb_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const GlobalJob *jobs,
+                            const uint32_t *order, uint32_t count, const GlobalScoring &sc, uint32_t *z,
+                            uint32_t *ops, uint32_t *n_ops, int32_t *score);
+// The widest query the global group kernel keeps in LDS.
+constexpr uint32_t kGlbGroupMaxQ = 512;
 
 // Mate rescue with device-resident windows (rescue_windows.hip).  An anchor as the device reads it:
 // the bwg_mate_anchor fields plus where its mate's reverse complement is written in the arena.

@@ -45,6 +45,7 @@ const char *mapping_name(int kind, int param, int width, char *buf, size_t len) 
     case kKswWindows: std::snprintf(buf, len, "matesw_windows"); break;
     case kGlbGroup: std::snprintf(buf, len, width == 16 ? "glb_group<%d,packed>" : "glb_group<%d>", param); break;
     case kGlbTrace: std::snprintf(buf, len, "glb_traceback"); break;
+    case kGlbPlain: std::snprintf(buf, len, "glb_plain"); break;
     case kExtPlain: std::snprintf(buf, len, "ext_plain"); break;
     case kKswPlain: std::snprintf(buf, len, "ksw_plain"); break;
     default: std::snprintf(buf, len, "?"); break;

@@ -15,7 +15,7 @@ namespace bwg {
 
 // Kernel mappings a bin can be run with.
 enum ItemKind { kExtGroup, kExtThread, kExtPlain, kKswFwd, kKswTracker, kKswRev, kKswPlain, kExtUngapped,
-                kExtRetryFilter, kKswMateRc, kKswWindows, kGlbGroup, kGlbTrace, kExtGroup8 };
+                kExtRetryFilter, kKswMateRc, kKswWindows, kGlbGroup, kGlbTrace, kExtGroup8, kGlbPlain };
 
 // The extension's 16- and 8-lane group mappings.
 inline bool is_ext_group(int kind) { return kind == kExtGroup || kind == kExtGroup8; }

@@ -265,6 +265,7 @@ __global__ __launch_bounds__(256) void glb_traceback_kernel(const GlobalJob *__r
     const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
     if (k >= n) return;
     const GlobalJob J = jobs[k];
+    if (J.k == 0) return;  // a long query: glb_plain_kernel walked its own codes
     const int32_t w = max(J.w, 0), qlen = (int32_t)J.qlen, tlen = (int32_t)J.tlen;
     const int32_t K = (int32_t)J.k, dw = (K + 7) / 8;
     const uint32_t *zj = z + J.z_off;
@@ -314,6 +315,103 @@ __global__ __launch_bounds__(256) void glb_traceback_kernel(const GlobalJob *__r
     if (c >= 0) push(1, (uint32_t)(c + 1));
     if (cnt > 0) o[cnt - 1] = last;
     n_ops[k] = cnt;
+}
+
+// ---- long queries: one job per thread -------------------------------------------------------
+// Queries beyond the group kernel's LDS columns (more than 512 bases, or a band wider than its
+// widest K) run the same recurrence one job per thread, with H and E over absolute columns and one
+// code byte per band cell in the job's region of z, and then walk their own codes back, leaving
+// the runs back to front at ops + ops_off as glb_traceback_kernel does.  A code byte holds in bits
+// 0-1 how H was reached (0 M, 1 E, 2 F; ties M over E over F), in bit 2 that the deletion
+// continues and in bit 5 that the insertion continues, so that (code >> 2*state) & 3 is the walk's
+// next state from state 0 (H), 1 (in a deletion) or 2 (in an insertion), sw.rs:650-688.
+// Region of a job in z: (qlen + 2) {H, E} pairs, then tlen rows of min(qlen, 2w + 1) code bytes.
+__global__ __launch_bounds__(64) void glb_plain_kernel(const uint8_t *__restrict__ arena,
+                                                       const uint8_t *__restrict__ ref,
+                                                       const GlobalJob *__restrict__ jobs,
+                                                       const uint32_t *__restrict__ order, uint32_t count,
+                                                       GlobalScoring sc, uint32_t *__restrict__ z,
+                                                       uint32_t *__restrict__ ops, uint32_t *__restrict__ n_ops,
+                                                       int32_t *__restrict__ score) {
+    const uint32_t slot = blockIdx.x * blockDim.x + threadIdx.x;
+    if (slot >= count) return;
+    const uint32_t idx = order[slot];
+    const GlobalJob J = jobs[idx];
+    const uint8_t *q = arena + J.q_off;
+    const TargetView t = target_view(arena, ref, J.t_off, J.tlen);
+    const int32_t qlen = (int32_t)J.qlen, tlen = (int32_t)J.tlen, w = max(J.w, 0);
+    const int32_t ncol = min(qlen, 2 * w + 1);
+    const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
+    int2 *eh = (int2 *)(z + J.z_off);
+    uint8_t *codes = (uint8_t *)(eh + qlen + 2);
+    // row -1 (sw.rs:614-621): a leading insertion within the band, nothing open
+    for (int32_t j = 0; j <= qlen; ++j)
+        eh[j] = make_int2(j == 0 ? 0 : (j <= w ? -(sc.o_ins + sc.e_ins * j) : kMinusInf), kMinusInf);
+    for (int32_t i = 0; i < tlen; ++i) {
+        const int32_t beg = max(i - w, 0), end = min(i + w + 1, qlen);
+        const int8_t *row = sc.mat + 5 * (int32_t)t[i];
+        uint8_t *zr = codes + (size_t)i * (size_t)ncol;  // cell (i, j) at zr[j - beg]
+        int32_t h_left = beg == 0 ? -(sc.o_del + sc.e_del * (i + 1)) : kMinusInf;  // H(i, j-1)
+        int32_t f = kMinusInf;
+        for (int32_t j = beg; j < end; ++j) {
+            const int2 x = eh[j];  // H(i-1, j-1) and E(i, j)
+            const int32_t m = x.x + row[q[j]];
+            uint32_t code = m < x.y ? 1u : 0u;
+            int32_t h = max(m, x.y);
+            if (h < f) {
+                code = 2u;
+                h = f;
+            }
+            const int32_t e_ext = x.y - sc.e_del, e_open = m - oe_del;
+            const int32_t f_ext = f - sc.e_ins, f_open = m - oe_ins;
+            if (e_ext > e_open) code |= 4u;
+            if (f_ext > f_open) code |= 32u;
+            eh[j] = make_int2(h_left, max(e_ext, e_open));  // H(i, j-1): the next row's diagonal
+            f = max(f_ext, f_open);
+            h_left = h;
+            zr[j - beg] = (uint8_t)code;
+        }
+        eh[end] = make_int2(h_left, kMinusInf);
+    }
+    score[idx] = eh[qlen].x;
+    // the walk back from the last cell of the band (sw.rs:650-688), runs merged as push_cigar does
+    uint32_t *o = ops + J.ops_off;
+    uint32_t cnt = 0, last = 0;  // `last` is the open run, not yet stored
+    auto push = [&](uint32_t op, uint32_t len) {
+        if (cnt > 0 && (last & 0xfu) == op) {
+            last += len << 4;
+        } else {
+            if (cnt > 0) o[cnt - 1] = last;
+            last = (len << 4) | op;
+            ++cnt;
+        }
+    };
+    int32_t i = tlen - 1, c = min(tlen - 1 + w + 1, qlen) - 1;
+    uint32_t state = 0;
+    while (i >= 0 && c >= 0) {
+        const int32_t off = c - max(i - w, 0);
+        if (off < 0 || off >= ncol) {  // off the band: only without a path inside it
+            n_ops[idx] = kTracebackOffBand;
+            return;
+        }
+        const uint32_t code = codes[(size_t)i * (size_t)ncol + (size_t)off];
+        state = (code >> (2 * state)) & 3u;
+        if (state == 0) {
+            push(0, 1);
+            --i;
+            --c;
+        } else if (state == 1) {
+            push(2, 1);
+            --i;
+        } else {
+            push(1, 1);
+            --c;
+        }
+    }
+    if (i >= 0) push(2, (uint32_t)(i + 1));
+    if (c >= 0) push(1, (uint32_t)(c + 1));
+    if (cnt > 0) o[cnt - 1] = last;
+    n_ops[idx] = cnt;
 }
 
 __global__ __launch_bounds__(256) void glb_cigar_gather_kernel(const GlobalJob *__restrict__ jobs, uint32_t n,
@@ -385,6 +483,15 @@ hipError_t launch_glb_traceback(hipStream_t st, const GlobalJob *jobs, uint32_t 
                                 uint32_t *n_ops) {
     if (n == 0) return hipSuccess;
     hipLaunchKernelGGL(glb_traceback_kernel, dim3((n + 255) / 256), dim3(256), 0, st, jobs, n, z, ops, n_ops);
+    return hipGetLastError();
+}
+
+hipError_t launch_glb_plain(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const GlobalJob *jobs,
+                            const uint32_t *order, uint32_t count, const GlobalScoring &sc, uint32_t *z,
+                            uint32_t *ops, uint32_t *n_ops, int32_t *score) {
+    if (count == 0) return hipSuccess;
+    hipLaunchKernelGGL(glb_plain_kernel, dim3((count + 63) / 64), dim3(64), 0, st, arena, ref, jobs, order, count, sc,
+                       z, ops, n_ops, score);
     return hipGetLastError();
 }
 

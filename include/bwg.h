@@ -243,7 +243,10 @@ int bwg_batch_side_bands(bwg_ctx *ctx, bwg_batch *b, int32_t *w_out);
  * all of the query against all of the target inside the band |j - i| <= w, gaps opened from
  * aligned pairs, unreachable cells at MINUS_INF, ties M over E over F, traceback from the band's
  * last cell.  Each job has its own band w[k] (the caller's clamp, cigar.rs:417-444), which must
- * cover the length difference (w[k] >= |qlen - tlen|, as that clamp guarantees); qlen <= 512.
+ * cover the length difference (w[k] >= |qlen - tlen|, as that clamp guarantees).  Queries up to
+ * 512 bases run in the binned group kernels; longer ones one job per thread, with a code byte per
+ * band cell in device memory (BWG_E2BIG when tlen * min(qlen, 2w + 1) exceeds 2^30, or a side
+ * exceeds 2^24 bases).
  * score[k] is the reference's score; the CIGAR words are len << 4 | op (0 = M, 1 = I, 2 = D), the
  * runs of job k following those of job k-1 in `cigar`, n_cigar[k] of them. */
 typedef struct { int32_t m; int8_t mat[25]; int32_t o_del, e_del, o_ins, e_ins; } bwg_global_params;

@@ -51,12 +51,12 @@ def test_scoring_sweep_matches_oracle(gpu_ctx, a, b, o_del, e_del, o_ins, e_ins)
 
 def test_edge_shapes_match_oracle(gpu_ctx):
     # empty sides, one-base sequences, bands of exactly the length difference, very wide bands,
-    # queries up to the 512-base limit (the widest group kernels), N bases
+    # queries up to 512 bases (the widest group kernels) and just past it, N bases
     _say("global edge shapes")
     rng = np.random.default_rng(7)
     qs, ts, ws = [], [], []
     for k in range(1500):
-        ql = int(rng.choice([0, 1, 2, 7, 33, 150, 300, 512]))
+        ql = int(rng.choice([0, 1, 2, 7, 33, 150, 300, 512, 513, 700]))
         tl = max(0, ql + int(rng.integers(-5, 6))) if k % 3 else int(rng.integers(0, 40))
         q = rng.integers(0, 5, ql).astype(np.uint8)
         t = rng.integers(0, 5, tl).astype(np.uint8)
@@ -91,6 +91,45 @@ def test_staged_replays_and_rejections(gpu_ctx):
     if bad_w[k] >= 0:
         with pytest.raises(bwg.BwgError):
             gpu_ctx.global_arrays(batch, p, bad_w)
-    long_q = bwg.JobBatch.from_pairs([np.zeros(513, np.uint8)], [np.zeros(513, np.uint8)])
-    with pytest.raises(bwg.BwgError):
-        gpu_ctx.global_arrays(long_q, p, np.array([10], np.int32))
+    # a code area beyond 1 GiB for one long query
+    huge = bwg.JobBatch.from_pairs([np.zeros(40000, np.uint8)], [np.zeros(40000, np.uint8)])
+    with pytest.raises(bwg.BwgError) as e:
+        gpu_ctx.global_arrays(huge, p, np.array([20000], np.int32))
+    assert e.value.code == bwg.BWG_E2BIG
+
+
+@pytest.mark.parametrize("seed", [3, 4])
+def test_long_queries_run_one_job_per_thread(gpu_ctx, seed):
+    """Queries past the group kernel's 512 columns (up to 3000 bases, bands of 20-700) mixed with
+    short ones in one batch: the long ones go to the one-job-per-thread kernel, whose scores and
+    CIGARs must equal the oracle's as the group kernels' do."""
+    _say("global long queries seed=%d" % seed)
+    rng = np.random.default_rng(seed)
+    qs, ts, ws = [], [], []
+    for k in range(160):
+        ql = int(rng.integers(513, 3000)) if k % 2 else int(rng.integers(1, 400))
+        q = rng.integers(0, 4, ql).astype(np.uint8)
+        t = list(q)
+        for _ in range(int(rng.integers(0, 12))):  # substitutions and indels
+            pos = int(rng.integers(0, len(t)))
+            r = rng.random()
+            if r < 0.5:
+                t[pos] = int(rng.integers(0, 4))
+            elif r < 0.75:
+                t[pos:pos] = [int(x) for x in rng.integers(0, 4, int(rng.integers(1, 30)))]
+            elif len(t) > 40:
+                del t[pos:pos + int(rng.integers(1, 30))]
+        t = np.asarray(t, np.uint8)
+        qs.append(q)
+        ts.append(t)
+        d = abs(len(q) - len(t))
+        ws.append(d + int(rng.choice([0, 5, 20, 100, 700])))
+    batch = bwg.JobBatch.from_pairs(qs, ts)
+    s, n = _check(gpu_ctx, batch, np.asarray(ws, np.int32))
+    assert (n[1::2] > 1).sum() > 40  # long jobs with gaps
+    st = gpu_ctx.stage_global(batch, bwg.global_params(5, bwg.default_mat(), 6, 1, 6, 1), np.asarray(ws, np.int32))
+    try:
+        st.run()
+        assert any(r["kernel"] == "glb_plain" for r in st.report())
+    finally:
+        st.close()
