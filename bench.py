@@ -567,7 +567,7 @@ def main():
                     help="region-to-CIGAR jobs of the banded global alignment workload")
     ap.add_argument("--matesw-anchors", type=int, default=200_000,
                     help="anchors of the mate-rescue round (4 orientation jobs each)")
-    ap.add_argument("--serve-rates", default="1e3,5e4,2e5,2e6,1e7,3e7,6e7",
+    ap.add_argument("--serve-rates", default="1e3,5e4,2e5,2e6,1e7,3e7,6e7,1e8",
                     help="offered loads (jobs/s) of the serving sweep; empty to skip")
     ap.add_argument("--serve-seconds", type=float, default=3.0,
                     help="seconds per offered load (longer at low loads, for at least ~40 chunks)")
