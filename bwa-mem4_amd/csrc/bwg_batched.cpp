@@ -124,10 +124,10 @@ inline uint32_t work_key(uint64_t w) {
 // The second half of binning: order the bins by stage then descending work, give them their
 // offsets, and order the jobs by (bin, descending work, index) from each job's bin (its index in
 // `bins` as created) and work key wk (65535 - work_key, so ascending wk is descending work).
-// Small batches: one comparison sort; large ones: a stable LSD radix sort over the 24-bit (32-bit
-// past 256 bins) key (bin rank << 16 | wk), 8 bits a pass, each pass split over `parts` threads
-// (per-part digit histograms, offsets in (digit, part) order, a stable scatter), which gives the
-// same order.
+// Small batches: one comparison sort; large ones: a stable LSD radix sort over the key
+// (bin rank << 16 | wk), 12 bits a pass (two passes up to 256 bins), each pass split over `parts`
+// threads (per-part digit histograms, offsets in (digit, part) order, a stable scatter), which
+// gives the same order.
 template <class BinOf>
 HostVec<uint32_t> order_jobs(size_t n, std::vector<Item> &bins, BinOf bin_of, const uint16_t *wk, int parts) {
     std::vector<int> perm(bins.size());
@@ -159,36 +159,47 @@ HostVec<uint32_t> order_jobs(size_t n, std::vector<Item> &bins, BinOf bin_of, co
         });
         return order;
     }
+    // 12-bit digits: two passes for up to 256 bins, the first digit's histogram taken while the
+    // keys are built and the last pass writing the order array itself
+    constexpr int kDigit = 12, kRadix = 1 << kDigit;
+    int key_bits = 16;
+    while ((size_t)1 << (key_bits - 16) < bins.size()) ++key_bits;
+    const int passes = (key_bits + kDigit - 1) / kDigit;
     HostVec<uint64_t> a(n), t(n);
-    parallel_for(n, parts, [&](size_t lo, size_t hi, int) {
-        for (size_t k = lo; k < hi; ++k)
+    std::vector<uint32_t> hist((size_t)parts * kRadix);
+    auto digit = [](uint64_t v, int pass) { return (uint32_t)(v >> (32 + kDigit * pass)) & (uint32_t)(kRadix - 1); };
+    parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
+        uint32_t *h = &hist[(size_t)q * kRadix];
+        std::fill(h, h + kRadix, 0u);
+        for (size_t k = lo; k < hi; ++k) {
             a[k] = ((uint64_t)((remap[bin_of(k)] << 16) | wk[k]) << 32) | (uint64_t)k;
+            h[digit(a[k], 0)]++;
+        }
     });
-    const int passes = bins.size() <= 256 ? 3 : 4;
-    std::vector<uint32_t> hist((size_t)parts * 256);
     for (int pass = 0; pass < passes; ++pass) {
-        const int sh = 32 + 8 * pass;
-        std::fill(hist.begin(), hist.end(), 0u);
-        parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
-            uint32_t *h = &hist[(size_t)q * 256];
-            for (size_t k = lo; k < hi; ++k) h[(a[k] >> sh) & 255]++;
-        });
-        uint32_t run = 0;
-        for (int d = 0; d < 256; ++d)
+        uint32_t run = 0;  // offsets in (digit, part) order: the scatter is stable
+        for (int d = 0; d < kRadix; ++d)
             for (int q = 0; q < parts; ++q) {
-                const uint32_t cnt = hist[(size_t)q * 256 + d];
-                hist[(size_t)q * 256 + d] = run;
+                const uint32_t cnt = hist[(size_t)q * kRadix + d];
+                hist[(size_t)q * kRadix + d] = run;
                 run += cnt;
             }
+        const bool last = pass + 1 == passes;
         parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
-            uint32_t *h = &hist[(size_t)q * 256];
-            for (size_t k = lo; k < hi; ++k) t[h[(a[k] >> sh) & 255]++] = a[k];
+            uint32_t *h = &hist[(size_t)q * kRadix];
+            if (last)
+                for (size_t k = lo; k < hi; ++k) order[h[digit(a[k], pass)]++] = (uint32_t)a[k];
+            else
+                for (size_t k = lo; k < hi; ++k) t[h[digit(a[k], pass)]++] = a[k];
         });
+        if (last) break;
         a.swap(t);
+        parallel_for(n, parts, [&](size_t lo, size_t hi, int q) {
+            uint32_t *h = &hist[(size_t)q * kRadix];
+            std::fill(h, h + kRadix, 0u);
+            for (size_t k = lo; k < hi; ++k) h[digit(a[k], pass + 1)]++;
+        });
     }
-    parallel_for(n, parts, [&](size_t lo, size_t hi, int) {
-        for (size_t k = lo; k < hi; ++k) order[k] = (uint32_t)a[k];
-    });
     return order;
 }
 

@@ -6,6 +6,7 @@
 
 #include <algorithm>
 #include <cstring>
+#include <deque>
 #include <iterator>
 #include <new>
 #include <system_error>
@@ -90,6 +91,96 @@ void DevicePool::release_all() {
 // ---------------------------------------------------------------------------------------------
 // StreamSet
 // ---------------------------------------------------------------------------------------------
+namespace {
+
+// The host worker threads behind run_parts: started on first use, kept for the life of the
+// process (a thread start costs tens of microseconds, and a large call runs several passes).
+// Calls from several threads share the workers; each call's parts are claimed with an atomic
+// counter, by the workers and by the calling thread alike.  A worker counts itself into a call
+// (users) while it may touch it, and the caller returns only after taking the call out of the
+// queue with no worker inside it.
+class HostWorkers {
+  public:
+    static HostWorkers &get() {
+        static HostWorkers *w = new HostWorkers();  // never destroyed: workers run until exit
+        return *w;
+    }
+    void run(int parts, const std::function<void(int)> &fn) {
+        Call call;
+        call.fn = &fn;
+        call.parts = parts;
+        call.left.store(parts);
+        {
+            std::lock_guard<std::mutex> g(mu_);
+            calls_.push_back(&call);
+        }
+        for (int q = 1; q < parts; ++q) cv_.notify_one();  // as many workers as the call can use
+        work_on(call);
+        std::unique_lock<std::mutex> g(mu_);
+        const auto it = std::find(calls_.begin(), calls_.end(), &call);
+        if (it != calls_.end()) calls_.erase(it);
+        done_.wait(g, [&] { return call.users == 0 && call.left.load() == 0; });
+    }
+
+  private:
+    struct Call {
+        const std::function<void(int)> *fn = nullptr;
+        int parts = 0;
+        int users = 0;  // workers inside the call (under mu_)
+        std::atomic<int> next{0}, left{0};
+    };
+    HostWorkers() {
+        for (int k = 0; k < 15; ++k) {
+            try {
+                std::thread([this] { loop(); }).detach();
+            } catch (const std::system_error &) {
+                break;  // fewer workers: the callers run more of their parts themselves
+            }
+        }
+    }
+    // claim and run parts of `call` until none is left to claim
+    static void work_on(Call &call) {
+        for (;;) {
+            const int q = call.next.fetch_add(1);
+            if (q >= call.parts) return;
+            (*call.fn)(q);
+            call.left.fetch_sub(1);
+        }
+    }
+    void loop() {
+        for (;;) {
+            Call *call = nullptr;
+            {
+                std::unique_lock<std::mutex> g(mu_);
+                cv_.wait(g, [&] { return !calls_.empty(); });
+                call = calls_.front();
+                ++call->users;
+                // every part claimed: no one else needs to find the call
+                if (call->next.load() >= call->parts) calls_.pop_front();
+            }
+            work_on(*call);
+            {
+                std::lock_guard<std::mutex> g(mu_);
+                --call->users;
+            }
+            done_.notify_all();
+        }
+    }
+    std::mutex mu_;
+    std::condition_variable cv_, done_;
+    std::deque<Call *> calls_;
+};
+
+}  // namespace
+
+void run_parts(int parts, const std::function<void(int)> &fn) {
+    if (parts <= 1) {
+        if (parts == 1) fn(0);
+        return;
+    }
+    HostWorkers::get().run(parts, fn);
+}
+
 int host_parts(const Tuning &tu, size_t n) {
     static const int avail = [] {
         cpu_set_t set;

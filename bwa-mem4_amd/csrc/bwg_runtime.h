@@ -6,6 +6,7 @@
 #include <atomic>
 #include <condition_variable>
 #include <cstdint>
+#include <functional>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -158,18 +159,16 @@ struct NoInitAlloc : std::allocator<T> {
 template <class T>
 using HostVec = std::vector<T, NoInitAlloc<T>>;
 int host_parts(const Tuning &tu, size_t n);
+// fn(part) for every part in [0, parts), on the process's host worker threads and the caller (which
+// takes parts too, so a call completes even when every worker is busy); returns when all are done.
+void run_parts(int parts, const std::function<void(int)> &fn);
 template <class Fn>
 void parallel_for(size_t n, int parts, Fn &&fn) {
     if (parts <= 1) {
         fn((size_t)0, n, 0);
         return;
     }
-    std::vector<std::thread> th;
-    th.reserve((size_t)parts - 1);
-    for (int q = 1; q < parts; ++q)
-        th.emplace_back([&fn, n, parts, q] { fn(n * q / parts, n * (q + 1) / parts, q); });
-    fn((size_t)0, n / parts, 0);
-    for (auto &t : th) t.join();
+    run_parts(parts, [&fn, n, parts](int q) { fn(n * q / parts, n * (q + 1) / parts, q); });
 }
 
 // An execution lane of the synchronous entry points: its own stream set, so that calls from
