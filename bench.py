@@ -162,7 +162,7 @@ def resc_bin_table(rep, bins, batch, res, max_sc=1):
     for r in rep:
         row = {"kernel": r["kernel"], "jobs": r["jobs"], "us": round(r["us"], 2)}
         sel = bins == r["index"]
-        packed = "pk<" in r["kernel"]
+        packed = r["kernel"].startswith("ksw_pk")
         peak = VALU_PK16_PEAK_TOPS if packed else VALU_PEAK_TOPS
         if sel.any():
             fwd_of[r["kernel"].split(">")[0]] = sel
@@ -211,7 +211,7 @@ def _short_kernel(name):
 def _trace_name(bin_name):
     """The trace name of the kernel a bench bin label (the library's mapping name) launches."""
     import re
-    m = re.match(r"(ext_group8|ext_group|ext_thread|ksw_pk|ksw_group)<(\d+),i(\d+)(,reverse)?>$", bin_name)
+    m = re.match(r"(ext_group8|ext_group|ext_thread|ksw_pk8|ksw_pk|ksw_group)<(\d+),i(\d+)(,reverse)?>$", bin_name)
     if not m:
         return bin_name + "_kernel"
     fam, k, width, rev = m.group(1), int(m.group(2)), int(m.group(3)), bool(m.group(4))
@@ -220,7 +220,9 @@ def _trace_name(bin_name):
         return "ext_group_kernel<%d, %s, %d>" % (k, narrow, 8 if fam == "ext_group8" else 16)
     if fam == "ext_thread":
         return "ext_thread_kernel<%d, %s>" % (k, narrow)
-    return "%s_kernel<%d, %s>" % (fam, k // 2 if fam == "ksw_pk" else k, "true" if rev else "false")
+    if fam in ("ksw_pk", "ksw_pk8"):  # template: column pairs per lane, direction, group lanes
+        return "ksw_pk_kernel<%d, %s, %d>" % (k // 2, "true" if rev else "false", 8 if fam == "ksw_pk8" else 16)
+    return "%s_kernel<%d, %s>" % (fam, k, "true" if rev else "false")
 
 
 def _split_at_markers(rows, what, with_markers=False):
@@ -967,7 +969,7 @@ def main():
         r = res["resc_batched"]
         # forward-pass nominal cells x 13 over the whole rescue launch sequence (forward, tracker,
         # reverse); every forward bin of this workload is the packed 16-bit kernel
-        packed = resc_bins is not None and all("pk<" in b["kernel"] for b in resc_bins if "cells" in b
+        packed = resc_bins is not None and all(b["kernel"].startswith("ksw_pk") for b in resc_bins if "cells" in b
                                                and "reverse" not in b["kernel"])
         peak = VALU_PK16_PEAK_TOPS if packed else VALU_PEAK_TOPS
         ach = nominal * K_RESC / r["kernel_s"] / 1e12

@@ -235,6 +235,10 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
         if (v != 8 && v != 16) return BWG_EINVAL;
         t.ext_group_lanes = (int)v;
         return BWG_OK;
+    case BWG_OPT_KSW_GROUP_LANES:
+        if (v != 8 && v != 16) return BWG_EINVAL;
+        t.ksw_group_lanes = (int)v;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -270,6 +274,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_COALESCE: return t.coalesce;
     case BWG_OPT_HOST_THREADS: return t.host_threads;
     case BWG_OPT_EXT_GROUP_LANES: return t.ext_group_lanes;
+    case BWG_OPT_KSW_GROUP_LANES: return t.ksw_group_lanes;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns.load();

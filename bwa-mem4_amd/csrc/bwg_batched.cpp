@@ -258,8 +258,9 @@ int ext_grid(const bwg_ctx *c, const Tuning &tu, int kind, int param, bool narro
 
 int ksw_grid(const bwg_ctx *c, const Tuning &tu, int K, bool rev, uint32_t count, bool packed = false) {
     const int per_cu = packed ? ksw_pk_blocks_per_cu(K, rev) : ksw_group_blocks_per_cu(K, rev);
+    const int64_t groups = packed ? 256 / ksw_param_lanes(K) : 16;  // groups per block
     return (int)std::max<int64_t>(1, std::min<int64_t>((int64_t)per_cu * c->num_cus * tu.ksw_group_oversub,
-                                                       ((int64_t)count + 15) / 16));
+                                                       ((int64_t)count + groups - 1) / groups));
 }
 
 // Length class of an extension job (see bwg_tuner.h) and its minimal group-kernel K.
@@ -624,6 +625,10 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
         t.param = K;
         t.width = 16;
         ent.cands.push_back(t);
+        if (ksw_pk8_scoring_ok(b->ksc)) {  // and with eight jobs a wavefront
+            t.param = ksw_param(K, 8);
+            ent.cands.push_back(t);
+        }
     }
     uint32_t *d = (uint32_t *)c->pool.alloc(((size_t)S + 1) * 4);
     KswJob *dj = (KswJob *)c->pool.alloc((size_t)S * sizeof(KswJob));
@@ -699,6 +704,7 @@ int plan_ksw(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         const int K = 2 * (cl % 8 + 1);
         chosen[cl] = tu.ksw_k_step ? std::min(16, K + 2) : K;
         chosen_pk[cl] = cl < 8 && !tu.ksw_force_wide;
+        if (chosen_pk[cl] && tu.ksw_group_lanes == 8 && ksw_pk8_scoring_ok(sc)) chosen[cl] = ksw_param(chosen[cl], 8);
         if (!tu.autotune) continue;
         // a rescue-window batch's descriptors are written on the device, so it only reads the table
         if (tu.tune_new && !b->matesw && members[cl].size() >= kTuneMinJobs && !c->tuner.lookup(sig, cl, nullptr)) {

@@ -72,8 +72,16 @@ inline bool ksw_pk_scoring_ok(const KswScoring &sc) {
     return sc.o_del >= 0 && sc.o_ins >= 0 && sc.e_del >= 0 && sc.e_ins >= 0 && sc.o_del + sc.e_del < 16384 &&
            sc.o_ins + sc.e_ins < 16384 && sc.e_ins < 4000;
 }
-int ksw_pk_blocks_per_cu(int K, bool rev);
-hipError_t launch_ksw_pk(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref,
+// A packed bin's param: the 16-lane K (2..16), or kKsw8Lanes + K for 8-lane groups with 2K
+// columns (K pairs) per lane over the same padded query.  The 8-lane form carries F with up to 16
+// extension penalties of drift, so it needs e_ins below 1000 (ksw_pk8_scoring_ok).
+constexpr int kKsw8Lanes = 32;
+inline int ksw_param(int K, int lanes) { return lanes == 8 ? kKsw8Lanes + K : K; }
+inline int ksw_param_k(int param) { return param > kKsw8Lanes ? param - kKsw8Lanes : param; }
+inline int ksw_param_lanes(int param) { return param > kKsw8Lanes ? 8 : 16; }
+inline bool ksw_pk8_scoring_ok(const KswScoring &sc) { return ksw_pk_scoring_ok(sc) && sc.e_ins < 1000; }
+int ksw_pk_blocks_per_cu(int param, bool rev);
+hipError_t launch_ksw_pk(hipStream_t st, int param, bool rev, const uint8_t *arena, const uint8_t *ref,
                          const KswJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
                          const KswScoring &sc, bwg_ksw_result *out, int32_t *rowmax, int grid);
 hipError_t launch_ksw_tracker(hipStream_t st, const KswJob *jobs, const uint32_t *list, uint32_t count,

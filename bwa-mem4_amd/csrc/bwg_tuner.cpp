@@ -1,6 +1,8 @@
 // Autotuner table and report (the timing runs live with the planner in bwg_batched.cpp).
 #include "bwg_tuner.h"
 
+#include "bwg_batched_kernels.h"
+
 #include <cstdio>
 #include <cstring>
 
@@ -34,9 +36,18 @@ const char *mapping_name(int kind, int param, int width, char *buf, size_t len) 
     case kExtThread: std::snprintf(buf, len, "ext_thread<%d,i%d>", param, width); break;
     case kExtGroup: std::snprintf(buf, len, "ext_group<%d,i%d>", param, width); break;
     case kExtGroup8: std::snprintf(buf, len, "ext_group8<%d,i%d>", param, width); break;
-    case kKswFwd: std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16>" : "ksw_group<%d,i32>", param); break;
+    // packed 8-lane bins are named by their columns per lane, twice the 16-lane K
+    case kKswFwd:
+        if (width == 16 && ksw_param_lanes(param) == 8)
+            std::snprintf(buf, len, "ksw_pk8<%d,i16>", 2 * ksw_param_k(param));
+        else
+            std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16>" : "ksw_group<%d,i32>", param);
+        break;
     case kKswRev:
-        std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16,reverse>" : "ksw_group<%d,i32,reverse>", param);
+        if (width == 16 && ksw_param_lanes(param) == 8)
+            std::snprintf(buf, len, "ksw_pk8<%d,i16,reverse>", 2 * ksw_param_k(param));
+        else
+            std::snprintf(buf, len, width == 16 ? "ksw_pk<%d,i16,reverse>" : "ksw_group<%d,i32,reverse>", param);
         break;
     case kKswTracker: std::snprintf(buf, len, "ksw_tracker"); break;
     case kExtUngapped: std::snprintf(buf, len, "ext_ungapped"); break;

@@ -216,10 +216,11 @@ __global__ __launch_bounds__(256) void ksw_group_kernel(const uint8_t *__restric
 // target codes x KH column pairs) sits in LDS, lane-major, one 4-byte read per pair and row.
 typedef short pk16 __attribute__((ext_vector_type(2)));
 
-// Query column of a row maximum's 32-bit key h << 10 | (15-g) << 6 | (63-c): lane g owns columns
-// g*K .. g*K+K-1.
+// Query column of a row maximum's 32-bit key h << 10 | (GL-1-g) << 6 | (63-c): lane g of a GL-lane
+// group owns columns g*K .. g*K+K-1.
+template <int GL>
 __device__ __forceinline__ int32_t pk_key_column(int32_t key, int K) {
-    return (15 - ((key >> 6) & 15)) * K + 63 - (key & 63);
+    return (GL - 1 - ((key >> 6) & 15)) * K + 63 - (key & 63);
 }
 
 __device__ __forceinline__ pk16 pk_splat(int32_t v) { return (pk16){(short)v, (short)v}; }
@@ -235,7 +236,10 @@ __device__ __forceinline__ pk16 pk_key(pk16 h, pk16 k) {
     return pk_from(r);
 }
 
-template <int KH, bool REV>
+// GL = 8: the same pass with 8-lane groups (GroupLanes<8>, two per DPP row), eight jobs a
+// wavefront and twice the columns per lane for the same query, so that the row's fixed work (the
+// profile reads, the scan, the row maximum and its bookkeeping) is shared by twice the cells.
+template <int KH, bool REV, int GL>
 __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__ arena,
                                                      const uint8_t *__restrict__ ref,
                                                      const KswJob *__restrict__ jobs,
@@ -251,12 +255,12 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         tblT[threadIdx.x] = tc < 5 ? (uint8_t)sc.mat[tc * 5 + qc] : (uint8_t)0;
     }
     __syncthreads();
-    const int lane = threadIdx.x & 63;
-    const int g = lane & 15;
+    using L = GroupLanes<GL>;
+    const int g = L::g();
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
     const int32_t Ke = K * e_ins, KHe = KH * e_ins;
-    const uint32_t g64 = (uint32_t)(15 - g) * 64u;  // this lane's field of the row-maximum key
+    const uint32_t g64 = (uint32_t)(GL - 1 - g) * 64u;  // this lane's field of the row-maximum key
     const pk16 e_del2 = pk_splat(e_del), oe_del2 = pk_splat(oe_del), zero2 = pk_splat(0);
 
     bool active = false, exhausted = false;
@@ -266,7 +270,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
     int32_t *rm = nullptr;
     pk16 hrow[KH], erow[KH], kk[KH];
 
-    GroupSlots slots(count, REV);  // the forward pass's long jobs balance better dynamically
+    GroupSlotsT<GL> slots(count, REV);  // the forward pass's long jobs balance better dynamically
     for (;;) {
         if (!active && !exhausted) {
             const uint32_t slot = slots.take(head);
@@ -341,7 +345,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         // H(i-1, column-1): lower half of pair 0 from the previous lane's last column (0 left of
         // column 0), upper half of pair 0 from this lane's last lower column, then pair p-1
         // ({this lane's last pair, the previous lane's last pair} >> 16): one funnel shift
-        const uint32_t prev_pair = (uint32_t)row_shift_in((int32_t)pk_bits(hrow[KH - 1]), 0);
+        const uint32_t prev_pair = (uint32_t)L::shift_in((int32_t)pk_bits(hrow[KH - 1]), 0);
         pk16 hd = pk_from(__builtin_amdgcn_alignbit(pk_bits(hrow[KH - 1]), prev_pair, 16u));
         // F along each half carried with its drift G_p = F_p + p*e_ins: F_{p+1} = max(F_p - e_ins,
         // M_p - oe_ins) becomes G_{p+1} = max(G_p, M_p + (p+1)*e_ins - oe_ins), one op per pair
@@ -358,8 +362,8 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
         grun = grun - pk_splat(KHe);  // F leaving each half's last pair
         const int32_t c_lo = max((int32_t)grun.x, 0), c_hi = max((int32_t)grun.y, 0);
         const int32_t U = max(c_hi, c_lo - KHe) + g * Ke;
-        const int32_t incl = row_scan_max_nonneg(U);
-        const int32_t excl = row_shift_in(incl, -Ke);
+        const int32_t incl = L::scan_max_nonneg(U);
+        const int32_t excl = L::shift_in(incl, -Ke);
         const int32_t f_in = excl - (g - 1) * Ke;
         pk16 f = (pk16){(short)f_in, (short)max(f_in - KHe, c_lo)};  // F + p*e_ins at pair p
         pk16 key2 = zero2;
@@ -372,7 +376,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
             hrow[p] = h;
         }
         const uint32_t kbest = (uint32_t)max((int32_t)key2.x, (int32_t)key2.y);  // >= 0
-        const int32_t key = row_allmax((int32_t)(__umul24(kbest >> 6, 960u) + kbest + g64));
+        const int32_t key = L::allmax((int32_t)(__umul24(kbest >> 6, 960u) + kbest + g64));
         const int32_t imax = key >> 10;
         if (!REV && __builtin_expect(g == 0, 1)) *rm = imax;  // under the exec mask, no branch
         if (!REV) rm += 64;  // row i+1's slot (a running pointer: one 64-bit add a row)
@@ -383,7 +387,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
                 if (gmax >= endsc) {
                     // first row reaching the forward score: the alignment starts rte/rqe back
                     if (g == 0 && gmax == endsc) {
-                        const int32_t rqe = pk_key_column(key, K);
+                        const int32_t rqe = pk_key_column<GL>(key, K);
                         out[idx].tb = te - i;
                         out[idx].qb = qe - rqe;
                     }
@@ -391,7 +395,7 @@ __global__ __launch_bounds__(256) void ksw_pk_kernel(const uint8_t *__restrict__
                 }
             } else {
                 te = i;
-                qe = pk_key_column(key, K);
+                qe = pk_key_column<GL>(key, K);
             }
         }
         ++i;
@@ -483,13 +487,13 @@ int occ() {
     return v;
 }
 
-template <int KH, bool REV>
+template <int KH, bool REV, int GL>
 int occ_pk() {
     static std::atomic<int> cache{0};
     int v = cache.load(std::memory_order_relaxed);
     if (v > 0) return v;
     int blocks = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ksw_pk_kernel<KH, REV>, 256, 0) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&blocks, ksw_pk_kernel<KH, REV, GL>, 256, 0) != hipSuccess)
         blocks = 1;
     v = blocks > 0 ? blocks : 1;
     cache.store(v, std::memory_order_relaxed);
@@ -498,32 +502,59 @@ int occ_pk() {
 
 }  // namespace
 
-int ksw_pk_blocks_per_cu(int K, bool rev) {
+int ksw_pk_blocks_per_cu(int param, bool rev) {
+    const int K = ksw_param_k(param);
+    if (ksw_param_lanes(param) == 8) {
+        switch (K) {
+#define BWG_K(k) case k: return rev ? occ_pk<k, true, 8>() : occ_pk<k, false, 8>();
+            BWG_KSW_GROUP_KS
+#undef BWG_K
+        default: return 1;
+        }
+    }
     switch (K) {
-#define BWG_K(k) case k: return rev ? occ_pk<k / 2, true>() : occ_pk<k / 2, false>();
+#define BWG_K(k) case k: return rev ? occ_pk<k / 2, true, 16>() : occ_pk<k / 2, false, 16>();
         BWG_KSW_GROUP_KS
 #undef BWG_K
     default: return 1;
     }
 }
 
-hipError_t launch_ksw_pk(hipStream_t st, int K, bool rev, const uint8_t *arena, const uint8_t *ref,
+// param: the 16-lane K, or ksw_param(K, 8) for 8-lane groups with 2K columns (K pairs) per lane.
+hipError_t launch_ksw_pk(hipStream_t st, int param, bool rev, const uint8_t *arena, const uint8_t *ref,
                          const KswJob *jobs, const uint32_t *order, uint32_t count, uint32_t *head,
                          const KswScoring &sc, bwg_ksw_result *out, int32_t *rowmax, int grid) {
+    const int K = ksw_param_k(param);
+#define BWG_LAUNCH(KH, GL)                                                                                    \
+    do {                                                                                                      \
+        if (rev)                                                                                              \
+            hipLaunchKernelGGL((ksw_pk_kernel<KH, true, GL>), dim3(grid), dim3(256), 0, st, arena, ref, jobs,  \
+                               order, count, head, sc, out, rowmax);                                          \
+        else                                                                                                  \
+            hipLaunchKernelGGL((ksw_pk_kernel<KH, false, GL>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
+                               order, count, head, sc, out, rowmax);                                          \
+    } while (0)
+    if (ksw_param_lanes(param) == 8) {
+        switch (K) {
+#define BWG_K(k)              \
+    case k:                   \
+        BWG_LAUNCH(k, 8);     \
+        return hipGetLastError();
+            BWG_KSW_GROUP_KS
+#undef BWG_K
+        default: return hipErrorInvalidValue;
+        }
+    }
     switch (K) {
-#define BWG_K(k)                                                                                          \
-    case k:                                                                                               \
-        if (rev)                                                                                          \
-            hipLaunchKernelGGL((ksw_pk_kernel<k / 2, true>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
-                               order, count, head, sc, out, rowmax);                                      \
-        else                                                                                              \
-            hipLaunchKernelGGL((ksw_pk_kernel<k / 2, false>), dim3(grid), dim3(256), 0, st, arena, ref, jobs, \
-                               order, count, head, sc, out, rowmax);                                      \
+#define BWG_K(k)              \
+    case k:                   \
+        BWG_LAUNCH(k / 2, 16); \
         return hipGetLastError();
         BWG_KSW_GROUP_KS
 #undef BWG_K
     default: return hipErrorInvalidValue;
     }
+#undef BWG_LAUNCH
 }
 
 int ksw_group_blocks_per_cu(int K, bool rev) {

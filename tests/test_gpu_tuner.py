@@ -119,17 +119,25 @@ def test_forced_rescue_mappings_agree(gpu_ctx):
     gpu_ctx.set_form(bwg.FORM_BATCHED)
     gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
     try:
-        # minimal K or one above; packed 16-bit pairs where exact, or 32-bit state always
-        for kstep, width in [(0, 16), (1, 16), (0, 32), (1, 32)]:
-            _say("forced rescue k_step=%d width=%d" % (kstep, width))
+        # minimal K or one above; packed 16-bit pairs where exact (16 or 8 lanes per job), or 32-bit
+        # state always
+        for kstep, width, lanes in [(0, 16, 16), (1, 16, 16), (0, 32, 16), (1, 32, 16), (0, 16, 8), (1, 16, 8)]:
+            _say("forced rescue k_step=%d width=%d lanes=%d" % (kstep, width, lanes))
             gpu_ctx.set_option(bwg.OPT_KSW_K_STEP, kstep)
             gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, width)
+            gpu_ctx.set_option(bwg.OPT_KSW_GROUP_LANES, lanes)
             gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
             got = gpu_ctx.align2_arrays(batch, params)
-            assert _first_bad(got, want) is None, (kstep, width, _first_bad(got, want))
+            assert _first_bad(got, want) is None, (kstep, width, lanes, _first_bad(got, want))
+            if lanes == 8:
+                st = gpu_ctx.stage(batch, params)
+                st.run()
+                assert any(r["kernel"].startswith("ksw_pk8<") for r in st.report())
+                st.close()
     finally:
         gpu_ctx.set_option(bwg.OPT_KSW_K_STEP, 0)
         gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, 16)
+        gpu_ctx.set_option(bwg.OPT_KSW_GROUP_LANES, 16)
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
@@ -177,13 +185,16 @@ def test_rescue_score_width_boundary(gpu_ctx, a, b, o, e, lo, hi):
     gpu_ctx.set_form(bwg.FORM_BATCHED)
     gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
     try:
-        for width in (16, 32):
+        # the 8-lane packed form applies only with e < 1000; otherwise the option changes nothing
+        for width, lanes in ((16, 16), (32, 16), (16, 8)):
             gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, width)
+            gpu_ctx.set_option(bwg.OPT_KSW_GROUP_LANES, lanes)
             gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 0)
             got = gpu_ctx.align2_arrays(batch, params)
-            assert _first_bad(got, want) is None, (width, _first_bad(got, want))
+            assert _first_bad(got, want) is None, (width, lanes, _first_bad(got, want))
     finally:
         gpu_ctx.set_option(bwg.OPT_KSW_SCORE_WIDTH, 16)
+        gpu_ctx.set_option(bwg.OPT_KSW_GROUP_LANES, 16)
         gpu_ctx.set_option(bwg.OPT_AUTOTUNE, 1)
 
 
