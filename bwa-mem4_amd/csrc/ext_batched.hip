@@ -81,66 +81,86 @@ __device__ __forceinline__ void fill_subst_table(int8_t *tbl, const ExtScoring &
     if (t < 40) tbl[t] = (t & 7) < 5 ? sc.mat[(t >> 3) * 5 + (t & 7)] : (int8_t)0;
 }
 
+// LDS bank layout.  The LDS serves one half-wavefront (32 lanes) per cycle from 32 banks of 4
+// bytes, so the NG = 32 / GL groups of a half-wavefront share each access.  Their per-column words
+// are interleaved: column j of the group with index q = gid % NG within its half-wavefront sits at
+// word NG*j + q of the half-wavefront's region.  A group's lanes read columns kr apart (kr odd),
+// words NG*kr apart, which is NG times a permutation of 0..GL-1 modulo 32: the GL lanes hit GL
+// distinct banks, all congruent to q modulo NG, so the groups of a half-wavefront never share a
+// bank whatever their windows.
+template <int GL> constexpr int groups_per_half() { return 32 / GL; }
+
 // The column state of one group, the reference's eh arrays over absolute columns 0..qcap+1.  Wide:
-// {H[j], E[j]} pairs of int32.  Narrow: H and E as two arrays of uint16, exact whenever H and E
-// stay below 65536; they never exceed h0 + qlen*max_sc, which the planner checks for narrow bins.
-// Narrow state halves the group's LDS, which is what limits the kernel's occupancy for wide
-// windows.
-template <bool NARROW> struct EhView;
-template <> struct EhView<false> {
-    int2 *p;
-    __device__ __forceinline__ EhView(void *base, int) : p((int2 *)base) {}
-    __device__ __forceinline__ int2 load(int j) const { return p[j]; }
-    __device__ __forceinline__ int32_t load_h(int j) const { return p[j].x; }
-    __device__ __forceinline__ int32_t load_e(int j) const { return p[j].y; }
-    __device__ __forceinline__ void set_h(int j, int32_t v) const { p[j].x = v; }
-    __device__ __forceinline__ void set_e(int j, int32_t v) const { p[j].y = v; }
-    __device__ __forceinline__ void init(int j, int32_t h) const { p[j] = make_int2(h, 0); }
+// H and E as int32, each in its own interleaved array.  Narrow: one word per column, H in its low
+// and E in its high 16 bits, exact whenever H and E stay below 65536; they never exceed
+// h0 + qlen*max_sc, which the planner checks for narrow bins.  Narrow state halves the group's LDS,
+// which is what limits the kernel's occupancy for wide windows.  `col0` is the group's word of
+// column 0; column j is NG words further per column.
+template <bool NARROW, int NG> struct EhView;
+template <int NG> struct EhView<false, NG> {
+    int32_t *hp, *ep;
+    __device__ __forceinline__ EhView(int32_t *col0, int qcap) : hp(col0), ep(col0 + NG * (qcap + 2)) {}
+    __device__ __forceinline__ int2 load(int j) const { return make_int2(hp[NG * j], ep[NG * j]); }
+    __device__ __forceinline__ int32_t load_h(int j) const { return hp[NG * j]; }
+    __device__ __forceinline__ int32_t load_e(int j) const { return ep[NG * j]; }
+    __device__ __forceinline__ void set_h(int j, int32_t v) const { hp[NG * j] = v; }
+    __device__ __forceinline__ void set_e(int j, int32_t v) const { ep[NG * j] = v; }
+    __device__ __forceinline__ void init(int j, int32_t h) const {
+        hp[NG * j] = h;
+        ep[NG * j] = 0;
+    }
 };
-template <> struct EhView<true> {
-    uint16_t *hp, *ep;
-    __device__ __forceinline__ EhView(void *base, int qcap)
-        : hp((uint16_t *)base), ep((uint16_t *)base + (qcap + 2)) {}
+template <int NG> struct EhView<true, NG> {
+    uint16_t *p;  // H of column j at p[2*NG*j], E at p[2*NG*j + 1]
+    __device__ __forceinline__ EhView(int32_t *col0, int) : p((uint16_t *)col0) {}
     // The widened values are pinned where they are loaded: widening a 16-bit value in a later block
     // (where the column's use sits behind a branch) costs an AND there, where ds_read_u16 has
     // already zero-extended it.
     __device__ __forceinline__ int2 load(int j) const {
-        int32_t h = hp[j], e = ep[j];
+        int32_t h = p[2 * NG * j], e = p[2 * NG * j + 1];
         asm("" : "+v"(h));
         asm("" : "+v"(e));
         return make_int2(h, e);
     }
     __device__ __forceinline__ int32_t load_h(int j) const {
-        int32_t h = hp[j];
+        int32_t h = p[2 * NG * j];
         asm("" : "+v"(h));
         return h;
     }
     __device__ __forceinline__ int32_t load_e(int j) const {
-        int32_t e = ep[j];
+        int32_t e = p[2 * NG * j + 1];
         asm("" : "+v"(e));
         return e;
     }
-    __device__ __forceinline__ void set_h(int j, int32_t v) const { hp[j] = (uint16_t)v; }
-    __device__ __forceinline__ void set_e(int j, int32_t v) const { ep[j] = (uint16_t)v; }
+    __device__ __forceinline__ void set_h(int j, int32_t v) const { p[2 * NG * j] = (uint16_t)v; }
+    __device__ __forceinline__ void set_e(int j, int32_t v) const { p[2 * NG * j + 1] = (uint16_t)v; }
     __device__ __forceinline__ void init(int j, int32_t h) const {
-        hp[j] = (uint16_t)h;
-        ep[j] = 0;
+        *(uint32_t *)(p + 2 * NG * j) = (uint32_t)(uint16_t)h;
     }
 };
 
-// Block LDS layout (bytes): the column state of every group, qcap + 2 columns each; then every
-// group's score planes, plane c holding mat[c][query[j]] for j < qcap (five int8 planes); then slack.  A lane
-// always loads all columns of its block, including columns past the window (their values are not
-// used); those loads may run past the group's own region into the next one, and the slack keeps
-// the last group's loads inside the allocation.
-__host__ __device__ __forceinline__ int eh_bytes(int qcap, bool narrow) { return (narrow ? 4 : 8) * (qcap + 2); }
-__host__ __device__ __forceinline__ int plane_bytes(int qcap) { return 5 * qcap; }
+// Block LDS layout, in 4-byte words, for G = 256/GL groups in 8 half-wavefronts of NG groups:
+//  * eh: G * eh_words(qcap) words; half-wavefront hw owns NG * eh_words words from
+//    hw * NG * eh_words, interleaved as above;
+//  * score planes 0-3: G * qcap words; half-wavefront hw owns NG * qcap words from hw * NG * qcap,
+//    word NG*j + q holding mat[c][query[j]] in byte c for target codes c = 0..3;
+//  * score plane 4 (target N): G * qcap bytes; the half-wavefronts hw and hw + 1, hw + 2, hw + 3
+//    (hw a multiple of 4) share NG * qcap words, word NG*j + q, byte hw & 3, so that every plane has
+//    the same column stride of 4*NG bytes;
+//  * slack.  A lane always loads all columns of its block, including columns past the window
+//    (their values are not used), up to GL*K - 1 columns past the group's first window column,
+//    which is at most qlen: those loads run past the group's region into the next, and the slack,
+//    32*K words (the largest overrun at a stride of NG words, NG*GL = 32), keeps the last region's
+//    loads inside the allocation.
+__host__ __device__ __forceinline__ int eh_words(int qcap, bool narrow) { return (narrow ? 1 : 2) * (qcap + 2); }
 __host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int K, bool narrow) {
-    return (size_t)(256 / GL) * (eh_bytes(qcap, narrow) + plane_bytes(qcap)) + (size_t)GL * K * 8 + 64;
+    const size_t groups = (size_t)(256 / GL);
+    return groups * (4 * (size_t)eh_words(qcap, narrow) + 5 * (size_t)qcap) + (size_t)128 * K + 64;
 }
 
 // The cells of one DP row for one group, with the row's window spread over the 16 lanes in blocks
-// of KR columns: lane g owns window columns base .. base + KR - 1, base = beg + g*KR.  Returns
+// of KR columns: lane g owns window columns base .. base + KR - 1, base = beg + g*KR.  `srow` is
+// the row's score plane, column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns
 // the lane's row-maximum key (h << 13 | absolute column), 0 when the lane has no window column.
 // Only window columns store: the columns past the window keep their stale eh values, which a
 // later row reads when the band widens again (by up to two columns a row), as the reference's do.
@@ -150,7 +170,8 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
     using L = GroupLanes<GL>;
     const int32_t base = beg + g * KR;  // first window column of this lane
     const int32_t nval = min(max(end - base, 0), KR);
-    const int8_t *sb = srow + base;
+    constexpr int kSStride = 4 * groups_per_half<GL>();
+    const int8_t *sb = srow + base * kSStride;
     // pass 1: the loads are unconditional and independent, so they issue back to back; columns
     // past the window compute values that are never used
     int2 v[KR];
@@ -158,7 +179,7 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
         v[k] = eh.load(base + k);
-        s[k] = sb[k];
+        s[k] = sb[k * kSStride];
     }
     // F from 0 at the block's first column, carried with its drift: G_k = F_k + k*e_ins, so
     // F_{k+1} = max(F_k - e_ins, M_k - oe_ins, 0) becomes G_{k+1} = max(G_k, M_k + (k+1)*e_ins -
@@ -238,8 +259,15 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
     using L = GroupLanes<GL>;
     const int g = L::g();      // lane within the group
     const int gid = L::gid();  // group within the block
-    const EhView<NARROW> eh((char *)lds + gid * eh_bytes(qcap, NARROW), qcap);
-    int8_t *planes = (int8_t *)lds + groups_per_block<GL>() * eh_bytes(qcap, NARROW) + gid * plane_bytes(qcap);
+    // the group's place in the bank-interleaved block layout (see block_lds_bytes)
+    constexpr int NG = groups_per_half<GL>();
+    constexpr int G = groups_per_block<GL>();
+    const int hw = gid / NG, q = gid % NG;
+    int32_t *const lw = (int32_t *)lds;
+    const int ehw = eh_words(qcap, NARROW);
+    const EhView<NARROW, NG> eh(lw + hw * NG * ehw + q, qcap);
+    int32_t *const plane03 = lw + G * ehw + hw * NG * qcap + q;  // word of column 0, byte c = code c
+    int8_t *const plane4 = (int8_t *)(lw + G * ehw + G * qcap + (hw >> 2) * NG * qcap + q) + (hw & 3);
 
     const int32_t oe_del = sc.o_del + sc.e_del, oe_ins = sc.o_ins + sc.e_ins;
     const int32_t e_del = sc.e_del, e_ins = sc.e_ins;
@@ -255,7 +283,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 exhausted = true;
             } else {
                 const ExtJob jb = jobs[order[slot]];
-                const uint8_t *q = arena + jb.q_off;
+                const uint8_t *qry = arena + jb.q_off;
                 J.t = target_view(arena, ref, jb.t_off, jb.tlen);
                 J.qlen = (int32_t)jb.qlen;
                 J.tlen = (int32_t)jb.tlen;
@@ -268,9 +296,11 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                     eh.init(j, j <= J.qlen ? h : 0);
                 }
                 for (int j = g; j < J.qlen; j += GL) {
-                    const int qc = q[j];
-#pragma unroll
-                    for (int c = 0; c < 5; ++c) planes[c * qcap + j] = sc.mat[c * 5 + qc];
+                    const int qc = qry[j];
+                    plane03[NG * j] = (int32_t)((uint32_t)(uint8_t)sc.mat[qc] | (uint32_t)(uint8_t)sc.mat[5 + qc] << 8 |
+                                                (uint32_t)(uint8_t)sc.mat[10 + qc] << 16 |
+                                                (uint32_t)(uint8_t)sc.mat[15 + qc] << 24);
+                    plane4[4 * NG * j] = sc.mat[20 + qc];
                 }
                 J.i = 0;
                 J.beg = 0;
@@ -305,7 +335,7 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         const int32_t kr = min(((wmax + GL - 1) / GL) | 1, K);
         if (!active) continue;
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
-        const int8_t *srow = planes + __umul24((uint32_t)J.tcur, (uint32_t)qcap);
+        const int8_t *srow = J.tcur < 4 ? (const int8_t *)plane03 + J.tcur : plane4;
         // This row's target base (requested a row ago) is consumed before the next one is
         // requested: the vector-memory counter is in order, so a wait placed after the new request
         // would wait for it too, a full memory round trip every row.
