@@ -23,7 +23,9 @@ HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-Wall",
              "-I" + os.path.join(ROOT, "include"), "-I" + CSRC]
 # Per-source flags.  ext_batched.hip: without the load-store vectorizer the group kernel's score
 # bytes stay single ds_read_i8 loads; merged into wide loads they each cost a bit-field extract.
-SOURCE_FLAGS = {"ext_batched.hip": ["-mllvm", "-amdgpu-load-store-vectorizer=false"]}
+# Without the SLP vectorizer a slot's two 16-bit halves (H of one column, E of the next) stay two
+# ds_write_b16; merged, each slot's store costs a v_perm_b32 in a kernel bound by VALU issue.
+SOURCE_FLAGS = {"ext_batched.hip": ["-mllvm", "-amdgpu-load-store-vectorizer=false", "-fno-slp-vectorize"]}
 LIB = os.path.join(HERE, "libbwg.so")
 WORKLOAD_LIB = os.path.join(HERE, "libbwg_workload.so")
 

@@ -3,12 +3,14 @@
 // Two persistent kernels, each draining a device work queue of length-binned jobs:
 //
 //  * ext_group_kernel<K>: one job per 16-lane DPP row ("group"), four jobs per wavefront.  Row i
-//    of the DP is evaluated in parallel over the live window [beg, end): lane g owns the kr
-//    consecutive window columns beg + g*kr .. beg + g*kr + kr-1 (window-relative blocking), where
+//    of the DP is evaluated in parallel over the live window [beg, end): the lanes own blocks of kr
+//    consecutive columns aligned on the window's right end (window-relative blocking), where
 //    kr <= K is chosen per row as the smallest odd width that covers the widest window among the
 //    wavefront's four groups (odd keeps the 16 lanes' LDS accesses in distinct banks; K bounds
 //    the registers).  The eh arrays of the reference live in LDS over absolute columns 0..qlen, so
-//    columns outside the window keep their stale values exactly as in the reference.  M and E
+//    columns past the window keep their stale values exactly as in the reference; the slots below
+//    the window are kept at zero, so the first block's columns left of the window compute zeros
+//    and every column store is unconditional.  M and E
 //    depend only on the previous row; F is the max-plus recurrence
 //    F(j+1) = max(F(j) - e_ins, max(M(j) - oe_ins, 0)), evaluated in two passes: each lane runs it
 //    over its block from F_in = 0, the blocks are combined by an exclusive max-scan across the row
@@ -90,16 +92,18 @@ __device__ __forceinline__ void fill_subst_table(int8_t *tbl, const ExtScoring &
 // bank whatever their windows.
 template <int GL> constexpr int groups_per_half() { return 32 / GL; }
 
-// The column state of one group, the reference's eh arrays over absolute columns 0..qcap+1.  Wide:
-// H and E as int32, each in its own interleaved array.  Narrow: one word per column, H in its low
-// and E in its high 16 bits, exact whenever H and E stay below 65536; they never exceed
-// h0 + qlen*max_sc, which the planner checks for narrow bins.  Narrow state halves the group's LDS,
-// which is what limits the kernel's occupancy for wide windows.  `col0` is the group's word of
-// column 0; column j is NG words further per column.
+// The column state of one group, the reference's eh arrays over absolute columns 0..qcap+1, below
+// which K zero columns pad the left end of the row blocks.  Wide: H and E as int32, each in its
+// own interleaved array.  Narrow: one word per column, H in its low and E in its high 16 bits,
+// exact whenever H and E stay below 65536; they never exceed h0 + qlen*max_sc, which the planner
+// checks for narrow bins.  Narrow state halves the group's LDS, which is what limits the kernel's
+// occupancy for wide windows.  `col0` is the group's word of column 0; column j is NG words further
+// per column.
 template <bool NARROW, int NG> struct EhView;
 template <int NG> struct EhView<false, NG> {
     int32_t *hp, *ep;
-    __device__ __forceinline__ EhView(int32_t *col0, int qcap) : hp(col0), ep(col0 + NG * (qcap + 2)) {}
+    __device__ __forceinline__ EhView(int32_t *col0, int qcap, int pad)
+        : hp(col0), ep(col0 + NG * (pad + qcap + 2)) {}
     __device__ __forceinline__ int2 load(int j) const { return make_int2(hp[NG * j], ep[NG * j]); }
     __device__ __forceinline__ int32_t load_h(int j) const { return hp[NG * j]; }
     __device__ __forceinline__ int32_t load_e(int j) const { return ep[NG * j]; }
@@ -112,7 +116,7 @@ template <int NG> struct EhView<false, NG> {
 };
 template <int NG> struct EhView<true, NG> {
     uint16_t *p;  // H of column j at p[2*NG*j], E at p[2*NG*j + 1]
-    __device__ __forceinline__ EhView(int32_t *col0, int) : p((uint16_t *)col0) {}
+    __device__ __forceinline__ EhView(int32_t *col0, int, int) : p((uint16_t *)col0) {}
     // The widened values are pinned where they are loaded: widening a 16-bit value in a later block
     // (where the column's use sits behind a branch) costs an AND there, where ds_read_u16 has
     // already zero-extended it.
@@ -140,40 +144,47 @@ template <int NG> struct EhView<true, NG> {
 };
 
 // Block LDS layout, in 4-byte words, for G = 256/GL groups in 8 half-wavefronts of NG groups:
-//  * eh: G * eh_words(qcap) words; half-wavefront hw owns NG * eh_words words from
-//    hw * NG * eh_words, interleaved as above;
+//  * eh: G * eh_words(qcap, K) words; half-wavefront hw owns NG * eh_words words from
+//    hw * NG * eh_words, interleaved as above, column j of group q at word NG*(K + j) + q (the
+//    wide form's E array follows its H array);
 //  * score planes 0-3: G * qcap words; half-wavefront hw owns NG * qcap words from hw * NG * qcap,
 //    word NG*j + q holding mat[c][query[j]] in byte c for target codes c = 0..3;
 //  * score plane 4 (target N): G * qcap bytes; the half-wavefronts hw and hw + 1, hw + 2, hw + 3
 //    (hw a multiple of 4) share NG * qcap words, word NG*j + q, byte hw & 3, so that every plane has
-//    the same column stride of 4*NG bytes;
-//  * slack.  A lane always loads all columns of its block, including columns past the window
-//    (their values are not used), up to GL*K - 1 columns past the group's first window column,
-//    which is at most qlen: those loads run past the group's region into the next, and the slack,
-//    32*K words (the largest overrun at a stride of NG words, NG*GL = 32), keeps the last region's
-//    loads inside the allocation.
-__host__ __device__ __forceinline__ int eh_words(int qcap, bool narrow) { return (narrow ? 1 : 2) * (qcap + 2); }
+//    the same column stride of 4*NG bytes.
+// A row touches columns from end - KR >= -K up to end <= qlen: the eh pad covers the columns below 0,
+// and the score bytes read there (multiplied by a zero H) come from the region before the plane's.
+__host__ __device__ __forceinline__ int eh_words(int qcap, int K, bool narrow) {
+    return (narrow ? 1 : 2) * (K + qcap + 2);
+}
 __host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int K, bool narrow) {
     const size_t groups = (size_t)(256 / GL);
-    return groups * (4 * (size_t)eh_words(qcap, narrow) + 5 * (size_t)qcap) + (size_t)128 * K + 64;
+    return groups * (4 * (size_t)eh_words(qcap, K, narrow) + 5 * (size_t)qcap) + 64;
 }
 
-// The cells of one DP row for one group, with the row's window spread over the 16 lanes in blocks
-// of KR columns: lane g owns window columns base .. base + KR - 1, base = beg + g*KR.  `srow` is
-// the row's score plane, column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns
-// the lane's row-maximum key (h << 13 | absolute column), 0 when the lane has no window column.
-// Only window columns store: the columns past the window keep their stale eh values, which a
-// later row reads when the band widens again (by up to two columns a row), as the reference's do.
+// The cells of one DP row for one group, with the row's window [beg, end) spread over the group's
+// lanes in blocks of KR columns aligned on its right end: with nl = ceil((end - beg) / KR) lanes in
+// use, lane g < nl owns columns base .. base + KR - 1, base = end - (nl - g)*KR.  Only lane 0's
+// block can reach left of the window, by fewer than KR columns, onto columns below beg.  Every
+// slot below beg holds zero (the caller keeps it so, and pads below column 0), and on zero inputs
+// a column computes zero H and E and passes F = 0 on: those columns load zeros, store zeros back,
+// and hand the window's first column the F = 0 it starts from, so no column store needs a
+// predicate.  Columns at or past end are never touched: they keep their stale values, which a
+// later row reads when the band widens again, as the reference's do.  `srow` is the row's score
+// plane, column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns the lane's
+// row-maximum key (h << 13 | absolute column); a lane without columns returns 0, and a column
+// left of the window has h = 0, so neither can win a row whose maximum is positive.
 template <int KR, int GL, class EH>
 __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t beg, int32_t end, int g,
                                              int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
     using L = GroupLanes<GL>;
-    const int32_t base = beg + g * KR;  // first window column of this lane
-    const int32_t nval = min(max(end - base, 0), KR);
+    const int32_t nl = (end - beg + KR - 1) / KR;  // lanes with columns (<= GL; <= 0 for an empty window)
+    const bool live = g < nl;
+    // a lane without columns computes the block left of end, which it does not store
+    const int32_t base = end - (live ? nl - g : 1) * KR;
     constexpr int kSStride = 4 * groups_per_half<GL>();
     const int8_t *sb = srow + base * kSStride;
-    // pass 1: the loads are unconditional and independent, so they issue back to back; columns
-    // past the window compute values that are never used
+    // pass 1: the loads are independent, so they issue back to back
     int2 v[KR];
     int32_t s[KR];
 #pragma unroll
@@ -199,6 +210,7 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
     }
     grun -= KR * e_ins;
     // F entering this lane's block: exclusive max-scan of U_l = c_l + l*KR*e_ins over the row
+    // (lanes without columns come after every lane with columns, so they feed none of them)
     const int32_t Ke = KR * e_ins;
     const int32_t U = grun + g * Ke;
     const int32_t incl = L::scan_max_nonneg(U);
@@ -206,20 +218,24 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
     const int32_t f_in = excl - (g - 1) * Ke;
 
     int32_t key = 0;
+    int32_t hh[KR], ee[KR];
 #pragma unroll
     for (int k = 0; k < KR; ++k) {
         const int32_t f = max(f_in, gg[k]) - k * e_ins;
         const int32_t h = max(max(mm[k], v[k].y), f);
-        const int32_t e = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
-        // expected taken: the store runs under the exec mask with no branch around it
-        if (__builtin_expect(k < nval, 1)) {
-            // lane-relative column; the lane's first column is added once below
-            key = max(key, (h << kKeyShift) | k);
-            eh.set_e(base + k, e);      // E(i+1, j)
-            eh.set_h(base + k + 1, h);  // H(i, j): the diagonal of column j+1 next row
+        hh[k] = h;
+        ee[k] = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
+        // lane-relative column; the lane's first column is added once below
+        key = max(key, (h << kKeyShift) | k);
+    }
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < KR; ++k) {
+            eh.set_e(base + k, ee[k]);      // E(i+1, j)
+            eh.set_h(base + k + 1, hh[k]);  // H(i, j): the diagonal of column j+1 next row
         }
     }
-    return nval > 0 ? key + base : 0;
+    return live ? key + base : 0;
 }
 
 // row_cells at the odd block width kr (lo <= kr <= hi, both odd), chosen by a binary search
@@ -264,8 +280,8 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
     constexpr int G = groups_per_block<GL>();
     const int hw = gid / NG, q = gid % NG;
     int32_t *const lw = (int32_t *)lds;
-    const int ehw = eh_words(qcap, NARROW);
-    const EhView<NARROW, NG> eh(lw + hw * NG * ehw + q, qcap);
+    const int ehw = eh_words(qcap, K, NARROW);
+    const EhView<NARROW, NG> eh(lw + hw * NG * ehw + NG * K + q, qcap, K);
     int32_t *const plane03 = lw + G * ehw + hw * NG * qcap + q;  // word of column 0, byte c = code c
     int8_t *const plane4 = (int8_t *)(lw + G * ehw + G * qcap + (hw >> 2) * NG * qcap + q) + (hw & 3);
 
@@ -291,9 +307,10 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
                 J.out_idx = order[slot];
                 J.w = clamp_band(sc.w, J.qlen, sc);
                 // row -1 (sw.rs:263-279): H[0] = h0, H[j] = max(h0 - oe_ins - (j-1)*e_ins, 0)
-                for (int j = g; j <= J.qlen + 1; j += GL) {
+                // (and the zero pad below column 0, which the row blocks read left of the window)
+                for (int j = g - K; j <= J.qlen + 1; j += GL) {
                     const int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
-                    eh.init(j, j <= J.qlen ? h : 0);
+                    eh.init(j, j >= 0 && j <= J.qlen ? h : 0);
                 }
                 for (int j = g; j < J.qlen; j += GL) {
                     const int qc = qry[j];
@@ -334,6 +351,10 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         const int32_t wmax = L::wave_max(wwin);
         const int32_t kr = min(((wmax + GL - 1) / GL) | 1, K);
         if (!active) continue;
+        // Every slot below the window's first column holds zero, which the row blocks rely on.
+        // Tightening only moves beg past dead (zero) slots; the band moves it at most one column
+        // a row (i - w <= J.beg + 1), past a slot that may be live and that no row reads again.
+        if (g == 0 && beg > J.beg) eh.init(J.beg, 0);
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
         const int8_t *srow = J.tcur < 4 ? (const int8_t *)plane03 + J.tcur : plane4;
         // This row's target base (requested a row ago) is consumed before the next one is
