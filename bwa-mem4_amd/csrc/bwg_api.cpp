@@ -133,6 +133,7 @@ int bwg_create(int device, bwg_ctx **out) {
     if (!c) return BWG_ENOMEM;
     c->device = device;
     c->num_cus = prop.multiProcessorCount;
+    for (float &v : c->dp_share) v = -1.0f;
     // the allocation-free entry point's scratch address is looked up here, with the device current
     if (hipSetDevice(device) != hipSuccess || c->ss.create() != BWG_OK ||
         hipEventCreate(&c->na_ev[0]) != hipSuccess || hipEventCreate(&c->na_ev[1]) != hipSuccess ||

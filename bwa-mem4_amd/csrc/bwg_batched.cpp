@@ -37,7 +37,11 @@ struct Item {
     uint32_t offset = 0, count = 0, head = 0;
     int grid = 1;
     double work = 0;
+    double dp_share = -1;  // extension DP bin after the pre-pass: measured share of its jobs the DP runs
 };
+
+// Bin code of an item, the key of the context's measured DP shares.
+inline int item_code(const Item &it) { return (it.kind << 8) | ((it.param & 127) << 1) | (it.narrow ? 1 : 0); }
 
 struct Plan {
     ExtScoring esc{};      // extension: the scoring this plan's kernels run with (its band w)
@@ -986,12 +990,30 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
             e = hipEventRecord(ss.fork, ss.main);
             for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) e = hipStreamWaitEvent(ss.aux[k], ss.fork, 0);
         }
-        // largest nominal work first, each onto the stream with the least work queued
-        // (longest-processing-time-first over the stage's items)
+        // largest expected work first, each onto the stream with the least work queued
+        // (longest-processing-time-first over the stage's items).  A bin that drains a compacted
+        // queue runs only the jobs the pre-pass (or the retry filter) passes on: its nominal work is
+        // weighted by the share of its jobs measured on this batch's previous run, else on earlier
+        // batches of the context.  Short-query bins lose most of their jobs to the pre-pass; counted
+        // at their nominal work, they were dealt streams ahead of long bins.
+        std::vector<double> wexp(nitems);
+        for (int k = 0; k < nitems; ++k) {
+            const Item &it = p->items[i + k];
+            double share = 1.0;
+            if (p->compact && is_ext_dp(it.kind)) {
+                if (it.dp_share >= 0) {
+                    share = it.dp_share;
+                } else {
+                    std::lock_guard<std::mutex> g(c->dp_mu);
+                    const float v = c->dp_share[item_code(it)];
+                    if (v >= 0) share = v;
+                }
+            }
+            wexp[k] = it.work * share;
+        }
         std::vector<size_t> ord(nitems);
         std::iota(ord.begin(), ord.end(), i);
-        std::stable_sort(ord.begin(), ord.end(),
-                         [&](size_t x, size_t y) { return p->items[x].work > p->items[y].work; });
+        std::stable_sort(ord.begin(), ord.end(), [&](size_t x, size_t y) { return wexp[x - i] > wexp[y - i]; });
         double load[kAuxStreams + 1] = {};
         const bool timed = b->tu.item_timing;
         if (timed && p->ev0.size() != p->items.size()) {
@@ -1007,7 +1029,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
             int slot = 0;
             for (int k = 1; k <= kAuxStreams; ++k)
                 if (load[k] < load[slot]) slot = k;
-            load[slot] += std::max(p->items[r].work, 1.0);
+            load[slot] += std::max(wexp[r - i], 1.0);
             hipStream_t st = (!par || slot == 0) ? ss.main : ss.aux[slot - 1];
             if (timed) e = hipEventRecord(p->ev0[r], st);
             if (e == hipSuccess) e = launch_item(b, p, p->items[r], st);
@@ -1063,6 +1085,31 @@ hipError_t emit_all(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     return e;
 }
 
+// The compacted-queue lengths the last run left on the device (each bin's jobs that the ungapped
+// pre-pass or the retry filter passed to the DP), as each item's share of its jobs, recorded in the
+// plan and in the context's table for later batches.  Reads after the last run in stream order.
+hipError_t learn_plan_shares(bwg_ctx *c, Plan *p, hipStream_t st) {
+    if (!p->compact || p->n_bins == 0) return hipSuccess;
+    std::vector<uint32_t> cnt((size_t)kCountStride * p->n_bins);
+    hipError_t e = hipMemcpyAsync(cnt.data(), p->d_heads + p->n_bins, cnt.size() * 4, hipMemcpyDeviceToHost, st);
+    if (e == hipSuccess) e = hipStreamSynchronize(st);
+    if (e != hipSuccess) return e;
+    std::lock_guard<std::mutex> g(c->dp_mu);
+    for (Item &it : p->items)
+        if (is_ext_dp(it.kind) && it.count > 0) {
+            it.dp_share = std::min(1.0, (double)cnt[(size_t)kCountStride * it.head] / (double)it.count);
+            if (p->retry_round == 0) c->dp_share[item_code(it)] = (float)it.dp_share;
+        }
+    return hipSuccess;
+}
+
+hipError_t learn_dp_shares(bwg_ctx *c, bwg_batch *b, hipStream_t st) {
+    hipError_t e = b->plan ? learn_plan_shares(c, (Plan *)b->plan, st) : hipSuccess;
+    for (void *q : b->retry_plans)
+        if (e == hipSuccess) e = learn_plan_shares(c, (Plan *)q, st);
+    return e;
+}
+
 void free_plan(bwg_ctx *c, Plan *p) {
     for (size_t k = 0; k < p->ev0.size(); ++k) {
         if (p->ev0[k]) (void)hipEventDestroy(p->ev0[k]);
@@ -1089,6 +1136,12 @@ int enqueue_batched(bwg_ctx *c, bwg_batch *b, StreamSet &ss, int64_t *launches) 
     int rc = plan_batch(c, b, ss.main);
     if (rc) return rc;
     hipError_t e = hipSuccess;
+    // Before the launch sequence is captured, learn from the eager run how many jobs each compacted
+    // bin actually ran, so that the captured sequence deals the bins by their real work.
+    if (b->tu.use_graphs && !b->tu.item_timing && !b->graph && b->batched_runs >= 1 && b->kind == 0) {
+        e = learn_dp_shares(c, b, ss.main);
+        if (e != hipSuccess) return bwg_map_hip(e);
+    }
     // Capture the launch sequence once the batch has run eagerly; the eager run is also the
     // reference every later replay is compared with by the tests.
     if (b->tu.use_graphs && !b->tu.item_timing && !b->graph && b->batched_runs >= 1) {

@@ -216,6 +216,12 @@ struct bwg_ctx {
         return tuning;
     }
     bwg::Tuner tuner;
+    // share of a bin's jobs that reach the DP after the extension's ungapped pre-pass, by bin code
+    // (kind << 8 | param << 1 | 16-bit state), measured on earlier runs; < 0 when not yet seen.  It
+    // weights the bins' nominal work when a launch sequence deals them onto the streams.
+    static constexpr int kDpShareCodes = 16 << 8;
+    std::mutex dp_mu;
+    float dp_share[kDpShareCodes];
     std::atomic<int64_t> launches{0}, jobs{0};
     std::atomic<int64_t> last_kernel_ns{0}, last_main_ns{0}, last_main_launches{0};
     std::atomic<int64_t> phase_ns[bwg::kNumPhases] = {};

@@ -163,25 +163,26 @@ __host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int
 }
 
 // The cells of one DP row for one group, with the row's window [beg, end) spread over the group's
-// lanes in blocks of KR columns aligned on its right end: with nl = ceil((end - beg) / KR) lanes in
-// use, lane g < nl owns columns base .. base + KR - 1, base = end - (nl - g)*KR.  Only lane 0's
-// block can reach left of the window, by fewer than KR columns, onto columns below beg.  Every
-// slot below beg holds zero (the caller keeps it so, and pads below column 0), and on zero inputs
-// a column computes zero H and E and passes F = 0 on: those columns load zeros, store zeros back,
-// and hand the window's first column the F = 0 it starts from, so no column store needs a
-// predicate.  Columns at or past end are never touched: they keep their stale values, which a
-// later row reads when the band widens again, as the reference's do.  `srow` is the row's score
+// lanes in blocks of KR columns aligned on its right end: lane g owns the columns base .. base +
+// KR - 1, base = end - (GL - g)*KR, and takes part when its block reaches into the window (the
+// group's KR*GL columns cover it).  Only the first such lane's block can reach left of the window,
+// by fewer than KR columns, onto columns below beg.  Every slot below beg holds zero (the caller
+// keeps it so, and pads below column 0), and on zero inputs a column computes zero H and E and
+// passes F = 0 on: those columns load zeros, store zeros back, and hand the window's first column
+// the F = 0 it starts from, so no column store needs a predicate.  Columns at or past end are never
+// touched: they keep their stale values, which a later row reads when the band widens again, as
+// the reference's do.  Lanes whose block lies left of the window compute the last block's columns
+// again, store nothing and feed 0 to the F scan, whose identity it is.  `srow` is the row's score
 // plane, column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns the lane's
-// row-maximum key (h << 13 | absolute column); a lane without columns returns 0, and a column
+// row-maximum key (h << 13 | absolute column); a lane outside the window returns 0, and a column
 // left of the window has h = 0, so neither can win a row whose maximum is positive.
 template <int KR, int GL, class EH>
 __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t beg, int32_t end, int g,
                                              int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
     using L = GroupLanes<GL>;
-    const int32_t nl = (end - beg + KR - 1) / KR;  // lanes with columns (<= GL; <= 0 for an empty window)
-    const bool live = g < nl;
-    // a lane without columns computes the block left of end, which it does not store
-    const int32_t base = end - (live ? nl - g : 1) * KR;
+    const int32_t rgap = (GL - 1 - g) * KR;  // columns between this lane's block and the window's end
+    const bool live = rgap < end - beg;
+    const int32_t base = end - KR - (live ? rgap : 0);
     constexpr int kSStride = 4 * groups_per_half<GL>();
     const int8_t *sb = srow + base * kSStride;
     // pass 1: the loads are independent, so they issue back to back
@@ -209,10 +210,11 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
         grun = max(max(grun, m + ((k + 1) * e_ins - oe_ins)), (k + 1) * e_ins);
     }
     grun -= KR * e_ins;
-    // F entering this lane's block: exclusive max-scan of U_l = c_l + l*KR*e_ins over the row
-    // (lanes without columns come after every lane with columns, so they feed none of them)
+    // F entering this lane's block: exclusive max-scan of U_l = c_l + l*KR*e_ins over the row (a
+    // lane left of the window feeds 0; the first lane in the window then starts from F <= 0, which
+    // the F floor in gg turns into the F = 0 a window starts from)
     const int32_t Ke = KR * e_ins;
-    const int32_t U = grun + g * Ke;
+    const int32_t U = live ? grun + g * Ke : 0;
     const int32_t incl = L::scan_max_nonneg(U);
     const int32_t excl = L::shift_in(incl, -Ke);
     const int32_t f_in = excl - (g - 1) * Ke;
@@ -353,20 +355,21 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         if (!active) continue;
         // Every slot below the window's first column holds zero, which the row blocks rely on.
         // Tightening only moves beg past dead (zero) slots; the band moves it at most one column
-        // a row (i - w <= J.beg + 1), past a slot that may be live and that no row reads again.
-        if (g == 0 && beg > J.beg) eh.init(J.beg, 0);
+        // a row (i - w <= J.beg + 1), past a slot that may be live and that no row reads again:
+        // slot beg - 1 is cleared every row, which is that slot when the band moved beg and a zero
+        // slot (or the pad) otherwise.  The group's lanes all store the same value to the same
+        // address, so the edge stores need no exec mask.
+        eh.init(beg - 1, 0);
         const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
         const int8_t *srow = J.tcur < 4 ? (const int8_t *)plane03 + J.tcur : plane4;
         // This row's target base (requested a row ago) is consumed before the next one is
         // requested: the vector-memory counter is in order, so a wait placed after the new request
         // would wait for it too, a full memory round trip every row.
         asm volatile("" ::"v"(J.tcur) : "memory");
-        const int32_t tnext = i + 1 < J.tlen ? J.t[i + 1] : 0;  // prefetched for the next row
+        const int32_t tnext = J.t[min(i + 1, J.tlen - 1)];  // prefetched for the next row
         int32_t key = row_cells_at<1, K, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
-        if (g == 0) {
-            eh.set_h(beg, h1_init);  // H(i, beg-1)
-            eh.set_e(end, 0);        // E at the sentinel column
-        }
+        eh.set_h(beg, h1_init);  // H(i, beg-1)
+        eh.set_e(end, 0);        // E at the sentinel column
         wave_lds_sync();
         key = L::allmax(key);  // h << 13 | absolute column
         const int32_t row_max = key >> kKeyShift;
@@ -376,20 +379,20 @@ __global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restric
         const int32_t h1_last = eh.load_h(end);
         const int32_t e_beg = eh.load_e(beg);
 
-        bool done = false;
         const int32_t j_exit = beg < end ? end : beg;
         if (j_exit == J.qlen && J.gscore <= h1_last) {
             J.gbest_i = i;
             J.gscore = h1_last;
         }
-        if (row_max == 0) {
-            done = true;
-        } else if (row_max > J.best) {
-            J.best = row_max;
-            J.best_i = i;
-            J.best_j = mj;
-            J.max_off = max(J.max_off, abs(mj - i));
-        } else if (sc.zdrop > 0 && J.best - row_max > sc.zdrop) {
+        // a row whose maximum is 0 ends the extension; a greater maximum moves the best cell
+        // (selects, not a branch: it is taken on most rows); a lower one may trigger z-drop
+        bool done = row_max == 0;
+        const bool improve = row_max > J.best;
+        J.max_off = improve ? max(J.max_off, abs(mj - i)) : J.max_off;
+        J.best_i = improve ? i : J.best_i;
+        J.best_j = improve ? mj : J.best_j;
+        J.best = improve ? row_max : J.best;
+        if (!done && !improve && sc.zdrop > 0 && J.best - row_max > sc.zdrop) {
             // the gap term is never negative, so a drop within zdrop cannot end the extension
             const int32_t di = i - J.best_i, dj = mj - J.best_j;
             const int32_t dd = di > dj ? di - dj : dj - di, ge = di > dj ? e_del : e_ins;
