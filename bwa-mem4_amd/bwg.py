@@ -39,6 +39,7 @@ OPT_ITEM_TIMING, OPT_COALESCE = 14, 15
 OPT_HOST_THREADS = 16
 OPT_EXT_GROUP_LANES = 17
 OPT_KSW_GROUP_LANES = 18
+OPT_BIN_STREAMS = 19
 STAT_LAUNCHES, STAT_JOBS, STAT_LAST_KERNEL_NS, STAT_LAST_MAIN_NS, STAT_LAST_MAIN_LAUNCHES = 100, 101, 102, 103, 104
 STAT_POOL_IN_USE, STAT_POOL_CACHED, STAT_POOL_PEAK = 105, 106, 107
 # host wall time of the phases of the last synchronous extension / rescue call (ns)

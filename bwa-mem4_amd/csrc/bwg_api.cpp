@@ -240,6 +240,10 @@ int bwg_set_option(bwg_ctx *c, int key, int64_t v) {
         if (v != 8 && v != 16) return BWG_EINVAL;
         t.ksw_group_lanes = (int)v;
         return BWG_OK;
+    case BWG_OPT_BIN_STREAMS:
+        if (v < 1 || v > kAuxStreams + 1) return BWG_EINVAL;
+        t.bin_streams = (int)v;
+        return BWG_OK;
     default: return BWG_EINVAL;
     }
 }
@@ -276,6 +280,7 @@ int64_t bwg_get_stat(const bwg_ctx *c, int key) {
     case BWG_OPT_HOST_THREADS: return t.host_threads;
     case BWG_OPT_EXT_GROUP_LANES: return t.ext_group_lanes;
     case BWG_OPT_KSW_GROUP_LANES: return t.ksw_group_lanes;
+    case BWG_OPT_BIN_STREAMS: return t.bin_streams;
     case BWG_STAT_LAUNCHES: return c->launches.load();
     case BWG_STAT_JOBS: return c->jobs.load();
     case BWG_STAT_LAST_KERNEL_NS: return c->last_kernel_ns.load();

@@ -37,6 +37,8 @@ struct Item {
     uint32_t offset = 0, count = 0, head = 0;
     int grid = 1;
     double work = 0;
+    double est_ns = 0;     // extension DP bin: the autotuner's time per job of each job's class, summed
+    bool est_all = false;  // every job of the bin has a tuned class (est_ns is its whole time alone)
     double dp_share = -1;  // extension DP bin after the pre-pass: measured share of its jobs the DP runs
 };
 
@@ -82,6 +84,7 @@ inline size_t seg_word(const Plan *p) { return (size_t)p->n_bins * (1 + kCountSt
 constexpr size_t kTuneMinJobs = 4096;
 constexpr size_t kTuneSample = 65536;
 constexpr int kTuneReps = 3;
+constexpr double kTuneTie = 1.03;
 
 constexpr int kExtKs[] = {3, 5, 7, 9, 11, 13, 15, 17, 21, 25, 33};
 constexpr int kNumExtKs = (int)(sizeof(kExtKs) / sizeof(kExtKs[0]));
@@ -392,10 +395,20 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, const ExtScoring &esc, uint64_t sig
     if (e1) (void)hipEventDestroy(e1);
     c->pool.free(d);
     if (e != hipSuccess) return bwg_map_hip(e);
+    // The first candidate (in the order of ext_candidates: the minimal K with 16-bit state first)
+    // within kTuneTie of the fastest: near-ties are decided by timing noise, and the minimal K
+    // needs the fewest registers and the least LDS, and merges a class into the bin of its
+    // neighbour's minimal K, which the batch's concurrent bins run better than an extra bin.
+    double fastest = 1e30;
     for (size_t k = 0; k < best.size(); ++k) {
         ent.cands[k].ns_per_job = best[k];
-        if (best[k] < best[ent.choice]) ent.choice = (int)k;
+        fastest = std::min(fastest, best[k]);
     }
+    for (size_t k = 0; k < best.size(); ++k)
+        if (best[k] <= fastest * kTuneTie) {
+            ent.choice = (int)k;
+            break;
+        }
     c->tuner.store(sig, cls, ent);
     return BWG_OK;
 }
@@ -430,6 +443,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
     });
     const uint64_t sig = scoring_sig(sc);
     std::vector<int> choice_kind(n_classes, -1), choice_param(n_classes, 0), choice_width(n_classes, 16);
+    std::vector<double> choice_ns(n_classes, 0.0);
     for (int cl = 0; cl < n_classes; ++cl) {
         if (!tu.autotune) break;
         size_t n_members = 0;
@@ -448,6 +462,7 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             choice_kind[cl] = ent.cands[ent.choice].kind;
             choice_param[cl] = ent.cands[ent.choice].param;
             choice_width[cl] = ent.cands[ent.choice].width;
+            choice_ns[cl] = ent.cands[ent.choice].ns_per_job;
         }
     }
     // Each job's kernel mapping as a code (kind << 8 | param << 1 | narrow), a pure function of its
@@ -482,6 +497,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         uint32_t count = 0;
         int32_t qcap = 0;
         double work = 0.0;
+        double est_ns = 0.0;
+        uint32_t untimed = 0;  // jobs without a tuned class
     };
     std::vector<CodeAcc> acc((size_t)parts * kCodes);
     HostVec<uint16_t> code(n), wk(n);
@@ -495,6 +512,9 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             wk[k] = (uint16_t)(65535 - std::min<uint32_t>(work_key(w), 65535));
             a[cd].count++;
             a[cd].work += (double)w;
+            const int cl = cls[k];
+            if (cl >= 0 && choice_ns[cl] > 0.0) a[cd].est_ns += choice_ns[cl];
+            else a[cd].untimed++;
             if (is_ext_group((int)(cd >> 8))) a[cd].qcap = std::max(a[cd].qcap, qcap_class((int)j.qlen));
         }
     });
@@ -505,6 +525,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
             const CodeAcc &a = acc[(size_t)q * kCodes + cd];
             t.count += a.count;
             t.work += a.work;
+            t.est_ns += a.est_ns;
+            t.untimed += a.untimed;
             t.qcap = std::max(t.qcap, a.qcap);
         }
         if (!t.count) continue;
@@ -512,6 +534,8 @@ int plan_ext(bwg_ctx *c, bwg_batch *b, Plan *p, hipStream_t st) {
         const int bi = find(kind, (cd >> 1) & 127, (cd & 1) != 0);
         bins[bi].count += t.count;
         bins[bi].work += t.work;
+        bins[bi].est_ns += t.est_ns;
+        bins[bi].est_all = (bins[bi].count == t.count) ? t.untimed == 0 : bins[bi].est_all && t.untimed == 0;
         if (is_ext_group(kind)) bins[bi].qcap = std::max(bins[bi].qcap, t.qcap);
         bin_of_code[cd] = (int16_t)bi;
     }
@@ -672,10 +696,20 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
     c->pool.free(dj);
     c->pool.free(drm);
     if (e != hipSuccess) return bwg_map_hip(e);
+    // The first candidate (in the order of ext_candidates: the minimal K with 16-bit state first)
+    // within kTuneTie of the fastest: near-ties are decided by timing noise, and the minimal K
+    // needs the fewest registers and the least LDS, and merges a class into the bin of its
+    // neighbour's minimal K, which the batch's concurrent bins run better than an extra bin.
+    double fastest = 1e30;
     for (size_t k = 0; k < best.size(); ++k) {
         ent.cands[k].ns_per_job = best[k];
-        if (best[k] < best[ent.choice]) ent.choice = (int)k;
+        fastest = std::min(fastest, best[k]);
     }
+    for (size_t k = 0; k < best.size(); ++k)
+        if (best[k] <= fastest * kTuneTie) {
+            ent.choice = (int)k;
+            break;
+        }
     c->tuner.store(sig, cls, ent);
     return BWG_OK;
 }
@@ -996,6 +1030,21 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
         // weighted by the share of its jobs measured on this batch's previous run, else on earlier
         // batches of the context.  Short-query bins lose most of their jobs to the pre-pass; counted
         // at their nominal work, they were dealt streams ahead of long bins.
+        // When every DP bin of the stage has a time per job from the autotuner for each of its jobs,
+        // the bins are dealt by that time (the same share applied) rather than by cells: a bin of
+        // short, narrow rows costs more per cell than one of long, wide rows.
+        // Other items of such a stage (the plain bin) are converted at the timed bins' mean time
+        // per nominal cell.
+        bool timed_stage = true;
+        double t_ns = 0.0, t_work = 0.0;
+        for (int k = 0; k < nitems; ++k) {
+            const Item &it = p->items[i + k];
+            if (!is_ext_dp(it.kind)) continue;
+            if (!(it.est_all && it.est_ns > 0)) timed_stage = false;
+            t_ns += it.est_ns;
+            t_work += it.work;
+        }
+        const double ns_per_cell = t_work > 0 ? t_ns / t_work : 0.0;
         std::vector<double> wexp(nitems);
         for (int k = 0; k < nitems; ++k) {
             const Item &it = p->items[i + k];
@@ -1009,7 +1058,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
                     if (v >= 0) share = v;
                 }
             }
-            wexp[k] = it.work * share;
+            wexp[k] = (!timed_stage ? it.work : is_ext_dp(it.kind) ? it.est_ns : it.work * ns_per_cell) * share;
         }
         std::vector<size_t> ord(nitems);
         std::iota(ord.begin(), ord.end(), i);
@@ -1027,7 +1076,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
         for (size_t r : ord) {
             if (e != hipSuccess) break;
             int slot = 0;
-            for (int k = 1; k <= kAuxStreams; ++k)
+            for (int k = 1; k < std::min(b->tu.bin_streams, kAuxStreams + 1); ++k)
                 if (load[k] < load[slot]) slot = k;
             load[slot] += std::max(wexp[r - i], 1.0);
             hipStream_t st = (!par || slot == 0) ? ss.main : ss.aux[slot - 1];

@@ -129,6 +129,7 @@ struct Tuning {
     int host_threads = 0;          // BWG_OPT_HOST_THREADS: 0 = the CPUs this process may use, at most 16
     int ext_group_lanes = 16;      // BWG_OPT_EXT_GROUP_LANES: lanes per job of untuned group bins (16 or 8)
     int ksw_group_lanes = 16;      // BWG_OPT_KSW_GROUP_LANES: lanes per job of untuned packed rescue bins
+    int bin_streams = kAuxStreams + 1;  // BWG_OPT_BIN_STREAMS: streams a stage's concurrent bins are dealt onto
     int form = BWG_FORM_BATCHED;   // BWG_OPT_FORM
 };
 

@@ -151,6 +151,7 @@ enum {
     BWG_OPT_HOST_THREADS = 16,    /* host threads for a call's validation and planning (0: auto, calls of 64k+ jobs; 1..16: that many, 1024+ jobs a thread) */
     BWG_OPT_EXT_GROUP_LANES = 17, /* untuned extension group bins: 16 (default) or 8 lanes per job, eight jobs a wavefront */
     BWG_OPT_KSW_GROUP_LANES = 18, /* untuned packed rescue bins: 16 (default) or 8 lanes per job, eight jobs a wavefront */
+    BWG_OPT_BIN_STREAMS = 19,     /* batched: streams a stage's concurrent bins are dealt onto (1..4, default 4) */
     BWG_STAT_LAUNCHES = 100,    /* device kernel launches issued by this context (proof the GPU ran) */
     BWG_STAT_JOBS = 101,        /* jobs completed on the device */
     BWG_STAT_LAST_KERNEL_NS = 102, /* device time of the last batch's kernels (HIP events) */
