@@ -262,8 +262,18 @@ constexpr int groups_per_block() {
     return 256 / GL;
 }
 
+// Waves per SIMD the register allocation aims for: the 16-lane K = 11 kernel with 16-bit state
+// fits 96 registers (5 waves; its LDS at the class's query lengths allows 5-6 blocks a CU, one
+// register spilled), where it would otherwise take 100 (4 waves).  The same push for K = 7 and
+// K = 9 (6 waves at 80 registers) was slower.
 template <int K, bool NARROW, int GL>
-__global__ __launch_bounds__(256) void ext_group_kernel(const uint8_t *__restrict__ arena,
+constexpr int ext_group_min_waves() {
+    return (K == 11 && NARROW && GL == 16) ? 5 : 1;
+}
+
+template <int K, bool NARROW, int GL>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ext_group_min_waves<K, NARROW, GL>())))
+void ext_group_kernel(const uint8_t *__restrict__ arena,
                                                         const uint8_t *__restrict__ ref,
                                                         const ExtJob *__restrict__ jobs,
                                                         const uint32_t *__restrict__ order,
