@@ -100,6 +100,17 @@ void orc_align2_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, con
                       int32_t max_sc, int batched_semantics, orc_ksw_result *out,
                       int64_t *cells_out, int nthreads);
 
+/* SIMD CPU baseline (ksw_simd.c): the reference's inter-sequence form on AVX2, 16 jobs per chunk,
+ * the same results as orc_extend_batch and orc_align2_batch (batched semantics). */
+void orc_simd_extend_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                           const uint64_t *t_off, const uint32_t *t_len, const int32_t *h0, int32_t m,
+                           const int8_t *mat, int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins,
+                           int32_t w, int32_t end_bonus, int32_t zdrop, orc_ext_result *out, int nthreads);
+void orc_simd_align2_batch(size_t n, const uint8_t *arena, const uint64_t *q_off, const uint32_t *q_len,
+                           const uint64_t *t_off, const uint32_t *t_len, int32_t m, const int8_t *mat,
+                           int32_t o_del, int32_t e_del, int32_t o_ins, int32_t e_ins, int32_t minsc,
+                           int32_t max_sc, orc_ksw_result *out, int nthreads);
+
 /* Lockstep twins (ksw_lockstep.c): crates/bwa-neon/src/batched.rs:947-1251 and
  * crates/bwa-neon/src/matesw.rs:933-1232, `lanes` jobs per pass in SoA layout. */
 void orc_lockstep_extend(size_t n, const uint8_t *const *queries, const int32_t *qlens,

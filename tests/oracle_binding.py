@@ -120,6 +120,12 @@ def lib():
                                        C.c_int]
         L.orc_brute_global.restype = _I
         L.orc_brute_global.argtypes = [_I, _P, _I, _P, _P, _I, _I, _I, _I]
+        L.orc_simd_extend_batch.restype = None
+        L.orc_simd_extend_batch.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _I, _P,
+                                            _I, _I, _I, _I, _I, _I, _I, _P, C.c_int]
+        L.orc_simd_align2_batch.restype = None
+        L.orc_simd_align2_batch.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _I, _P,
+                                            _I, _I, _I, _I, _I, _I, _P, C.c_int]
         L.orc_run_side.restype = None
         L.orc_run_side.argtypes = [C.c_size_t, _P, _P, _P, _P, _P, _P, _P, _I, _P, _I, _I, _I, _I, _I, _I, _I,
                                    _I, _P, _P, C.c_int]
@@ -389,6 +395,28 @@ def align2_batch(batch, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, batched=
                            o_del, e_del, o_ins, e_ins, minsc, max_sc, 1 if batched else 0,
                            out.ctypes.data, C.addressof(cells), nthreads)
     return (out, cells.value) if want_cells else out
+
+
+def simd_extend_batch(batch, mat, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, nthreads=8):
+    """The SIMD CPU baseline (oracle/ksw_simd.c): the same results as extend_batch."""
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=EXT_DTYPE)
+    ma, mp = _mat(mat)
+    lib().orc_simd_extend_batch(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                                batch.t_off.ctypes.data, batch.t_len.ctypes.data, batch.h0.ctypes.data,
+                                5, mp, o_del, e_del, o_ins, e_ins, w, end_bonus, zdrop, out.ctypes.data, nthreads)
+    return out
+
+
+def simd_align2_batch(batch, mat, o_del, e_del, o_ins, e_ins, minsc, max_sc, nthreads=8):
+    """The SIMD CPU baseline: the same results as align2_batch with batched semantics."""
+    n = len(batch.q_len)
+    out = np.zeros(n, dtype=KSW_DTYPE)
+    ma, mp = _mat(mat)
+    lib().orc_simd_align2_batch(n, batch.arena.ctypes.data, batch.q_off.ctypes.data, batch.q_len.ctypes.data,
+                                batch.t_off.ctypes.data, batch.t_len.ctypes.data, 5, mp,
+                                o_del, e_del, o_ins, e_ins, minsc, max_sc, out.ctypes.data, nthreads)
+    return out
 
 
 def _ptr_arrays(seqs):
