@@ -85,6 +85,10 @@ constexpr size_t kTuneMinJobs = 4096;
 constexpr size_t kTuneSample = 65536;
 constexpr int kTuneReps = 3;
 constexpr double kTuneTie = 1.03;
+// Short-query classes: the ungapped pre-pass resolves most of their jobs, so their bins run few
+// wavefronts and end as latency-bound tails of the batch; one common bin is worth more than a few
+// percent per class, measured on all jobs of the class.
+constexpr double kTuneTieShort = 1.10;
 
 constexpr int kExtKs[] = {3, 5, 7, 9, 11, 13, 15, 17, 21, 25, 33};
 constexpr int kNumExtKs = (int)(sizeof(kExtKs) / sizeof(kExtKs[0]));
@@ -313,8 +317,9 @@ struct ExtClassifier {
     }
 };
 
-// Candidate mappings of a class, the default first: lanes per job (thread or group kernel), the
-// group kernel's columns per lane (minimal K or the next), and the stored score width.
+// Candidate mappings of a class, in the order the near-tie rule prefers: lanes per job (thread or
+// group kernel), the group kernel's columns per lane (minimal K or the next), and the stored score
+// width.
 std::vector<TuneCandidate> ext_candidates(int cls) {
     std::vector<TuneCandidate> v;
     auto add = [&](int kind, int param, int width) {
@@ -325,9 +330,12 @@ std::vector<TuneCandidate> ext_candidates(int cls) {
         v.push_back(t);
     };
     if (cls < kExtShortClasses) {
+        // one mapping common to every short class first: the near-tie rule then gathers them into
+        // one bin (after the pre-pass each holds few jobs, and every extra bin is a latency-bound
+        // tail of the batch)
         const int qmax = 4 * cls + 4;
-        if (qmax <= 16) add(kExtThread, 16, 16);
         add(kExtThread, 32, 16);
+        if (qmax <= 16) add(kExtThread, 16, 16);
         add(kExtGroup, kExtKs[0], 32);
         add(kExtThread, qmax <= 16 ? 16 : 32, 32);
     } else {
@@ -395,17 +403,19 @@ int tune_ext_class(bwg_ctx *c, bwg_batch *b, const ExtScoring &esc, uint64_t sig
     if (e1) (void)hipEventDestroy(e1);
     c->pool.free(d);
     if (e != hipSuccess) return bwg_map_hip(e);
-    // The first candidate (in the order of ext_candidates: the minimal K with 16-bit state first)
-    // within kTuneTie of the fastest: near-ties are decided by timing noise, and the minimal K
-    // needs the fewest registers and the least LDS, and merges a class into the bin of its
-    // neighbour's minimal K, which the batch's concurrent bins run better than an extra bin.
+    // The first candidate (in the order of ext_candidates) within kTuneTie of the fastest:
+    // near-ties are decided by timing noise, and the first candidates (the minimal K with 16-bit
+    // state for window classes, one mapping common to every short class) need the fewest
+    // registers and the least LDS and merge neighbouring classes into one bin, which the batch's
+    // concurrent bins run better than extra bins.
     double fastest = 1e30;
     for (size_t k = 0; k < best.size(); ++k) {
         ent.cands[k].ns_per_job = best[k];
         fastest = std::min(fastest, best[k]);
     }
+    const double tie = cls < kExtShortClasses ? kTuneTieShort : kTuneTie;
     for (size_t k = 0; k < best.size(); ++k)
-        if (best[k] <= fastest * kTuneTie) {
+        if (best[k] <= fastest * tie) {
             ent.choice = (int)k;
             break;
         }
@@ -696,20 +706,10 @@ int tune_ksw_class(bwg_ctx *c, bwg_batch *b, uint64_t sig, int cls, const std::v
     c->pool.free(dj);
     c->pool.free(drm);
     if (e != hipSuccess) return bwg_map_hip(e);
-    // The first candidate (in the order of ext_candidates: the minimal K with 16-bit state first)
-    // within kTuneTie of the fastest: near-ties are decided by timing noise, and the minimal K
-    // needs the fewest registers and the least LDS, and merges a class into the bin of its
-    // neighbour's minimal K, which the batch's concurrent bins run better than an extra bin.
-    double fastest = 1e30;
     for (size_t k = 0; k < best.size(); ++k) {
         ent.cands[k].ns_per_job = best[k];
-        fastest = std::min(fastest, best[k]);
+        if (best[k] < best[ent.choice]) ent.choice = (int)k;
     }
-    for (size_t k = 0; k < best.size(); ++k)
-        if (best[k] <= fastest * kTuneTie) {
-            ent.choice = (int)k;
-            break;
-        }
     c->tuner.store(sig, cls, ent);
     return BWG_OK;
 }

@@ -122,8 +122,16 @@ inline KswScoring ksw_scoring(const bwg_ksw_params &p) {
 __host__ __device__ inline int32_t clamp_band(int32_t w, int32_t qlen, const ExtScoring &s) {
     int64_t base = (int64_t)qlen * s.max_sc + s.end_bonus;
     int64_t ni = base - s.o_ins, nd = base - s.o_del;
-    int32_t a = ni <= 0 ? 1 : (int32_t)(ni / s.e_ins + 1);
-    int32_t b = nd <= 0 ? 1 : (int32_t)(nd / s.e_del + 1);
+    // With qlen < 2^24, max_sc < 128 and bwa's non-negative penalties a positive numerator fits 32
+    // unsigned bits, and the quotient is a 32-bit division (a 64-bit one costs the GPU about a
+    // hundred instructions at every job start); the result is the 64-bit expression's either way
+    constexpr int64_t k32 = (int64_t)1 << 32;
+    int32_t a = ni <= 0 ? 1
+                : ni < k32 ? (int32_t)((int64_t)((uint32_t)ni / (uint32_t)s.e_ins) + 1)
+                           : (int32_t)(ni / s.e_ins + 1);
+    int32_t b = nd <= 0 ? 1
+                : nd < k32 ? (int32_t)((int64_t)((uint32_t)nd / (uint32_t)s.e_del) + 1)
+                           : (int32_t)(nd / s.e_del + 1);
     if (w > a) w = a;
     if (w > b) w = b;
     return w;

@@ -225,12 +225,13 @@ def test_autotuner_tunes_once_and_stays_exact():
             assert len(times) >= 2 and all(t > 0 for t in times), l
             chosen = float(re.search(r"\*\S+ ([0-9.]+) ns/job", l).group(1))
             if l.startswith("extend"):
-                # extension classes: the first candidate within 3% of the fastest (near-ties go to
-                # the minimal K); the report rounds to 0.001 ns
+                # extension classes: the first candidate within 3% of the fastest (10% for the
+                # short-query classes); the report rounds to 0.001 ns
+                tie = 1.10 if l.startswith("extend qlen") else 1.03
                 k = times.index(chosen)
                 eps = 0.0015
-                assert chosen <= 1.03 * min(times) + eps, l
-                assert all(t > 1.03 * min(times) - eps for t in times[:k]), l
+                assert chosen <= tie * min(times) + eps, l
+                assert all(t > tie * min(times) - eps for t in times[:k]), l
             else:
                 assert chosen == min(times), l
         # a second batch with the same scoring reuses the table and gives the same results
