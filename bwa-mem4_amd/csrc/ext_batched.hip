@@ -42,13 +42,17 @@ constexpr int kKeyMask = (1 << kKeyShift) - 1;
 struct GroupJob {
     // per-job constants (identical in the 16 lanes of the group)
     TargetView t;
-    int32_t qlen, tlen, h0, w;
+    int32_t qlen, tlast, h0, w;  // tlast = tlen - 1
     uint32_t out_idx;
     // row state
     int32_t i, beg, end, best, best_i, best_j, gbest_i, gscore, max_off;
     int32_t tcur;  // target code of row i
-    int32_t h1base;  // h0 - (o_del + e_del*(i+1)), kept as a running difference
+    uint32_t h1c;  // max(h0 - (o_del + e_del*(i+1)), 0), decremented with saturation row by row
 };
+
+// The row end of a group without a job: its window (end - beg) is negative whatever beg is, so
+// the wavefront's widest window ignores it.
+constexpr int32_t kIdleEnd = INT32_MIN / 2;
 
 // Column state.  The reference's eh arrays hold {H[j], E[j]} per absolute column.  Wide storage
 // keeps them as two int32; narrow storage packs H into the low and E into the high 16 bits of one
@@ -174,8 +178,12 @@ __host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int
 // the reference's do.  Lanes whose block lies left of the window compute the last block's columns
 // again, store nothing and feed 0 to the F scan, whose identity it is.  `srow` is the row's score
 // plane, column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns the lane's
-// row-maximum key (h << 13 | absolute column); a lane outside the window returns 0, and a column
-// left of the window has h = 0, so neither can win a row whose maximum is positive.
+// row-maximum key (h << 13 | absolute column).  A column left of the window has h = 0, so it cannot
+// win a row whose maximum is positive.  A lane left of the window returns the key of the last
+// block's columns computed with an incoming F <= 0; the last lane computes the same columns from
+// the same loads with its true incoming F, which is never lower, so its key is never below the
+// dead lane's.  When the window is empty every column is below beg and every h is 0, so the key
+// is below 1 << 13 (negative where the columns are below 0).
 template <int KR, int GL, class EH>
 __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t beg, int32_t end, int g,
                                              int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
@@ -228,7 +236,7 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
         hh[k] = h;
         ee[k] = max(max(v[k].y - e_del, mm[k] - oe_del), 0);
         // lane-relative column; the lane's first column is added once below
-        key = max(key, (h << kKeyShift) | k);
+        key = k == 0 ? h << kKeyShift : max(key, (h << kKeyShift) | k);
     }
     if (live) {
 #pragma unroll
@@ -237,7 +245,7 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
             eh.set_h(base + k + 1, hh[k]);  // H(i, j): the diagonal of column j+1 next row
         }
     }
-    return live ? key + base : 0;
+    return key + base;
 }
 
 // row_cells at the odd block width kr (lo <= kr <= hi, both odd), chosen by a binary search
@@ -309,12 +317,13 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
             const uint32_t slot = slots.take(head);
             if (slot >= count) {
                 exhausted = true;
+                J.end = kIdleEnd;  // a done group takes a new job or gets here on the next pass
             } else {
                 const ExtJob jb = jobs[order[slot]];
                 const uint8_t *qry = arena + jb.q_off;
                 J.t = target_view(arena, ref, jb.t_off, jb.tlen);
                 J.qlen = (int32_t)jb.qlen;
-                J.tlen = (int32_t)jb.tlen;
+                J.tlast = (int32_t)jb.tlen - 1;
                 J.h0 = jb.h0;
                 J.out_idx = order[slot];
                 J.w = clamp_band(sc.w, J.qlen, sc);
@@ -337,12 +346,17 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
                 J.best = J.h0;
                 J.best_i = J.best_j = J.gbest_i = J.gscore = -1;
                 J.max_off = 0;
-                J.tcur = J.tlen > 0 ? J.t[0] : 0;
-                J.h1base = (int32_t)((uint32_t)J.h0 - (uint32_t)sc.o_del - (uint32_t)e_del);
-                active = J.tlen > 0;
-                if (!active && g == 0) {
-                    bwg_ext_result r = {J.h0, 0, 0, 0, -1, 0};
-                    out[J.out_idx] = r;
+                J.tcur = jb.tlen > 0 ? J.t[0] : 0;
+                // H(i, -1) = max(h0 - (o_del + e_del*(i+1)), 0): its value at row 0, then minus
+                // e_del a row, saturating at 0 (e_del >= 1, so it never rises again)
+                J.h1c = (uint32_t)max((int32_t)((uint32_t)J.h0 - (uint32_t)sc.o_del - (uint32_t)e_del), 0);
+                active = jb.tlen > 0;
+                if (!active) {
+                    J.end = kIdleEnd;
+                    if (g == 0) {
+                        bwg_ext_result r = {J.h0, 0, 0, 0, -1, 0};
+                        out[J.out_idx] = r;
+                    }
                 }
             }
         }
@@ -352,15 +366,15 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
 
         // ---- one DP row ------------------------------------------------------------------------
         const int32_t i = J.i;
-        // an idle group's values are not used (its window width below is 0)
+        // an idle group's window is negative (kIdleEnd) or empty (no job yet)
         const int32_t beg = max(J.beg, i - J.w);
         const int32_t end = min(min(J.end, i + J.w + 1), J.qlen);
-        // Block width of this row: the widest live window among the wavefront's four groups, in
-        // an odd number of columns per lane (at most K, since the bin's K covers the job's widest
+        // Block width of this row: the widest window among the wavefront's four groups, in an
+        // odd number of columns per lane (at most K, since the bin's K covers the job's widest
         // possible window).  The band narrows as rows go by, and the wavefront's time per row
-        // follows its widest group, so the row's columns are spread over every lane again.
-        const int32_t wwin = active ? max(end - beg, 0) : 0;
-        const int32_t wmax = L::wave_max(wwin);
+        // follows its widest group, so the row's columns are spread over every lane again.  A
+        // wavefront whose groups all have empty windows takes the narrowest body (kr <= 1).
+        const int32_t wmax = L::wave_max(end - beg);
         const int32_t kr = min(((wmax + GL - 1) / GL) | 1, K);
         if (!active) continue;
         // Every slot below the window's first column holds zero, which the row blocks rely on.
@@ -370,13 +384,14 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
         // slot (or the pad) otherwise.  The group's lanes all store the same value to the same
         // address, so the edge stores need no exec mask.
         eh.init(beg - 1, 0);
-        const int32_t h1_init = beg == 0 ? max(J.h1base, 0) : 0;
+        const int32_t h1_init = beg == 0 ? (int32_t)J.h1c : 0;
         const int8_t *srow = J.tcur < 4 ? (const int8_t *)plane03 + J.tcur : plane4;
         // This row's target base (requested a row ago) is consumed before the next one is
         // requested: the vector-memory counter is in order, so a wait placed after the new request
-        // would wait for it too, a full memory round trip every row.
+        // would wait for it too, a full memory round trip every row.  The next row's base is
+        // loaded into the same variable, so no copy at the row's end waits for it.
         asm volatile("" ::"v"(J.tcur) : "memory");
-        const int32_t tnext = J.t[min(i + 1, J.tlen - 1)];  // prefetched for the next row
+        J.tcur = J.t[min(i + 1, J.tlast)];
         int32_t key = row_cells_at<1, K, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
         eh.set_h(beg, h1_init);  // H(i, beg-1)
         eh.set_e(end, 0);        // E at the sentinel column
@@ -394,11 +409,14 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
             J.gbest_i = i;
             J.gscore = h1_last;
         }
-        // a row whose maximum is 0 ends the extension; a greater maximum moves the best cell
-        // (selects, not a branch: it is taken on most rows); a lower one may trigger z-drop
-        bool done = row_max == 0;
+        // a row whose maximum is 0 ends the extension (an empty window's key is below 1 << 13);
+        // a greater maximum moves the best cell (selects, not a branch: it is taken on most rows);
+        // a lower one may trigger z-drop.  |mj - i| as one sum of absolute differences: mj < 8192
+        // and i < 65536 fit its 16-bit operands.
+        bool done = key < (1 << kKeyShift);
         const bool improve = row_max > J.best;
-        J.max_off = improve ? max(J.max_off, abs(mj - i)) : J.max_off;
+        const int32_t off = (int32_t)__builtin_amdgcn_sad_u16((uint32_t)mj, (uint32_t)i, 0u);
+        J.max_off = improve ? max(J.max_off, off) : J.max_off;
         J.best_i = improve ? i : J.best_i;
         J.best_j = improve ? mj : J.best_j;
         J.best = improve ? row_max : J.best;
@@ -443,11 +461,10 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
                 }
             }
             J.beg = nb;
-            J.end = ne + 2 < J.qlen ? ne + 2 : J.qlen;
+            J.end = ne + 2;  // the next row's window clamps it to qlen
             J.i = i + 1;
-            J.tcur = tnext;
-            J.h1base = (int32_t)((uint32_t)J.h1base - (uint32_t)e_del);
-            if (J.i >= J.tlen) done = true;
+            J.h1c = __builtin_elementwise_sub_sat(J.h1c, (uint32_t)e_del);
+            if (i >= J.tlast) done = true;
         }
         if (done) {
             if (g == 0) {
