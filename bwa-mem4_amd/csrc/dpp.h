@@ -82,6 +82,8 @@ struct GroupLanes<16> {
     __device__ static __forceinline__ int allmin(int x) { return row_allmin(x); }
     __device__ static __forceinline__ int scan_max_nonneg(int x) { return row_scan_max_nonneg(x); }
     __device__ static __forceinline__ int shift_in(int x, int fill) { return row_shift_in(x, fill); }
+    // value of the previous lane of the group; the group's first lane receives 0
+    __device__ static __forceinline__ int shift_in_zero(int x) { return dpp<kRowShr + 1, true>(0, x); }
     __device__ static __forceinline__ int wave_max(int x) { return wave_max_of_rows(x); }
 };
 
@@ -111,6 +113,7 @@ struct GroupLanes<8> {
         return x;
     }
     __device__ static __forceinline__ int shift_in(int x, int fill) { return dpp<kRowShr + 2, false>(fill, x); }
+    __device__ static __forceinline__ int shift_in_zero(int x) { return dpp<kRowShr + 2, true>(0, x); }
     // the row's two groups first (a rotation by one pairs each lane with the other group), then
     // the rows
     __device__ static __forceinline__ int wave_max(int x) {

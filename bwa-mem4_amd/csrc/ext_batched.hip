@@ -97,7 +97,7 @@ __device__ __forceinline__ void fill_subst_table(int8_t *tbl, const ExtScoring &
 template <int GL> constexpr int groups_per_half() { return 32 / GL; }
 
 // The column state of one group, the reference's eh arrays over absolute columns 0..qcap+1, below
-// which K zero columns pad the left end of the row blocks.  Wide: H and E as int32, each in its
+// which ext_pad(K) zero columns pad the left end of the row blocks.  Wide: H and E as int32, each in its
 // own interleaved array.  Narrow: one word per column, H in its low and E in its high 16 bits,
 // exact whenever H and E stay below 65536; they never exceed h0 + qlen*max_sc, which the planner
 // checks for narrow bins.  Narrow state halves the group's LDS, which is what limits the kernel's
@@ -149,17 +149,20 @@ template <int NG> struct EhView<true, NG> {
 
 // Block LDS layout, in 4-byte words, for G = 256/GL groups in 8 half-wavefronts of NG groups:
 //  * eh: G * eh_words(qcap, K) words; half-wavefront hw owns NG * eh_words words from
-//    hw * NG * eh_words, interleaved as above, column j of group q at word NG*(K + j) + q (the
+//    hw * NG * eh_words, interleaved as above, column j of group q at word NG*(ext_pad(K) + j) + q (the
 //    wide form's E array follows its H array);
 //  * score planes 0-3: G * qcap words; half-wavefront hw owns NG * qcap words from hw * NG * qcap,
 //    word NG*j + q holding mat[c][query[j]] in byte c for target codes c = 0..3;
 //  * score plane 4 (target N): G * qcap bytes; the half-wavefronts hw and hw + 1, hw + 2, hw + 3
 //    (hw a multiple of 4) share NG * qcap words, word NG*j + q, byte hw & 3, so that every plane has
 //    the same column stride of 4*NG bytes.
-// A row touches columns from end - KR >= -K up to end <= qlen: the eh pad covers the columns below 0,
+// A row touches columns from -ext_pad(K) up to end <= qlen: the eh pad covers the columns below 0,
 // and the score bytes read there (multiplied by a zero H) come from the region before the plane's.
+// Zero columns below column 0: a multiple of 16 and at least K + 15, so that a row block moved
+// below the window keeps its column's residue modulo 16 (its LDS banks) and stays below column 0.
+__host__ __device__ constexpr int ext_pad(int K) { return (K + 30) / 16 * 16; }
 __host__ __device__ __forceinline__ int eh_words(int qcap, int K, bool narrow) {
-    return (narrow ? 1 : 2) * (K + qcap + 2);
+    return (narrow ? 1 : 2) * (ext_pad(K) + qcap + 2);
 }
 __host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int K, bool narrow) {
     const size_t groups = (size_t)(256 / GL);
@@ -168,39 +171,38 @@ __host__ __device__ __forceinline__ size_t block_lds_bytes(int GL, int qcap, int
 
 // The cells of one DP row for one group, with the row's window [beg, end) spread over the group's
 // lanes in blocks of KR columns aligned on its right end: lane g owns the columns base .. base +
-// KR - 1, base = end - (GL - g)*KR, and takes part when its block reaches into the window (the
-// group's KR*GL columns cover it).  Only the first such lane's block can reach left of the window,
-// by fewer than KR columns, onto columns below beg.  Every slot below beg holds zero (the caller
-// keeps it so, and pads below column 0), and on zero inputs a column computes zero H and E and
-// passes F = 0 on: those columns load zeros, store zeros back, and hand the window's first column
-// the F = 0 it starts from, so no column store needs a predicate.  Columns at or past end are never
-// touched: they keep their stale values, which a later row reads when the band widens again, as
-// the reference's do.  Lanes whose block lies left of the window compute the last block's columns
-// again, store nothing and feed 0 to the F scan, whose identity it is.  `srow` is the row's score
-// plane, column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns the lane's
-// row-maximum key (h << 13 | absolute column).  A column left of the window has h = 0, so it cannot
-// win a row whose maximum is positive.  A lane left of the window returns the key of the last
-// block's columns computed with an incoming F <= 0; the last lane computes the same columns from
-// the same loads with its true incoming F, which is never lower, so its key is never below the
-// dead lane's.  When the window is empty every column is below beg and every h is 0, so the key
-// is below 1 << 13 (negative where the columns are below 0).
-template <int KR, int GL, class EH>
-__device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t beg, int32_t end, int g,
-                                             int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
+// KR - 1, base = end - (GL - g)*KR (the group's KR*GL columns cover the window).  Every slot below
+// beg holds zero (the caller keeps it so, and pads PAD >= KR columns below column 0), and on zero
+// inputs a column computes zero H and E and passes F = 0 on.  So the first block in the window
+// may reach left of it onto zero columns, and a lane whose block lies wholly left of the window
+// computes and stores zeros over zero slots (a block that would start below the pad is moved up
+// into it, keeping its residue modulo 16, so the lanes' blocks still start at distinct banks):
+// no column load or store needs a predicate, and such a block hands F = 0 on.  Columns at or past end are never touched: they keep their stale values, which a later row
+// reads when the band widens again, as the reference's do.  `srow` is the row's score plane,
+// column j at byte 4*NG*j (NG = 32/GL groups per half-wavefront).  Returns the lane's row-maximum
+// key (h << 13 | absolute column).  A column left of the window has h = 0, so it cannot win a row
+// whose maximum is positive; when the window is empty every column is below beg and every h is 0,
+// so the key is below 1 << 13 (negative where the columns are below 0).
+template <int KR, int GL, int PAD, class EH>
+__device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, int32_t end, int g, int32_t e_ins,
+                                             int32_t oe_ins, int32_t e_del, int32_t oe_del) {
     using L = GroupLanes<GL>;
-    const int32_t rgap = (GL - 1 - g) * KR;  // columns between this lane's block and the window's end
-    const bool live = rgap < end - beg;
-    const int32_t base = end - KR - (live ? rgap : 0);
+    static_assert(PAD % 16 == 0 && PAD >= KR + 15, "a moved block must keep its banks and stay below 0");
+    // the block's first column; a lane whose block would start below the pad's last 16 columns
+    // takes the column of the same residue modulo 16 there, -PAD + (nat & 15), with one and-or
+    // (such a lane's block lies left of the window, a live lane's starts above -KR)
+    const int32_t nat = end - (GL - g) * KR;
+    const int32_t base = max(nat, (nat & 15) | -PAD);
     constexpr int kSStride = 4 * groups_per_half<GL>();
     const int8_t *sb = srow + base * kSStride;
-    // pass 1: the loads are independent, so they issue back to back
+    // pass 1: the loads are independent, so they issue back to back (the score bytes first: the
+    // column loads end in the 16-bit state's widening pins, which wait for them)
     int2 v[KR];
     int32_t s[KR];
 #pragma unroll
-    for (int k = 0; k < KR; ++k) {
-        v[k] = eh.load(base + k);
-        s[k] = sb[k * kSStride];
-    }
+    for (int k = 0; k < KR; ++k) s[k] = sb[k * kSStride];
+#pragma unroll
+    for (int k = 0; k < KR; ++k) v[k] = eh.load(base + k);
     // F from 0 at the block's first column, carried with its drift: G_k = F_k + k*e_ins, so
     // F_{k+1} = max(F_k - e_ins, M_k - oe_ins, 0) becomes G_{k+1} = max(G_k, M_k + (k+1)*e_ins -
     // oe_ins, (k+1)*e_ins), one add and one three-way max per column
@@ -218,14 +220,13 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
         grun = max(max(grun, m + ((k + 1) * e_ins - oe_ins)), (k + 1) * e_ins);
     }
     grun -= KR * e_ins;
-    // F entering this lane's block: exclusive max-scan of U_l = c_l + l*KR*e_ins over the row (a
-    // lane left of the window feeds 0; the first lane in the window then starts from F <= 0, which
-    // the F floor in gg turns into the F = 0 a window starts from)
+    // F entering lane l's block: F(l) = max_{m<l} c_m - (l-1-m)*KR*e_ins over the blocks' exit
+    // values c_m >= 0 (0 for a block of zero columns), as an exclusive max-scan of U_l = c_l +
+    // (l+1)*KR*e_ins (>= 0, so the scan's zero fill is its identity), F(l) = scan(l) - l*KR*e_ins.
+    // The first lane gets 0, and a term from a zero block is <= 0, which the F floor in gg absorbs.
     const int32_t Ke = KR * e_ins;
-    const int32_t U = live ? grun + g * Ke : 0;
-    const int32_t incl = L::scan_max_nonneg(U);
-    const int32_t excl = L::shift_in(incl, -Ke);
-    const int32_t f_in = excl - (g - 1) * Ke;
+    const int32_t incl = L::scan_max_nonneg(grun + (g + 1) * Ke);
+    const int32_t f_in = L::shift_in_zero(incl) - g * Ke;
 
     int32_t key = 0;
     int32_t hh[KR], ee[KR];
@@ -238,29 +239,25 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
         // lane-relative column; the lane's first column is added once below
         key = k == 0 ? h << kKeyShift : max(key, (h << kKeyShift) | k);
     }
-    if (live) {
 #pragma unroll
-        for (int k = 0; k < KR; ++k) {
-            eh.set_e(base + k, ee[k]);      // E(i+1, j)
-            eh.set_h(base + k + 1, hh[k]);  // H(i, j): the diagonal of column j+1 next row
-        }
+    for (int k = 0; k < KR; ++k) {
+        eh.set_e(base + k, ee[k]);      // E(i+1, j)
+        eh.set_h(base + k + 1, hh[k]);  // H(i, j): the diagonal of column j+1 next row
     }
     return key + base;
 }
 
 // row_cells at the odd block width kr (lo <= kr <= hi, both odd), chosen by a binary search
 // over the wavefront-uniform kr so that each row runs one straight-line body.
-template <int LO, int HI, int GL, class EH>
-__device__ __forceinline__ int32_t row_cells_at(int kr, const EH &eh, const int8_t *srow, int32_t beg, int32_t end,
-                                                int g, int32_t e_ins, int32_t oe_ins, int32_t e_del,
-                                                int32_t oe_del) {
+template <int LO, int HI, int GL, int PAD, class EH>
+__device__ __forceinline__ int32_t row_cells_at(int kr, const EH &eh, const int8_t *srow, int32_t end, int g,
+                                                int32_t e_ins, int32_t oe_ins, int32_t e_del, int32_t oe_del) {
     if constexpr (LO >= HI) {
-        return row_cells<HI, GL>(eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells<HI, GL, PAD>(eh, srow, end, g, e_ins, oe_ins, e_del, oe_del);
     } else {
         constexpr int MID = LO + ((HI - LO) / 4) * 2;  // odd, LO <= MID < HI
-        if (kr <= MID)
-            return row_cells_at<LO, MID, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
-        return row_cells_at<MID + 2, HI, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        if (kr <= MID) return row_cells_at<LO, MID, GL, PAD>(kr, eh, srow, end, g, e_ins, oe_ins, e_del, oe_del);
+        return row_cells_at<MID + 2, HI, GL, PAD>(kr, eh, srow, end, g, e_ins, oe_ins, e_del, oe_del);
     }
 }
 
@@ -301,7 +298,8 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
     const int hw = gid / NG, q = gid % NG;
     int32_t *const lw = (int32_t *)lds;
     const int ehw = eh_words(qcap, K, NARROW);
-    const EhView<NARROW, NG> eh(lw + hw * NG * ehw + NG * K + q, qcap, K);
+    constexpr int PAD = ext_pad(K);
+    const EhView<NARROW, NG> eh(lw + hw * NG * ehw + NG * PAD + q, qcap, PAD);
     int32_t *const plane03 = lw + G * ehw + hw * NG * qcap + q;  // word of column 0, byte c = code c
     int8_t *const plane4 = (int8_t *)(lw + G * ehw + G * qcap + (hw >> 2) * NG * qcap + q) + (hw & 3);
 
@@ -329,7 +327,7 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
                 J.w = clamp_band(sc.w, J.qlen, sc);
                 // row -1 (sw.rs:263-279): H[0] = h0, H[j] = max(h0 - oe_ins - (j-1)*e_ins, 0)
                 // (and the zero pad below column 0, which the row blocks read left of the window)
-                for (int j = g - K; j <= J.qlen + 1; j += GL) {
+                for (int j = g - PAD; j <= J.qlen + 1; j += GL) {
                     const int32_t h = j == 0 ? J.h0 : max(J.h0 - oe_ins - (j - 1) * e_ins, 0);
                     eh.init(j, j >= 0 && j <= J.qlen ? h : 0);
                 }
@@ -392,7 +390,7 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
         // loaded into the same variable, so no copy at the row's end waits for it.
         asm volatile("" ::"v"(J.tcur) : "memory");
         J.tcur = J.t[min(i + 1, J.tlast)];
-        int32_t key = row_cells_at<1, K, GL>(kr, eh, srow, beg, end, g, e_ins, oe_ins, e_del, oe_del);
+        int32_t key = row_cells_at<1, K, GL, PAD>(kr, eh, srow, end, g, e_ins, oe_ins, e_del, oe_del);
         eh.set_h(beg, h1_init);  // H(i, beg-1)
         eh.set_e(end, 0);        // E at the sentinel column
         wave_lds_sync();
@@ -429,9 +427,10 @@ void ext_group_kernel(const uint8_t *__restrict__ arena,
         }
         if (!done) {
             // band tightening (sw.rs:464-480): first live column in [beg, end), last live column
-            // scanning down from end.  The edges are usually live, so test them first.
+            // scanning down from end.  The edges are usually live, so test them first.  A row that
+            // goes on has a positive cell, so beg < end here.
             int32_t nb = beg;
-            if (beg < end && (h1_init | e_beg) == 0) {
+            if ((h1_init | e_beg) == 0) {
                 for (;;) {
                     const int j = nb + g;
                     bool live = false;
