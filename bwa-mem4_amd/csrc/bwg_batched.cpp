@@ -339,10 +339,12 @@ std::vector<TuneCandidate> ext_candidates(int cls) {
         add(kExtGroup, kExtKs[0], 32);
         add(kExtThread, qmax <= 16 ? 16 : 32, 32);
     } else {
+        // 16-bit state before 32-bit state (twice the LDS a column): a near-tie between the two
+        // goes to the 16-bit mapping, one K up where that is the one that ties
         const int r = cls - kExtShortClasses;
         add(kExtGroup, kExtKs[r], 16);
-        add(kExtGroup, kExtKs[r], 32);
         if (r + 1 < kNumExtKs) add(kExtGroup, kExtKs[r + 1], 16);
+        add(kExtGroup, kExtKs[r], 32);
         // eight jobs a wavefront, 2K + 1 columns per lane cover the same window
         if (kExtKs[r] <= 15) add(kExtGroup8, 2 * kExtKs[r] + 1, 16);  // 16-bit state: fits LDS at qcap 512
     }
