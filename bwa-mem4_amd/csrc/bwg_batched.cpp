@@ -16,6 +16,7 @@
 // (queue reset, every stage's kernels and the fork/join dependencies) is replayed from a captured
 // hipGraph.
 #include <algorithm>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <numeric>
@@ -1047,7 +1048,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
             t_work += it.work;
         }
         const double ns_per_cell = t_work > 0 ? t_ns / t_work : 0.0;
-        std::vector<double> wexp(nitems);
+        std::vector<double> wexp(nitems), shares(nitems, -1.0);
         for (int k = 0; k < nitems; ++k) {
             const Item &it = p->items[i + k];
             double share = 1.0;
@@ -1059,6 +1060,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
                     const float v = c->dp_share[item_code(it)];
                     if (v >= 0) share = v;
                 }
+                if (it.dp_share >= 0 || share < 1.0) shares[k] = share;
             }
             wexp[k] = (!timed_stage ? it.work : is_ext_dp(it.kind) ? it.est_ns : it.work * ns_per_cell) * share;
         }
@@ -1075,18 +1077,44 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
                 if (e == hipSuccess) e = hipEventCreate(&p->ev1[k]);
             }
         }
+        // A one-job-per-lane bin draining a compacted queue holds few of its nominal jobs: its grid
+        // is sized for the declined share measured on an earlier run (the persistent loop takes any
+        // surplus from the queue counter), so that its one-wave blocks do not hold CU slots the
+        // long bins' grids use.  Such a bin lasts as long as its longest lanes' jobs whatever its
+        // share of the GPU, so it is not dealt by its (small) work: it goes after the stage's
+        // largest bin, on that bin's stream.  That bin's blocks fill the CUs while it runs and it
+        // ends well before the bins that share the GPU with it; dealt by work, the short bin went
+        // last onto some other stream and ran alone at the batch's end.
+        auto latency_bin = [&](size_t r) { return par && p->compact && p->items[r].kind == kExtThread; };
+        int first_slot = -1;
+        std::vector<size_t> deferred;
+        auto issue = [&](size_t r, int slot) {
+            hipStream_t st = (!par || slot == 0) ? ss.main : ss.aux[slot - 1];
+            Item it = p->items[r];
+            if (p->compact && it.kind == kExtThread && shares[r - i] >= 0) {
+                const double waves = std::ceil((double)it.count * shares[r - i] * 1.5 / 64.0) + 16.0;
+                it.grid = (int)std::max(1.0, std::min((double)it.grid, waves));
+            }
+            if (timed) e = hipEventRecord(p->ev0[r], st);
+            if (e == hipSuccess) e = launch_item(b, p, it, st);
+            if (timed && e == hipSuccess) e = hipEventRecord(p->ev1[r], st);
+            ++nl;
+        };
         for (size_t r : ord) {
             if (e != hipSuccess) break;
+            if (latency_bin(r)) {
+                deferred.push_back(r);
+                continue;
+            }
             int slot = 0;
             for (int k = 1; k < std::min(b->tu.bin_streams, kAuxStreams + 1); ++k)
                 if (load[k] < load[slot]) slot = k;
             load[slot] += std::max(wexp[r - i], 1.0);
-            hipStream_t st = (!par || slot == 0) ? ss.main : ss.aux[slot - 1];
-            if (timed) e = hipEventRecord(p->ev0[r], st);
-            if (e == hipSuccess) e = launch_item(b, p, p->items[r], st);
-            if (timed && e == hipSuccess) e = hipEventRecord(p->ev1[r], st);
-            ++nl;
+            if (first_slot < 0) first_slot = slot;
+            issue(r, slot);
         }
+        for (size_t r : deferred)
+            if (e == hipSuccess) issue(r, first_slot < 0 ? 0 : first_slot);
         if (par) {
             for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) {
                 e = hipEventRecord(ss.join[k], ss.aux[k]);
