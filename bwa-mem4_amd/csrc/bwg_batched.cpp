@@ -37,6 +37,7 @@ struct Item {
     int stage = 0;     // items of one stage run concurrently; stages run in order
     uint32_t offset = 0, count = 0, head = 0;
     int grid = 1;
+    int launch_grid = 0;  // grid of the last launch sequence issued (emit may size it below grid)
     double work = 0;
     double est_ns = 0;     // extension DP bin: the autotuner's time per job of each job's class, summed
     bool est_all = false;  // every job of the bin has a tuned class (est_ns is its whole time alone)
@@ -1095,6 +1096,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
                 const double waves = std::ceil((double)it.count * shares[r - i] * 1.5 / 64.0) + 16.0;
                 it.grid = (int)std::max(1.0, std::min((double)it.grid, waves));
             }
+            p->items[r].launch_grid = it.grid;
             if (timed) e = hipEventRecord(p->ev0[r], st);
             if (e == hipSuccess) e = launch_item(b, p, it, st);
             if (timed && e == hipSuccess) e = hipEventRecord(p->ev1[r], st);
@@ -1373,7 +1375,7 @@ std::string batched_report(bwg_batch *b) {
                                   : 32;
             std::snprintf(line, sizeof line, "%d round=%d stage=%d %s jobs=%u grid=%d us=%.3f\n", idx,
                           p->retry_round, it.stage, mapping_name(it.kind, it.param, width, name, sizeof name),
-                          it.count, it.grid, us);
+                          it.count, it.launch_grid > 0 ? it.launch_grid : it.grid, us);
             s += line;
         }
     }

@@ -108,3 +108,50 @@ def test_global_one_full_bin(gpu_ctx):
     assert np.array_equal(got_s, want_s)
     assert np.array_equal(got_n, want_n.astype(np.uint32))
     assert np.array_equal(got_c, want_c)
+
+
+def test_short_query_bin_sized_from_another_batch_share():
+    # After the ungapped pre-pass the one-job-per-lane bin's grid is sized from the share of its
+    # jobs the pre-pass declined, learned on earlier runs (per context and mapping).  Batch A's
+    # short queries are exact copies of their targets' starts: the walk accepts them all and the
+    # context learns a share near 0.  Batch B's short queries sit against random targets, so the
+    # walk declines nearly all of them; B's first run is sized from A's share, with a grid of a few
+    # wavefronts for 100k declined jobs, and the persistent loop must take every other job from
+    # the queue counter.  Both batches' runs (eager, then the replayed graph sized from B's own
+    # share) equal the oracle.  Each batch also holds longer queries, so that its bins run
+    # concurrently (the short bin's grid is resized only then).
+    _say("short-query bin sized from another batch's declined share")
+    rng = np.random.default_rng(95)
+    mat = bwg.default_mat()
+    params = bwg.ext_params(5, mat, 6, 1, 6, 1, 100, 5, 100)
+    longq, longt = _pairs(4000, 120, 240, seed=96)
+
+    def build(n_short, exact):
+        qs, ts = list(longq), list(longt)
+        for _ in range(n_short):
+            q = rng.integers(0, 4, int(rng.integers(12, 33))).astype(np.uint8)
+            t = rng.integers(0, 4, len(q) + int(rng.integers(0, 40))).astype(np.uint8)
+            if exact:
+                t[:len(q)] = q
+            qs.append(q)
+            ts.append(t)
+        return bwg.JobBatch.from_pairs(qs, ts, h0=rng.integers(19, 120, len(qs)).astype(np.int32))
+
+    ctx = bwg.Context(0)
+    ctx.set_option(bwg.OPT_AUTOTUNE, 0)  # the default mapping: queries up to 32 columns one per lane
+    try:
+        for name, batch in (("accepted", build(60000, True)), ("declined", build(100000, False))):
+            want = ob.extend_batch(batch, mat, 6, 1, 6, 1, 100, 5, 100, nthreads=16)
+            st = ctx.stage(batch, params)
+            for run in range(3):
+                _say("  batch of %s short queries, run %d" % (name, run))
+                st.run()
+                got = st.results()
+                assert _first_bad(got, want) is None, (name, run, _first_bad(got, want))
+                if name == "declined" and run == 0:
+                    # the scenario under test: the short bin ran on a grid sized from batch A
+                    short = [r for r in st.report() if r["kernel"].startswith("ext_thread<")]
+                    assert short and max(r["grid"] for r in short) < 100, short
+            st.close()
+    finally:
+        ctx.close()
