@@ -188,9 +188,9 @@ __device__ __forceinline__ int32_t row_cells(const EH &eh, const int8_t *srow, i
                                              int32_t oe_ins, int32_t e_del, int32_t oe_del) {
     using L = GroupLanes<GL>;
     static_assert(PAD % 16 == 0 && PAD >= KR + 15, "a moved block must keep its banks and stay below 0");
-    // the block's first column; a lane whose block would start below the pad's last 16 columns
-    // takes the column of the same residue modulo 16 there, -PAD + (nat & 15), with one and-or
-    // (such a lane's block lies left of the window, a live lane's starts above -KR)
+    // the block's first column; a block that would start below the pad (nat < -PAD) takes the
+    // column of the same residue modulo 16 among the pad's lowest 16, -PAD + (nat & 15), with one
+    // and-or (such a block lies left of the window; a block reaching into it starts above -KR)
     const int32_t nat = end - (GL - g) * KR;
     const int32_t base = max(nat, (nat & 15) | -PAD);
     constexpr int kSStride = 4 * groups_per_half<GL>();
