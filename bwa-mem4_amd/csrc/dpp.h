@@ -34,6 +34,15 @@ __device__ __forceinline__ int row_allmin(int x) {
     return x;
 }
 
+// Sum over the 16 lanes of the row, result in every lane of the row.
+__device__ __forceinline__ int row_allsum(int x) {
+    x += dpp<kRowRor + 1, true>(0, x);
+    x += dpp<kRowRor + 2, true>(0, x);
+    x += dpp<kRowRor + 4, true>(0, x);
+    x += dpp<kRowRor + 8, true>(0, x);
+    return x;
+}
+
 // Inclusive prefix max over the row for non-negative values (lanes shifted in from outside the
 // row read 0, the identity for non-negative inputs).
 __device__ __forceinline__ int row_scan_max_nonneg(int x) {

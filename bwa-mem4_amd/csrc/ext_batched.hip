@@ -652,27 +652,38 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
         const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
         const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
         bool bail = !(qlen > 0 && h0 > 0 && tlen >= qlen);
-        int32_t nmis = 0;
-        uint32_t bits[4] = {0u, 0u, 0u, 0u};
         uint32_t qv[8], tv[8];
+        // the target's diagonal bytes g, g + 16, ... through a running pointer, one add a step
+        const uint8_t *tp = t.p + (ptrdiff_t)g * t.step;
+        const ptrdiff_t t16 = (ptrdiff_t)16 * t.step;
 #pragma unroll
         for (int st = 0; st < 8; ++st) {  // independent loads of the first 128 columns
             const int32_t c = 16 * st + g;
             const bool in = !bail && c < qlen;
             qv[st] = in ? q[c] : 0u;
-            tv[st] = in ? t[c] : 0u;
+            tv[st] = in ? *tp : 0u;
+            tp += t16;
         }
+        // Mismatches and N bases among the first 128 columns: each lane counts its own columns
+        // (the zeros past qlen match), the group sums the counts and looks for a code >= 4 once.
+        // Both tests are on totals, so the verdict is the one a column-by-column walk reaches.
+        // The mismatch columns themselves are needed only by a job that has some and passes.
+        uint32_t lmis = 0, lcodes = 0;
 #pragma unroll
         for (int st = 0; st < 8; ++st) {
-            const int32_t c = 16 * st + g;
-            const bool in = !bail && c < qlen;
-            const uint64_t bn = __ballot(in && (qv[st] >= 4 || tv[st] >= 4));
-            const uint64_t bm = __ballot(in && qv[st] != tv[st]);
-            const uint32_t gn = (uint32_t)(bn >> (lane & ~15)) & 0xffffu;
-            const uint32_t gm = (uint32_t)(bm >> (lane & ~15)) & 0xffffu;
-            nmis += __popc(gm);
-            if (gn || nmis > xmax) bail = true;
-            else bits[st >> 1] |= gm << (16 * (st & 1));
+            lmis += qv[st] != tv[st] ? 1u : 0u;
+            lcodes |= qv[st] | tv[st];
+        }
+        const int32_t nmis = row_allsum((int32_t)lmis);
+        const uint64_t bn = __ballot(lcodes >= 4u);
+        if (((uint32_t)(bn >> (lane & ~15)) & 0xffffu) || nmis > xmax) bail = true;
+        uint32_t bits[4] = {0u, 0u, 0u, 0u};  // mismatch columns < 128 (bail and nmis are the group's)
+        if (!bail && nmis > 0) {
+#pragma unroll
+            for (int st = 0; st < 8; ++st) {
+                const uint64_t bm = __ballot(qv[st] != tv[st]);
+                bits[st >> 1] |= ((uint32_t)(bm >> (lane & ~15)) & 0xffffu) << (16 * (st & 1));
+            }
         }
         // past column 128 only a mismatch-free walk can be accepted
         for (int32_t c0 = 128; c0 < qlen && !bail; c0 += 16) {
