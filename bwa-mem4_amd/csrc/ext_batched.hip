@@ -612,9 +612,10 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 //    job order, neighbouring stretches of the arena: every fetched line is used, where a walk in bin
 //    order touches a fresh line for every descriptor, query and target it reads.  Lane g reads
 //    columns g, g+16, ... of the query and of the target's diagonal only (16 consecutive bytes per
-//    step, the first 128 columns loaded up front); the group counts mismatches and N bases with
-//    ballots and keeps the mismatch columns (< 128) as a 128-bit mask.  An accepted job's result is
-//    written directly; every job's verdict goes to one byte of `declined`.
+//    step, the first 128 columns loaded up front); each lane counts the mismatches and N bases of
+//    its columns, the group sums them once, and a job that passes with mismatches rebuilds their
+//    columns (< 128) as a 128-bit mask from ballots.  An accepted job's result is written
+//    directly; every job's verdict goes to one byte of `declined`.
 //  * ext_compact_kernel walks the order array's contiguous slot chunks and appends the declined
 //    jobs, in slot order within each chunk, to their bins' compacted queues with one atomic per bin
 //    (the counters sit kCountStride words apart, on separate cache lines).  Slot ranges
