@@ -281,6 +281,9 @@ int ext_class_of(const ExtJob &j, const ExtScoring &sc, const Tuning &tu, int *k
     const int q = (int)j.qlen;
     const int64_t bound = (int64_t)j.h0 + (int64_t)q * std::max(sc.max_sc, 0);
     if (bound >= (1 << 18) || q >= 8192) return -1;
+    // The group kernels' F scan adds up to 16 blocks of 33 columns' insertion extensions to exit
+    // values below 2^18, in int32; a larger extension penalty goes to the plain kernel.
+    if ((int64_t)sc.e_ins * 16 * 33 + (1 << 18) > INT32_MAX) return -1;
     const int w = clamp_band(sc.w, q, sc);
     *kidx = odd_k_index((int)std::min<int64_t>(q, 2 * (int64_t)w + 1));
     if (q > 32 && (*kidx < 0 || qcap_class(q) > tu.ext_group_max_qcap)) return -1;

@@ -252,3 +252,26 @@ def test_ungapped_fast_path_on_and_off(gpu_ctx):
             assert hits == (int(hit.sum()) if on else 0)
     finally:
         gpu_ctx.set_option(bwg.OPT_UNGAPPED, 1)
+
+
+@pytest.mark.parametrize("form", FORMS, ids=FORM_IDS)
+def test_large_insertion_extension_penalty(gpu_ctx, form):
+    # e_ins = 5e6: the group kernels' F scan would carry 16 blocks of 33 columns of it past int32,
+    # so the batched form runs these jobs in the plain kernel; the gap bound (e_ins * (len + 1)
+    # below 2^30) keeps the jobs under 200 columns.  Every field equals the oracle's.
+    _say("large insertion extension penalty, %s" % FORM_IDS[FORMS.index(form)])
+    rng = np.random.default_rng(97)
+    qs, ts = [], []
+    for _ in range(3000):
+        q = rng.integers(0, 4, int(rng.integers(1, 150))).astype(np.uint8)
+        t = np.concatenate([q, rng.integers(0, 4, int(rng.integers(0, 40))).astype(np.uint8)])
+        if len(q) > 4:
+            t = np.delete(t, int(rng.integers(1, len(q) - 1)))           # a deletion
+            t = np.insert(t, int(rng.integers(1, len(q) - 1)), 2)       # an insertion
+        qs.append(q)
+        ts.append(t)
+    batch = bwg.JobBatch.from_pairs(qs, ts, rng.integers(19, 150, len(qs)).astype(np.int32))
+    mat = bwg.default_mat()
+    got = _ext(gpu_ctx, form, batch, mat, 6, 1, 6, 5_000_000, 100, 5, 100)
+    want = ob.extend_batch(batch, mat, 6, 1, 6, 5_000_000, 100, 5, 100)
+    assert np.array_equal(got, want)
