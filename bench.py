@@ -253,10 +253,11 @@ def _split_at_markers(rows, what, with_markers=False):
 
 def _launch_sequences(piece):
     """A segment's dispatches split into launch sequences: each starts with the zero_u32_kernel
-    that clears the queue heads."""
+    that clears the queue heads, or with the extension's ungapped pre-pass, which clears them
+    itself."""
     runs, cur = [], None
     for r in piece:
-        if r[2].startswith("zero_u32_kernel"):
+        if r[2].startswith("zero_u32_kernel") or r[2].startswith("ext_ungapped_kernel"):
             cur = []
             runs.append(cur)
         if cur is not None:
@@ -268,7 +269,8 @@ def summarize_trace(trace_csv, segments_json):
     """Markdown summary of a rocprofv3 kernel trace (--kernel-trace --output-format csv) of a bench
     run made with --segments-out: the dispatches split at the marker copies into the labelled
     segments, per-kernel device times per segment, the device span of every timed step (the first
-    dispatch of a step is its counter-clearing zero_u32_kernel) with the roofline fraction that span
+    dispatch of a step clears the queue counters: the zero_u32_kernel, or the extension's ungapped
+    pre-pass) with the roofline fraction that span
     gives, and the serial per-bin device times with each bin's fraction recomputed from the trace
     beside the figure bench.py took with HIP events."""
     import csv

@@ -975,7 +975,7 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
         return launch_ext_ungapped(st, b->d_arena, b->d_ref, (const ExtJob *)b->d_jobs, p->d_order, it.count,
                                    p->d_heads + seg_word(p), p->d_heads + seg_word(p) + p->n_bins, p->n_bins,
                                    p->d_declined, p->d_corder, p->d_heads + p->n_bins, p->esc, p->ungapped_xmax,
-                                   (bwg_ext_result *)b->d_out, it.grid);
+                                   (bwg_ext_result *)b->d_out, it.grid, p->d_heads, (uint32_t)seg_word(p));
     case kExtPlain:
         return launch_ext_plain_scatter(st, b->d_arena, b->d_ref, (const ExtJob *)p->d_plain_jobs,
                                         p->d_order + it.offset, it.count, p->esc, (int2 *)p->d_plain_scratch,
@@ -1018,9 +1018,13 @@ hipError_t launch_item(bwg_batch *b, Plan *p, const Item &it, hipStream_t st) {
 // The launch sequence of a planned batch on a stream set: queue reset, then each stage's items,
 // concurrently over the streams when enabled.
 hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launches) {
-    // queue heads (and, with the ungapped pre-pass, the compacted queue lengths) start at zero
-    hipError_t e = launch_zero_u32(ss.main, p->d_heads, (uint32_t)(p->compact ? seg_word(p) : p->n_heads));
-    int64_t nl = 1;
+    // queue heads (and, with the ungapped pre-pass, the compacted queue lengths) start at zero:
+    // cleared by the pre-pass's first block where the sequence starts with it, else by a fill kernel
+    const bool prepass_first = !p->items.empty() && p->items[0].kind == kExtUngapped;
+    hipError_t e = prepass_first
+                       ? hipSuccess
+                       : launch_zero_u32(ss.main, p->d_heads, (uint32_t)(p->compact ? seg_word(p) : p->n_heads));
+    int64_t nl = prepass_first ? 0 : 1;
     size_t i = 0;
     while (i < p->items.size() && e == hipSuccess) {
         size_t j = i;

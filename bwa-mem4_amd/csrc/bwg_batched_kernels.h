@@ -38,11 +38,13 @@ hipError_t launch_ext_thread(hipStream_t st, int qcap, bool narrow, const uint8_
 // the declined jobs of the bins without skip set to corder[seg_off[b] + ccount[kCountStride*b]++], in
 // slot order within each block's chunk.  The order array is a permutation of the n_slots jobs.
 constexpr int kCountStride = 32;  // compacted-queue counters one 128-byte line apart
+// `clear` (n_clear words, the launch sequence's queue heads and counters) is zeroed by the walk's
+// first block, ahead of the compaction and the bins that use it.
 hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8_t *ref, const ExtJob *jobs,
                                const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
                                const uint32_t *skip, int n_bins, uint8_t *declined, uint32_t *corder,
                                uint32_t *ccount, const ExtScoring &sc, int32_t xmax, bwg_ext_result *out,
-                               int min_blocks);
+                               int min_blocks, uint32_t *clear = nullptr, uint32_t n_clear = 0);
 hipError_t launch_zero_u32(hipStream_t st, uint32_t *p, uint32_t n);
 hipError_t launch_fill_i32(hipStream_t st, int32_t *p, uint32_t n, int32_t v);
 // Band-retry acceptance between two rounds of a one-sided extension batch (the reference's

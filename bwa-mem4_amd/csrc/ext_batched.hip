@@ -636,7 +636,12 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
                                                            const ExtJob *__restrict__ jobs, uint32_t n_jobs,
                                                            int32_t a, int32_t b, int32_t xmax, int32_t zdrop,
                                                            bwg_ext_result *__restrict__ out,
-                                                           uint8_t *__restrict__ declined) {
+                                                           uint8_t *__restrict__ declined,
+                                                           uint32_t *__restrict__ clear, uint32_t n_clear) {
+    // the launch sequence's queue heads and counters, zeroed here instead of by a kernel of their
+    // own ahead of this one (nothing reads them before the compaction that follows)
+    if (blockIdx.x == 0)
+        for (uint32_t k = threadIdx.x; k < n_clear; k += blockDim.x) clear[k] = 0u;
     const int lane = threadIdx.x & 63;
     const int g = lane & 15;
     const uint32_t stride = gridDim.x * 16u;
@@ -973,13 +978,13 @@ hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8
                                const uint32_t *order, uint32_t n_slots, const uint32_t *seg_off,
                                const uint32_t *skip, int n_bins, uint8_t *declined, uint32_t *corder,
                                uint32_t *ccount, const ExtScoring &sc, int32_t xmax, bwg_ext_result *out,
-                               int min_blocks) {
-    if (n_slots == 0) return hipSuccess;
+                               int min_blocks, uint32_t *clear, uint32_t n_clear) {
+    if (n_slots == 0) return n_clear ? launch_zero_u32(st, clear, n_clear) : hipSuccess;
     if (n_bins > 255) return hipErrorInvalidValue;
     // the walk: a grid-stride loop of 16 jobs per block and step
     const uint32_t walk_blocks = std::min<uint32_t>((uint32_t)std::max(1, min_blocks), (n_slots + 15) / 16);
     hipLaunchKernelGGL(ext_ungapped_kernel, dim3(walk_blocks), dim3(256), 0, st, arena, ref, jobs, n_slots,
-                       (int32_t)sc.mat[0], -(int32_t)sc.mat[1], xmax, sc.zdrop, out, declined);
+                       (int32_t)sc.mat[0], -(int32_t)sc.mat[1], xmax, sc.zdrop, out, declined, clear, n_clear);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // the compaction: contiguous chunks of at most kUngappedChunkMax slots, at least min_blocks of them
