@@ -1109,9 +1109,12 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
             if (timed && e == hipSuccess) e = hipEventRecord(p->ev1[r], st);
             ++nl;
         };
+        // a stage of such bins alone is dealt by work like any other
+        bool others = false;
+        for (size_t r : ord) others |= !latency_bin(r);
         for (size_t r : ord) {
             if (e != hipSuccess) break;
-            if (latency_bin(r)) {
+            if (others && latency_bin(r)) {
                 deferred.push_back(r);
                 continue;
             }
@@ -1123,7 +1126,7 @@ hipError_t emit(bwg_ctx *c, bwg_batch *b, Plan *p, StreamSet &ss, int64_t *launc
             issue(r, slot);
         }
         for (size_t r : deferred)
-            if (e == hipSuccess) issue(r, first_slot < 0 ? 0 : first_slot);
+            if (e == hipSuccess) issue(r, first_slot);
         if (par) {
             for (int k = 0; k < kAuxStreams && e == hipSuccess; ++k) {
                 e = hipEventRecord(ss.join[k], ss.aux[k]);
