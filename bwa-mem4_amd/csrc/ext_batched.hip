@@ -622,7 +622,7 @@ __global__ __launch_bounds__(64) void ext_thread_kernel(const uint8_t *__restric
 //    [seg_off[b], seg_off[b+1]) belong to bin b; bins with skip[b] set (the plain bin) are left
 //    alone: that bin runs every one of its jobs from the order array and writes each result again,
 //    the same value the walk wrote for the ones it accepted.
-constexpr int kUngappedChunkMax = 512;
+constexpr int kUngappedChunkMax = 2048;
 constexpr int kUngappedBinsMax = 256;
 
 // Bin of `slot`, searching forward from `bin` (slots only grow).
@@ -756,30 +756,38 @@ __global__ __launch_bounds__(256) void ext_compact_kernel(const uint32_t *__rest
                                                           uint32_t *__restrict__ ccount) {
     __shared__ uint32_t s_bincnt[kUngappedBinsMax];  // declined jobs per bin (bins from the chunk's first)
     __shared__ uint32_t s_binpos[kUngappedBinsMax];  // their first compacted-queue position
+    __shared__ uint32_t s_seg[kUngappedBinsMax];   // the bins' first slots and skip flags, read
+    __shared__ uint32_t s_skip[kUngappedBinsMax];  // from LDS by every bin search below
     __shared__ uint32_t s_wave[4];
     const int lane = threadIdx.x & 63;
     const uint32_t lo = blockIdx.x * chunk;
     const uint32_t hi = min(n_slots, lo + chunk);
     const uint32_t n = hi > lo ? hi - lo : 0;
-    const int bin0 = bin_of_slot(lo, 0, seg_off, n_bins);
-    for (int k = threadIdx.x; k < kUngappedBinsMax; k += blockDim.x) s_bincnt[k] = 0;
+    for (int k = threadIdx.x; k < kUngappedBinsMax; k += blockDim.x) {
+        s_bincnt[k] = 0;
+        if (k < n_bins) {
+            s_seg[k] = seg_off[k];
+            s_skip[k] = skip[k];
+        }
+    }
     __syncthreads();
+    const int bin0 = bin_of_slot(lo, 0, s_seg, n_bins);
     // thread t owns the slots [t*per, t*per + per) of the chunk: at most kPer of them
     constexpr uint32_t kPer = (kUngappedChunkMax + 255) / 256;
     const uint32_t per = (n + blockDim.x - 1) / blockDim.x;
     const uint32_t t0 = min(n, threadIdx.x * per), t1 = min(n, t0 + per);
     uint32_t idx[kPer];
     uint32_t flags = 0, cnt = 0;
-    int bb = bin_of_slot(lo + t0, bin0, seg_off, n_bins);
+    int bb = bin_of_slot(lo + t0, bin0, s_seg, n_bins);
     const int first_bb = bb;
 #pragma unroll
     for (uint32_t u = 0; u < kPer; ++u) {
         const uint32_t k = t0 + u;
         idx[u] = 0;
         if (k < t1) {
-            bb = bin_of_slot(lo + k, bb, seg_off, n_bins);
+            bb = bin_of_slot(lo + k, bb, s_seg, n_bins);
             idx[u] = order[lo + k];
-            if (!skip[bb] && declined[idx[u]]) {
+            if (!s_skip[bb] && declined[idx[u]]) {
                 flags |= 1u << u;
                 ++cnt;
                 atomicAdd(&s_bincnt[bb - bin0], 1u);
@@ -788,10 +796,10 @@ __global__ __launch_bounds__(256) void ext_compact_kernel(const uint32_t *__rest
     }
     __syncthreads();
     // one reservation per bin of the chunk
-    const int nb = n ? bin_of_slot(hi - 1, bin0, seg_off, n_bins) - bin0 + 1 : 0;
+    const int nb = n ? bin_of_slot(hi - 1, bin0, s_seg, n_bins) - bin0 + 1 : 0;
     for (int k = threadIdx.x; k < nb; k += blockDim.x) {
         const uint32_t c = s_bincnt[k];
-        s_binpos[k] = c ? seg_off[bin0 + k] + atomicAdd(&ccount[kCountStride * (bin0 + k)], c) : 0u;
+        s_binpos[k] = c ? s_seg[bin0 + k] + atomicAdd(&ccount[kCountStride * (bin0 + k)], c) : 0u;
     }
     // exclusive scan of the declined counts in slot order
     uint32_t incl = cnt;  // inclusive scan over the wavefront
@@ -813,7 +821,7 @@ __global__ __launch_bounds__(256) void ext_compact_kernel(const uint32_t *__rest
         for (uint32_t u = 0; u < kPer; ++u) {
             const uint32_t k = t0 + u;
             if (k < t1) {
-                const int nbb = bin_of_slot(lo + k, bb, seg_off, n_bins);
+                const int nbb = bin_of_slot(lo + k, bb, s_seg, n_bins);
                 for (; bb < nbb; ++bb) bin_first += s_bincnt[bb - bin0];
                 if (flags >> u & 1u) {
                     corder[s_binpos[bb - bin0] + (before - bin_first)] = idx[u];
@@ -987,8 +995,8 @@ hipError_t launch_ext_ungapped(hipStream_t st, const uint8_t *arena, const uint8
                        (int32_t)sc.mat[0], -(int32_t)sc.mat[1], xmax, sc.zdrop, out, declined, clear, n_clear);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // the compaction: contiguous chunks of at most kUngappedChunkMax slots, at least min_blocks of them
-    uint32_t blocks = (uint32_t)std::max(1, min_blocks);
+    // the compaction: contiguous chunks of at most kUngappedChunkMax slots, about min_blocks / 4 of them
+    uint32_t blocks = (uint32_t)std::max(1, min_blocks / 4);
     uint32_t chunk = (n_slots + blocks - 1) / blocks;
     chunk = std::min<uint32_t>((chunk + 15) / 16 * 16, kUngappedChunkMax);
     blocks = (n_slots + chunk - 1) / chunk;
