@@ -646,14 +646,10 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
     const int g = lane & 15;
     const uint32_t stride = gridDim.x * 16u;
     static_assert(sizeof(ExtJob) == 32, "a descriptor is two 16-byte words");
-    uint32_t idx = blockIdx.x * 16u + (threadIdx.x >> 4);
-    ExtJob next{};
-    if (idx < n_jobs) next = jobs[idx];
-    // (the steps compare n_jobs - idx with the stride: no 32-bit index wraps)
-    for (bool more = idx < n_jobs; more; more = n_jobs - idx > stride, idx += stride) {
-        // the next job's descriptor is in flight while this job's bytes load
-        const ExtJob jb = next;
-        if (n_jobs - idx > stride) next = jobs[idx + stride];
+    // One job's walk; the loop below alternates two descriptor buffers, so that the next job's
+    // descriptor is in flight while this job's bytes load and no register copies move it between
+    // iterations.
+    auto walk = [&](const ExtJob &jb, const uint32_t idx) {
         const uint8_t *q = arena + jb.q_off;
         const TargetView t = target_view(arena, ref, jb.t_off, jb.tlen);
         const int32_t qlen = (int32_t)jb.qlen, tlen = (int32_t)jb.tlen, h0 = jb.h0;
@@ -745,6 +741,22 @@ __global__ __launch_bounds__(256) void ext_ungapped_kernel(const uint8_t *__rest
             if (!bail) out[idx] = r;
             declined[idx] = bail ? 1 : 0;
         }
+    };
+    uint32_t idx = blockIdx.x * 16u + (threadIdx.x >> 4);
+    ExtJob ja{}, jb{};
+    if (idx < n_jobs) ja = jobs[idx];
+    // (the steps compare n_jobs - idx with the stride: no 32-bit index wraps)
+    while (idx < n_jobs) {
+        const bool more_a = n_jobs - idx > stride;
+        if (more_a) jb = jobs[idx + stride];
+        walk(ja, idx);
+        if (!more_a) break;
+        idx += stride;
+        const bool more_b = n_jobs - idx > stride;
+        if (more_b) ja = jobs[idx + stride];
+        walk(jb, idx);
+        if (!more_b) break;
+        idx += stride;
     }
 }
 
